@@ -1,0 +1,173 @@
+/*
+ * rsgpu.h -- C ABI of the MI355X (gfx950) Reed-Solomon data path.
+ *
+ * A drop-in GPU backend for the data path of rustfs-erasure-codec 9.1.0:
+ * classic galois_8 / galois_16 ReedSolomon and Leopard GF8 / GF16 encode,
+ * verify, reconstruct, reconstruct_data, reconstruct_some, and the block-stream
+ * wrappers.  Each entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Shard buffers passed to the non-"_host" calls are DEVICE pointers; the
+ *     work is enqueued on `stream` (a hipStream_t, NULL = default stream) and
+ *     the call returns once it is enqueued.  The library keeps no pointer past
+ *     the completion of that work.
+ *   - Every call validates its arguments completely before enqueueing any
+ *     device work, in the reference's order (macros.rs:142-245 check_piece_count!
+ *     then check_slices!; recovery_plan.rs:262-387 for reconstruction), so a
+ *     validation error never modifies an output buffer.
+ *   - Return values: 0 = Ok, 1..16 = the reference's Error variants in
+ *     declaration order (src/errors.rs:4-21), >= 100 = device/runtime errors.
+ *   - A codec handle is safe to share between host threads; its decode cache is
+ *     guarded by a mutex (the reference keeps it behind a mutex too,
+ *     src/core/inversion_cache.rs:38-48).
+ */
+#ifndef RSGPU_H
+#define RSGPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: src/errors.rs:4-21 ---- */
+enum {
+  RSGPU_OK = 0,
+  RSGPU_TOO_FEW_SHARDS = 1,
+  RSGPU_TOO_MANY_SHARDS = 2,
+  RSGPU_TOO_FEW_DATA_SHARDS = 3,
+  RSGPU_TOO_MANY_DATA_SHARDS = 4,
+  RSGPU_TOO_FEW_PARITY_SHARDS = 5,
+  RSGPU_TOO_MANY_PARITY_SHARDS = 6,
+  RSGPU_TOO_FEW_BUFFER_SHARDS = 7,
+  RSGPU_TOO_MANY_BUFFER_SHARDS = 8,
+  RSGPU_INCORRECT_SHARD_SIZE = 9,
+  RSGPU_TOO_FEW_SHARDS_PRESENT = 10,
+  RSGPU_EMPTY_SHARD = 11,
+  RSGPU_INVALID_SHARD_FLAGS = 12,
+  RSGPU_INVALID_INDEX = 13,
+  RSGPU_INVALID_CUSTOM_MATRIX = 14,
+  RSGPU_UNSUPPORTED_CODEC_FAMILY = 15,
+  RSGPU_UNSUPPORTED_LEOPARD_PROTOTYPE = 16,
+  RSGPU_ERR_HIP = 100,             /* a HIP runtime call failed */
+  RSGPU_ERR_INVALID_ARGUMENT = 101, /* NULL handle/pointer or bad enum */
+  RSGPU_ERR_NO_DEVICE = 102,
+  RSGPU_ERR_UNSUPPORTED = 103,      /* shape beyond this build's device limits */
+  RSGPU_ERR_OUT_OF_MEMORY = 104,
+};
+
+/* ---- codec construction: src/core/options.rs:1-147, src/core/codec.rs:264-615 ---- */
+enum { RSGPU_FIELD_GF8 = 0, RSGPU_FIELD_GF16 = 1 };                /* galois_8 / galois_16 */
+enum { RSGPU_FAMILY_CLASSIC = 0, RSGPU_FAMILY_LEOPARD_GF8 = 1, RSGPU_FAMILY_LEOPARD_GF16 = 2 };
+enum { RSGPU_MATRIX_VANDERMONDE = 0, RSGPU_MATRIX_CAUCHY = 1, RSGPU_MATRIX_JERASURE_LIKE = 2,
+       RSGPU_MATRIX_CUSTOM = 3 };
+enum { RSGPU_LEOPARD_DISABLED = 0, RSGPU_LEOPARD_AS_NEEDED = 1, RSGPU_LEOPARD_PREFER_GF16 = 2,
+       RSGPU_LEOPARD_PREFER_LEOPARD = 3 };
+
+/* CodecOptions (src/core/options.rs:99-147). */
+typedef struct rsgpu_options {
+  int32_t field;            /* RSGPU_FIELD_* (the ReedSolomon<F> type parameter) */
+  int32_t codec_family;     /* RSGPU_FAMILY_* */
+  int32_t matrix_mode;      /* RSGPU_MATRIX_* (CUSTOM only via rsgpu_codec_with_custom_matrix) */
+  int32_t leopard_mode;     /* RSGPU_LEOPARD_* */
+  int32_t fast_one_parity;  /* single all-ones parity row when parity == 1 */
+  int32_t inversion_cache;  /* cache decode matrices (device-resident) */
+  uint64_t inversion_cache_capacity;  /* 0 = derived (codec.rs:44-59) */
+  uint64_t inversion_cache_max_bytes; /* 0 = 8 MiB (inversion_cache.rs:11) */
+} rsgpu_options;
+
+typedef struct rsgpu_codec rsgpu_codec;
+
+/* CodecOptions::default() */
+void rsgpu_options_default(rsgpu_options* opts);
+/* ReedSolomon::new / with_options (src/core/codec.rs:264-355). */
+int rsgpu_codec_new(size_t data_shards, size_t parity_shards, const rsgpu_options* opts,
+                    rsgpu_codec** out);
+/* ReedSolomon::with_custom_matrix (src/core/codec.rs:546-615).  `rows` holds
+ * n_rows rows of `row_len` field elements (1 byte each for GF8, the byte pair
+ * [x, c] for GF16); only the first parity_shards rows and data_shards columns
+ * are used, as in build_custom_matrix (codec.rs:199-227). */
+int rsgpu_codec_with_custom_matrix(size_t data_shards, size_t parity_shards, const uint8_t* rows,
+                                   size_t n_rows, size_t row_len, const rsgpu_options* opts,
+                                   rsgpu_codec** out);
+void rsgpu_codec_free(rsgpu_codec* codec);
+/* data_shard_count / parity_shard_count / codec_family (codec.rs:357-380). */
+int rsgpu_codec_info(const rsgpu_codec* codec, size_t* data_shards, size_t* parity_shards,
+                     int32_t* field, int32_t* codec_family);
+/* The (k+m) x k encoding matrix, row-major, element bytes as in custom rows
+ * (EncodingFormat::matrix, src/core/format.rs). */
+int rsgpu_codec_matrix(const rsgpu_codec* codec, uint8_t* out, size_t out_len);
+
+/* ---- shard views ---- */
+typedef struct rsgpu_cslice { const uint8_t* ptr; size_t len; } rsgpu_cslice;
+typedef struct rsgpu_slice { uint8_t* ptr; size_t len; } rsgpu_slice;
+/* ShardSlot (src/lib.rs:250-304): a buffer kept across calls with a presence
+ * flag.  Missing slots that reconstruction targets must have len == shard
+ * length; their bytes are overwritten and `present` is set to 1. */
+typedef struct rsgpu_slot { uint8_t* ptr; size_t len; int32_t present; } rsgpu_slot;
+
+/* ---- encode: src/core/encode.rs:935-998 ---- */
+/* ReedSolomon::encode: shards[0..k] data (read), shards[k..k+m] parity (written). */
+int rsgpu_encode(const rsgpu_codec* codec, const rsgpu_slice* shards, size_t n_shards,
+                 void* stream);
+/* ReedSolomon::encode_sep */
+int rsgpu_encode_sep(const rsgpu_codec* codec, const rsgpu_cslice* data, size_t n_data,
+                     const rsgpu_slice* parity, size_t n_parity, void* stream);
+
+/* ---- verify: src/core/verify.rs:14-237 ---- */
+/* ReedSolomon::verify / verify_with_workspace: *ok = 1 when every parity shard
+ * equals the recomputed parity.  Synchronous (waits for the stream).  Classic
+ * codecs compare in registers and never write parity; Leopard codecs re-encode
+ * into library-owned scratch. */
+int rsgpu_verify(const rsgpu_codec* codec, const rsgpu_cslice* shards, size_t n_shards,
+                 int32_t* ok, void* stream);
+/* Asynchronous verify: writes 0 (match) or nonzero (mismatch) to the device
+ * int32 at `device_flag` on `stream`. */
+int rsgpu_verify_async(const rsgpu_codec* codec, const rsgpu_cslice* shards, size_t n_shards,
+                       int32_t* device_flag, void* stream);
+/* ReedSolomon::verify_with_buffer (verify.rs:98-118): parity is recomputed into
+ * `buffer` (m device slices) and compared. */
+int rsgpu_verify_with_buffer(const rsgpu_codec* codec, const rsgpu_cslice* shards,
+                             size_t n_shards, const rsgpu_slice* buffer, size_t n_buffer,
+                             int32_t* ok, void* stream);
+
+/* ---- reconstruct: src/core/reconstruct.rs:539-790 ---- */
+/* ReedSolomon::reconstruct (all missing shards). */
+int rsgpu_reconstruct(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards, void* stream);
+/* ReedSolomon::reconstruct_data (missing data shards only). */
+int rsgpu_reconstruct_data(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
+                           void* stream);
+/* ReedSolomon::reconstruct_some (required flags of length k or k+m). */
+int rsgpu_reconstruct_some(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
+                           const uint8_t* required, size_t n_required, void* stream);
+
+/* ---- host-pointer variants (copy through library-owned device staging) ---- */
+int rsgpu_encode_host(const rsgpu_codec* codec, uint8_t* const* shards, size_t n_shards,
+                      size_t shard_len);
+int rsgpu_verify_host(const rsgpu_codec* codec, const uint8_t* const* shards, size_t n_shards,
+                      size_t shard_len, int32_t* ok);
+int rsgpu_reconstruct_host(const rsgpu_codec* codec, uint8_t* const* shards, uint8_t* present,
+                           size_t n_shards, size_t shard_len, int32_t data_only);
+
+/* ---- the matrix-apply primitive (SelectedEncodeKernel::execute,
+ * src/galois_8/codegen.rs:46-79; code_some_slices encode.rs:327-602):
+ * out_j = XOR_i rows[j*k + i] * in_i over GF(2^8) (field 0) or the tower
+ * field (field 1, rows as [x, c] byte pairs). ---- */
+int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k,
+                     const rsgpu_cslice* in, const rsgpu_slice* out, void* stream);
+
+/* ---- decode-matrix cache statistics (ReconstructionCacheStats, codec.rs:401-426) ---- */
+int rsgpu_cache_stats(const rsgpu_codec* codec, uint64_t* requests, uint64_t* hits,
+                      uint64_t* misses, uint64_t* entries);
+
+/* ---- library ---- */
+const char* rsgpu_status_name(int status);
+/* Binds the calling thread to `device` and builds the per-device tables. */
+int rsgpu_init(int device);
+/* Last HIP error text of the calling thread (empty if none). */
+const char* rsgpu_last_hip_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSGPU_H */

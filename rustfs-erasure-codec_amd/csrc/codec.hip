@@ -1,0 +1,923 @@
+// codec.hip -- host side of the classic data path behind the C ABI: codec
+// construction (encoding matrices), argument validation in the reference's
+// order, reconstruction planning, the device-resident decode-row cache, and the
+// launch plumbing for the matrix-apply kernels.
+#include <algorithm>
+#include <cstring>
+
+#include "internal.h"
+#include "leopard.h"
+
+namespace rsgpu {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string t_hip_error;
+
+void set_hip_error(hipError_t e, const char* where) {
+  t_hip_error = std::string(where) + ": " + hipGetErrorString(e);
+}
+
+#define HIP_TRY(expr)                              \
+  do {                                             \
+    hipError_t _e = (expr);                        \
+    if (_e != hipSuccess) {                        \
+      set_hip_error(_e, #expr);                    \
+      return _e == hipErrorOutOfMemory ? RSGPU_ERR_OUT_OF_MEMORY : RSGPU_ERR_HIP; \
+    }                                              \
+  } while (0)
+
+DeviceBuffer::~DeviceBuffer() {
+  if (p) (void)hipFree(p);
+}
+hipError_t DeviceBuffer::alloc(size_t n) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+  if (n == 0) n = 16;
+  hipError_t e = hipMalloc(&p, n);
+  if (e == hipSuccess) bytes = n;
+  return e;
+}
+
+DecodeEntry::~DecodeEntry() {
+  if (last_use) (void)hipEventDestroy(last_use);
+}
+
+// ------------------------------------------------------- field abstraction
+struct Field {
+  bool tower;
+  uint16_t one() const { return 1; }
+  uint16_t mul(uint16_t a, uint16_t b) const { return tower ? gf16_mul(a, b) : gf8_mul((uint8_t)a, (uint8_t)b); }
+  uint16_t div(uint16_t a, uint16_t b) const { return tower ? gf16_div(a, b) : gf8_div((uint8_t)a, (uint8_t)b); }
+  uint16_t exp(uint16_t a, size_t n) const { return tower ? gf16_exp(a, n) : gf8_exp((uint8_t)a, n); }
+  uint16_t nth(size_t n) const { return tower ? (uint16_t)n : (uint16_t)(uint8_t)n; }
+  size_t order() const { return tower ? 65536 : 256; }
+};
+
+// Gauss-Jordan elimination with the first-nonzero-below pivot rule
+// (src/matrix.rs:373-422).  Returns false for a singular matrix.
+static bool gauss_jordan(const Field& F, std::vector<uint16_t>& w, int rows, int cols) {
+  for (int r = 0; r < rows; ++r) {
+    if (w[r * cols + r] == 0)
+      for (int rb = r + 1; rb < rows; ++rb)
+        if (w[rb * cols + r] != 0) {
+          for (int c = 0; c < cols; ++c) std::swap(w[r * cols + c], w[rb * cols + c]);
+          break;
+        }
+    if (w[r * cols + r] == 0) return false;
+    if (w[r * cols + r] != 1) {
+      uint16_t s = F.div(1, w[r * cols + r]);
+      for (int c = 0; c < cols; ++c) w[r * cols + c] = F.mul(s, w[r * cols + c]);
+    }
+    for (int rb = 0; rb < rows; ++rb) {
+      if (rb == r) continue;
+      uint16_t f = w[rb * cols + r];
+      if (f)
+        for (int c = 0; c < cols; ++c) w[rb * cols + c] ^= F.mul(f, w[r * cols + c]);
+    }
+  }
+  return true;
+}
+
+static bool invert(const Field& F, const std::vector<uint16_t>& a, int n, std::vector<uint16_t>& out) {
+  std::vector<uint16_t> w((size_t)n * 2 * n, 0);
+  for (int r = 0; r < n; ++r) {
+    for (int c = 0; c < n; ++c) w[(size_t)r * 2 * n + c] = a[(size_t)r * n + c];
+    w[(size_t)r * 2 * n + n + r] = 1;
+  }
+  if (!gauss_jordan(F, w, n, 2 * n)) return false;
+  out.assign((size_t)n * n, 0);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) out[(size_t)r * n + c] = w[(size_t)r * 2 * n + n + c];
+  return true;
+}
+
+static void vandermonde(const Field& F, int rows, int cols, std::vector<uint16_t>& v) {
+  v.assign((size_t)rows * cols, 0);
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) v[(size_t)r * cols + c] = F.exp(F.nth(r), c);
+}
+
+// Encoding matrices (src/core/codec.rs:67-258).
+static int build_matrix(const Field& F, int mode, bool fast_one, int k, int m,
+                        std::vector<uint16_t>& out) {
+  const int total = k + m;
+  out.assign((size_t)total * k, 0);
+  if (fast_one && m == 1) {  // codec.rs:240-250
+    for (int c = 0; c < k; ++c) out[(size_t)c * k + c] = out[(size_t)k * k + c] = 1;
+    return RSGPU_OK;
+  }
+  if (mode == RSGPU_MATRIX_VANDERMONDE) {  // V * inv(V_top), codec.rs:67-89
+    std::vector<uint16_t> v, top((size_t)k * k), inv;
+    vandermonde(F, total, k, v);
+    std::copy(v.begin(), v.begin() + (size_t)k * k, top.begin());
+    if (!invert(F, top, k, inv)) return RSGPU_INVALID_CUSTOM_MATRIX;
+    for (int r = 0; r < total; ++r)
+      for (int c = 0; c < k; ++c) {
+        uint16_t s = 0;
+        for (int i = 0; i < k; ++i) s ^= F.mul(v[(size_t)r * k + i], inv[(size_t)i * k + c]);
+        out[(size_t)r * k + c] = s;
+      }
+    return RSGPU_OK;
+  }
+  if (mode == RSGPU_MATRIX_CAUCHY) {  // codec.rs:91-112
+    for (int r = 0; r < total; ++r) {
+      if (r < k) {
+        out[(size_t)r * k + r] = 1;
+        continue;
+      }
+      for (int c = 0; c < k; ++c) {
+        uint16_t den = F.nth(r) ^ F.nth(c);
+        if (den == 0) return RSGPU_INVALID_CUSTOM_MATRIX;
+        out[(size_t)r * k + c] = F.div(1, den);
+      }
+    }
+    return RSGPU_OK;
+  }
+  if (mode == RSGPU_MATRIX_JERASURE_LIKE) {  // codec.rs:114-197
+    std::vector<uint16_t>& vm = out;
+    vandermonde(F, total, k, vm);
+    auto at = [&](int r, int c) -> uint16_t& { return vm[(size_t)r * k + c]; };
+    at(0, 0) = 1;
+    for (int i = 1; i < k; ++i) at(0, i) = 0;
+    for (int i = 0; i + 1 < k; ++i) at(total - 1, i) = 0;
+    at(total - 1, k - 1) = 1;
+    for (int i = 0; i < k; ++i) {
+      int r = i;
+      while (r < total && at(r, i) == 0) ++r;
+      if (r != i) {
+        if (r >= total) return RSGPU_INVALID_CUSTOM_MATRIX;
+        for (int c = 0; c < k; ++c) std::swap(at(r, c), at(i, c));
+      } else if (at(i, i) == 0) {
+        return RSGPU_INVALID_CUSTOM_MATRIX;
+      }
+      uint16_t diag = at(i, i);
+      if (diag == 0) return RSGPU_INVALID_CUSTOM_MATRIX;
+      uint16_t scale = diag != 1 ? F.div(1, diag) : 1;
+      if (scale != 1)
+        for (int row = 0; row < total; ++row) at(row, i) = F.mul(at(row, i), scale);
+      for (int j = 0; j < k; ++j) {
+        uint16_t value = at(i, j);
+        if (j != i && value != 0)
+          for (int row = 0; row < total; ++row) at(row, j) ^= F.mul(value, at(row, i));
+      }
+    }
+    for (int j = 0; j < k; ++j) {
+      uint16_t value = at(k, j);
+      if (value == 0) return RSGPU_INVALID_CUSTOM_MATRIX;
+      if (value != 1) {
+        uint16_t s = F.div(1, value);
+        for (int row = k; row < total; ++row) at(row, j) = F.mul(at(row, j), s);
+      }
+    }
+    for (int row = k + 1; row < total; ++row) {
+      uint16_t value = at(row, 0);
+      if (value == 0) return RSGPU_INVALID_CUSTOM_MATRIX;
+      if (value != 1) {
+        uint16_t s = F.div(1, value);
+        for (int col = 0; col < k; ++col) at(row, col) = F.mul(at(row, col), s);
+      }
+    }
+    return RSGPU_OK;
+  }
+  return RSGPU_INVALID_CUSTOM_MATRIX;
+}
+
+static size_t derive_cache_capacity(size_t k, size_t m) {  // codec.rs:44-59
+  size_t h = (k + m) * std::max<size_t>(m, 1) * 2, p = 1;
+  while (p < h) p <<= 1;
+  return std::min<size_t>(std::max<size_t>(p, 128), 4096);
+}
+
+static int upload_codec(Codec& c) {
+  HIP_TRY(hipGetDevice(&c.device));
+  if (c.family != RSGPU_FAMILY_CLASSIC) return init_leopard(c);
+  HIP_TRY(c.d_matrix.alloc(c.matrix.size() * sizeof(uint16_t)));
+  HIP_TRY(hipMemcpy(c.d_matrix.p, c.matrix.data(), c.matrix.size() * sizeof(uint16_t),
+                    hipMemcpyHostToDevice));
+  build_tables_host(c.matrix.data() + c.k * c.k, (int)c.m, (int)c.k, c.field == RSGPU_FIELD_GF16,
+                    c.h_parity_tables);
+  HIP_TRY(c.d_parity_tables.alloc(c.h_parity_tables.size() * 4));
+  HIP_TRY(hipMemcpy(c.d_parity_tables.p, c.h_parity_tables.data(), c.h_parity_tables.size() * 4,
+                    hipMemcpyHostToDevice));
+  return RSGPU_OK;
+}
+
+// ------------------------------------------------------------- apply plumbing
+struct Views {
+  std::vector<const uint8_t*> in;
+  std::vector<uint8_t*> out;
+};
+
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+// Apply rows (tables [r][k_total][W], row stride k_total*W) to inputs -> outputs.
+// len is in bytes.  Splits k into kMaxIn chunks (later chunks accumulate) and r
+// into kMaxOut chunks.  mode kVerify compares with `out` and sets *flag.
+static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
+                     const std::vector<const uint8_t*>& in, const std::vector<uint8_t*>& out,
+                     size_t len, int mode, int32_t* flag, hipStream_t s, int nv = 0) {
+  const int W = words_per_coef(tower);
+  bool vec_ok = true;
+  for (auto p : in) vec_ok &= aligned16(p);
+  for (auto p : out) vec_ok &= aligned16(p);
+  if (mode == kVerify && k_total > kMaxIn) return RSGPU_ERR_UNSUPPORTED;  // handled by caller
+  const uint64_t n16 = vec_ok ? len / 16 : 0;
+  const size_t tail_begin = (size_t)n16 * 16;
+  if (nv == 0) nv = 2;
+  for (int r0 = 0; r0 < r_total; r0 += kMaxOut) {
+    const int rn = std::min(kMaxOut, r_total - r0);
+    for (int k0 = 0; k0 < k_total; k0 += kMaxIn) {
+      const int kn = std::min(kMaxIn, k_total - k0);
+      const int m = (mode == kVerify) ? kVerify : (k0 == 0 ? kWrite : kAccumulate);
+      if (n16) {
+        ApplyArgs a{};
+        for (int i = 0; i < kn; ++i) a.in[i] = in[k0 + i];
+        for (int j = 0; j < rn; ++j) a.out[j] = out[r0 + j];
+        a.tables = d_tables + (size_t)r0 * k_total * W;
+        a.flag = flag;
+        a.n16 = n16;
+        a.v_begin = 0;
+        a.k = kn;
+        a.tab_row_stride = k_total * W;
+        a.tab_col0 = k0;
+        a.r = rn;
+        HIP_TRY(launch_apply(a, m, nv, tower ? 1 : 0, s));
+      }
+      if (tail_begin < len) {
+        ByteArgs b{};
+        for (int i = 0; i < kn; ++i) b.in[i] = in[k0 + i];
+        for (int j = 0; j < rn; ++j) b.out[j] = out[r0 + j];
+        b.tables = d_tables + (size_t)r0 * k_total * W;
+        b.flag = flag;
+        b.begin = tail_begin;
+        b.end = len;
+        b.k = kn;
+        b.tab_row_stride = k_total * W;
+        b.tab_col0 = k0;
+        b.r = rn;
+        b.mode = m;
+        b.tower = tower ? 1 : 0;
+        HIP_TRY(launch_apply_bytes(b, s));
+      }
+    }
+  }
+  return RSGPU_OK;
+}
+
+// ---------------------------------------------------------------- validation
+static int elem_bytes(const Codec& c) { return c.field == RSGPU_FIELD_GF16 ? 2 : 1; }
+
+// check_slices!(multi) (macros.rs:142-155) for a run of lengths.
+template <class S>
+static int check_slices(const S* v, size_t n) {
+  size_t size = v[0].len;
+  if (size == 0) return RSGPU_EMPTY_SHARD;
+  for (size_t i = 0; i < n; ++i)
+    if (v[i].len != size) return RSGPU_INCORRECT_SHARD_SIZE;
+  return RSGPU_OK;
+}
+
+static int check_elem_len(const Codec& c, size_t len) {
+  if (c.field == RSGPU_FIELD_GF16 && (len & 1)) return RSGPU_INCORRECT_SHARD_SIZE;
+  return RSGPU_OK;
+}
+
+static thread_local DeviceBuffer t_flag;
+static thread_local int t_flag_device = -1;
+
+static int32_t* thread_flag(int device) {
+  if (t_flag_device != device || !t_flag.p) {
+    if (t_flag.alloc(64) != hipSuccess) return nullptr;
+    t_flag_device = device;
+  }
+  return (int32_t*)t_flag.p;
+}
+
+// ------------------------------------------------------------ encode / verify
+static int encode_sep_impl(const Codec& c, const rsgpu_cslice* data, size_t nd,
+                           const rsgpu_slice* parity, size_t np, hipStream_t s) {
+  if (nd < c.k) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (nd > c.k) return RSGPU_TOO_MANY_DATA_SHARDS;
+  if (np < c.m) return RSGPU_TOO_FEW_PARITY_SHARDS;
+  if (np > c.m) return RSGPU_TOO_MANY_PARITY_SHARDS;
+  int rc = check_slices(data, nd);
+  if (rc) return rc;
+  if ((rc = check_slices(parity, np))) return rc;
+  if (data[0].len != parity[0].len) return RSGPU_INCORRECT_SHARD_SIZE;
+  if ((rc = check_elem_len(c, data[0].len))) return rc;
+  for (size_t i = 0; i < nd; ++i)
+    if (!data[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  for (size_t i = 0; i < np; ++i)
+    if (!parity[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (c.family != RSGPU_FAMILY_CLASSIC) return leopard_encode(c, data, parity, data[0].len, s);
+  std::vector<const uint8_t*> in(nd);
+  std::vector<uint8_t*> out(np);
+  for (size_t i = 0; i < nd; ++i) in[i] = data[i].ptr;
+  for (size_t i = 0; i < np; ++i) out[i] = parity[i].ptr;
+  return run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m,
+                   c.field == RSGPU_FIELD_GF16, in, out, data[0].len, kWrite, nullptr, s);
+}
+
+static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* d_flag,
+                       hipStream_t s) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  if ((rc = check_elem_len(c, shards[0].len))) return rc;
+  for (size_t i = 0; i < n; ++i)
+    if (!shards[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  HIP_TRY(launch_fill_i32(d_flag, 0, 1, s));
+  if (c.family != RSGPU_FAMILY_CLASSIC) return leopard_verify(c, shards, shards[0].len, d_flag, s);
+  std::vector<const uint8_t*> in(c.k);
+  std::vector<uint8_t*> out(c.m);
+  for (size_t i = 0; i < c.k; ++i) in[i] = shards[i].ptr;
+  for (size_t i = 0; i < c.m; ++i) out[i] = const_cast<uint8_t*>(shards[c.k + i].ptr);
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  if (c.k <= (size_t)kMaxIn)
+    return run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m, tower, in, out,
+                     shards[0].len, kVerify, d_flag, s);
+  // Wide codecs: recompute into scratch, then compare (two passes).
+  const size_t len = shards[0].len;
+  DeviceBuffer scratch;
+  HIP_TRY(scratch.alloc(len * c.m));
+  std::vector<uint8_t*> tmp(c.m);
+  for (size_t j = 0; j < c.m; ++j) tmp[j] = (uint8_t*)scratch.p + j * len;
+  rc = run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m, tower, in, tmp, len,
+                 kWrite, nullptr, s);
+  if (rc) return rc;
+  // identity rows compare scratch vs stored parity: apply the unit matrix in verify mode
+  std::vector<uint16_t> eye(c.m * c.m, 0);
+  for (size_t j = 0; j < c.m; ++j) eye[j * c.m + j] = 1;
+  std::vector<uint32_t> ht;
+  build_tables_host(eye.data(), (int)c.m, (int)c.m, tower, ht);
+  DeviceBuffer dt;
+  HIP_TRY(dt.alloc(ht.size() * 4));
+  HIP_TRY(hipMemcpyAsync(dt.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, s));
+  std::vector<const uint8_t*> tin(tmp.begin(), tmp.end());
+  if (c.m > (size_t)kMaxIn) return RSGPU_ERR_UNSUPPORTED;
+  rc = run_apply((const uint32_t*)dt.p, (int)c.m, (int)c.m, tower, tin, out, len, kVerify, d_flag, s);
+  HIP_TRY(hipStreamSynchronize(s));  // scratch buffers are freed on return
+  return rc;
+}
+
+// ------------------------------------------------------------ reconstruction
+struct Plan {
+  std::vector<int> valid;    // first k present
+  std::vector<int> targets;  // missing requested: data ascending, then parity ascending
+  size_t shard_len = 0;
+  bool all_present = false;
+};
+
+// with_classic_reconstruct_plan / fill_classic_reconstruct_shape_from_suffix
+// (recovery_plan.rs:262-387) and the target selection of
+// fill_classic_reconstruct_plan (recovery_plan.rs:425-517).
+static int make_plan(const Codec& c, const rsgpu_slot* shards, size_t n, int goal,
+                     const uint8_t* required, size_t n_required, Plan& plan) {
+  const size_t total = c.k + c.m;
+  if (goal == 2 && n_required != c.k && n_required != total) return RSGPU_INVALID_SHARD_FLAGS;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  size_t shard_len = 0, present = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (!shards[i].present) continue;
+    if (shards[i].len == 0) return RSGPU_EMPTY_SHARD;
+    if (shard_len != 0 && shards[i].len != shard_len) return RSGPU_INCORRECT_SHARD_SIZE;
+    shard_len = shards[i].len;
+    ++present;
+  }
+  plan.shard_len = shard_len;
+  if (present == total) {
+    plan.all_present = true;
+    return RSGPU_OK;
+  }
+  if (present < c.k) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  int rc = check_elem_len(c, shard_len);
+  if (rc) return rc;
+  for (size_t i = 0; i < n && plan.valid.size() < c.k; ++i)
+    if (shards[i].present) plan.valid.push_back((int)i);
+  for (size_t i = 0; i < n; ++i) {
+    if (shards[i].present) continue;
+    bool want;
+    if (goal == 2) want = i < n_required && required[i];
+    else want = (i < c.k) || goal == 0;
+    if (want) plan.targets.push_back((int)i);
+  }
+  for (int t : plan.targets) {
+    if (!shards[t].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+    if (shards[t].len != shard_len) return RSGPU_INCORRECT_SHARD_SIZE;
+  }
+  for (int v : plan.valid)
+    if (!shards[v].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  return RSGPU_OK;
+}
+
+static std::string plan_key(const Codec& c, const rsgpu_slot* shards, const Plan& plan) {
+  std::string key(c.k + c.m, '0');
+  for (size_t i = 0; i < c.k + c.m; ++i) key[i] = shards[i].present ? '1' : '0';
+  for (int t : plan.targets) key[t] = 't';
+  return key;
+}
+
+// Host-side invertibility check of the missing-data system, used only for
+// custom matrices: the library's own constructions are MDS, so every survivor
+// set is solvable, but a user matrix may not be, and the reference reports that
+// as InvalidCustomMatrix before any output is written (reconstruct.rs:1130-1160).
+static bool custom_solvable(const Codec& c, const Plan& plan) {
+  const Field F{c.field == RSGPU_FIELD_GF16};
+  int p = 0;
+  while (p < (int)c.k && plan.valid[p] < (int)c.k) ++p;
+  int e = (int)c.k - p;
+  if (e == 0) return true;
+  std::vector<int> missing;
+  for (int d = 0, kn = 0; d < (int)c.k; ++d) {
+    if (kn < p && plan.valid[kn] == d) ++kn;
+    else missing.push_back(d);
+  }
+  std::vector<uint16_t> w((size_t)e * e);
+  for (int r = 0; r < e; ++r)
+    for (int col = 0; col < e; ++col) w[(size_t)r * e + col] = c.matrix[(size_t)plan.valid[p + r] * c.k + missing[col]];
+  return gauss_jordan(F, w, e, e);
+}
+
+static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan& plan,
+                            hipStream_t s, std::shared_ptr<DecodeEntry>& out) {
+  const std::string key = plan_key(c, shards, plan);
+  std::lock_guard<std::mutex> g(c.mu);
+  ++c.requests;
+  auto it = c.index.find(key);
+  if (it != c.index.end()) {
+    ++c.hits;
+    c.lru.splice(c.lru.begin(), c.lru, it->second);
+    out = it->second->second;
+    return RSGPU_OK;
+  }
+  ++c.misses;
+  if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable(c, plan)) return RSGPU_INVALID_CUSTOM_MATRIX;
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  auto e = std::make_shared<DecodeEntry>();
+  e->valid = plan.valid;
+  e->targets = plan.targets;
+  const size_t nt = plan.targets.size();
+  const size_t tab_bytes = nt * c.k * words_per_coef(tower) * 4;
+  HIP_TRY(e->tables.alloc(tab_bytes));
+  HIP_TRY(e->rows.alloc(nt * c.k * 2));
+  HIP_TRY(hipEventCreateWithFlags(&e->last_use, hipEventDisableTiming));
+  e->bytes = tab_bytes + nt * c.k * 2 + key.size();
+  DecodeJob job{};
+  job.k = (int)c.k;
+  job.m = (int)c.m;
+  job.n_targets = (int)nt;
+  job.tower = tower ? 1 : 0;
+  for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)plan.valid[i];
+  for (size_t i = 0; i < nt; ++i) job.targets[i] = (uint16_t)plan.targets[i];
+  int32_t* st = thread_flag(c.device);
+  if (!st) return RSGPU_ERR_OUT_OF_MEMORY;
+  hipError_t he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, (uint8_t*)e->rows.p,
+                                     (uint32_t*)e->tables.p, st + 1, s);
+  if (he != hipSuccess) {
+    set_hip_error(he, "decode_rows");
+    return he == hipErrorInvalidValue ? RSGPU_ERR_UNSUPPORTED : RSGPU_ERR_HIP;
+  }
+  out = e;
+  if (!c.use_cache) return RSGPU_OK;
+  size_t cap = c.cache_capacity;
+  while (!c.lru.empty() && (c.index.size() >= cap || c.cache_bytes + e->bytes > c.cache_max_bytes)) {
+    auto& victim = c.lru.back();
+    // A victim still read by queued kernels keeps its buffers alive through
+    // the shared_ptr held by the caller that launched them; the last owner
+    // frees it after synchronising on its event.
+    c.cache_bytes -= victim.second->bytes;
+    c.index.erase(victim.first);
+    c.lru.pop_back();
+  }
+  if (e->bytes <= c.cache_max_bytes) {
+    c.lru.emplace_front(key, e);
+    c.index[key] = c.lru.begin();
+    c.cache_bytes += e->bytes;
+  }
+  return RSGPU_OK;
+}
+
+static int reconstruct_impl(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
+                            const uint8_t* required, size_t n_required, hipStream_t s) {
+  if (c.family != RSGPU_FAMILY_CLASSIC)
+    return leopard_reconstruct(c, shards, n, goal, required, n_required, s);
+  Plan plan;
+  int rc = make_plan(c, shards, n, goal, required, n_required, plan);
+  if (rc || plan.all_present || plan.targets.empty()) return rc;
+  std::shared_ptr<DecodeEntry> e;
+  if ((rc = get_decode_entry(c, shards, plan, s, e))) return rc;
+  std::vector<const uint8_t*> in(c.k);
+  std::vector<uint8_t*> out(plan.targets.size());
+  for (size_t i = 0; i < c.k; ++i) in[i] = shards[plan.valid[i]].ptr;
+  for (size_t t = 0; t < plan.targets.size(); ++t) out[t] = shards[plan.targets[t]].ptr;
+  rc = run_apply((const uint32_t*)e->tables.p, (int)c.k, (int)plan.targets.size(),
+                 c.field == RSGPU_FIELD_GF16, in, out, plan.shard_len, kWrite, nullptr, s);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(e->last_use, s));
+  // An entry evicted meanwhile is released here; hipFree in its destructor
+  // waits for the queued kernels that still read it.
+  for (int t : plan.targets) shards[t].present = 1;
+  return RSGPU_OK;
+}
+
+}  // namespace rsgpu
+
+using namespace rsgpu;
+
+// ================================================================== C ABI
+extern "C" {
+
+void rsgpu_options_default(rsgpu_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->field = RSGPU_FIELD_GF8;
+  o->codec_family = RSGPU_FAMILY_CLASSIC;
+  o->matrix_mode = RSGPU_MATRIX_VANDERMONDE;
+  o->leopard_mode = RSGPU_LEOPARD_DISABLED;
+  o->inversion_cache = 1;
+}
+
+// resolve_codec_family (src/core/leopard.rs:191-223)
+static int resolve_family(int requested, int mode, bool byte_field, size_t total, size_t parity) {
+  if (requested != RSGPU_FAMILY_CLASSIC || !byte_field) return requested;
+  switch (mode) {
+    case RSGPU_LEOPARD_AS_NEEDED: return total <= 256 ? RSGPU_FAMILY_CLASSIC : RSGPU_FAMILY_LEOPARD_GF16;
+    case RSGPU_LEOPARD_PREFER_GF16: return RSGPU_FAMILY_LEOPARD_GF16;
+    case RSGPU_LEOPARD_PREFER_LEOPARD:
+      return (total <= 256 && parity <= 128) ? RSGPU_FAMILY_LEOPARD_GF8 : RSGPU_FAMILY_LEOPARD_GF16;
+    default: return RSGPU_FAMILY_CLASSIC;
+  }
+}
+
+// leopard_shard_combo_fits (src/core/leopard.rs:225-236)
+static bool leopard_fits(size_t k, size_t m, size_t order) {
+  size_t p = 1;
+  while (p < m) p <<= 1;
+  if (k == 0 || m == 0 || p > order) return false;
+  size_t rounded = (k + p - 1) / p * p;
+  return rounded <= order - p;
+}
+
+static int codec_common(size_t k, size_t m, const rsgpu_options* opts, Codec& c, bool custom) {
+  rsgpu_options o;
+  rsgpu_options_default(&o);
+  if (opts) o = *opts;
+  if (k == 0) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (m == 0) return RSGPU_TOO_FEW_PARITY_SHARDS;
+  if (o.field != RSGPU_FIELD_GF8 && o.field != RSGPU_FIELD_GF16) return RSGPU_ERR_INVALID_ARGUMENT;
+  const size_t total = k + m;
+  const bool byte_field = o.field == RSGPU_FIELD_GF8;
+  if (custom) {
+    if (total > (byte_field ? 256u : 65536u)) return RSGPU_TOO_MANY_SHARDS;
+    if (o.codec_family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+    o.matrix_mode = RSGPU_MATRIX_CUSTOM;
+  } else {
+    o.codec_family = resolve_family(o.codec_family, o.leopard_mode, byte_field, total, m);
+    size_t cap = o.codec_family == RSGPU_FAMILY_CLASSIC ? (byte_field ? 256 : 65536)
+                 : o.codec_family == RSGPU_FAMILY_LEOPARD_GF8 ? 256 : 65536;
+    if (total > cap) return RSGPU_TOO_MANY_SHARDS;
+    if (o.codec_family != RSGPU_FAMILY_CLASSIC) {  // validate_leopard_family (leopard.rs:144-270)
+      if (!byte_field) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+      if (!leopard_fits(k, m, o.codec_family == RSGPU_FAMILY_LEOPARD_GF8 ? 256 : 65536))
+        return RSGPU_TOO_MANY_SHARDS;
+    }
+  }
+  c.k = k;
+  c.m = m;
+  c.field = o.field;
+  c.family = o.codec_family;
+  c.matrix_mode = o.matrix_mode;
+  c.fast_one_parity = o.fast_one_parity != 0;
+  c.use_cache = o.inversion_cache != 0;
+  c.cache_capacity = o.inversion_cache_capacity ? o.inversion_cache_capacity : derive_cache_capacity(k, m);
+  c.cache_max_bytes = o.inversion_cache_max_bytes ? o.inversion_cache_max_bytes : (8u << 20);
+  if (c.family == RSGPU_FAMILY_CLASSIC && k > 255 && c.field == RSGPU_FIELD_GF16) {
+    // device decode solve holds the 16-bit survivor table in 256-entry arrays
+    return RSGPU_ERR_UNSUPPORTED;
+  }
+  return RSGPU_OK;
+}
+
+int rsgpu_codec_new(size_t k, size_t m, const rsgpu_options* opts, rsgpu_codec** out) {
+  if (!out) return RSGPU_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto h = std::make_unique<rsgpu_codec>();
+  Codec& c = h->c;
+  int rc = codec_common(k, m, opts, c, false);
+  if (rc) return rc;
+  if (c.family == RSGPU_FAMILY_CLASSIC) {
+    if (c.matrix_mode == RSGPU_MATRIX_CUSTOM) return RSGPU_INVALID_CUSTOM_MATRIX;
+    rc = build_matrix(Field{c.field == RSGPU_FIELD_GF16}, c.matrix_mode, c.fast_one_parity, (int)k,
+                      (int)m, c.matrix);
+    if (rc) return rc;
+  }
+  if ((rc = upload_codec(c))) return rc;
+  *out = h.release();
+  return RSGPU_OK;
+}
+
+int rsgpu_codec_with_custom_matrix(size_t k, size_t m, const uint8_t* rows, size_t n_rows,
+                                   size_t row_len, const rsgpu_options* opts, rsgpu_codec** out) {
+  if (!out) return RSGPU_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto h = std::make_unique<rsgpu_codec>();
+  Codec& c = h->c;
+  int rc = codec_common(k, m, opts, c, true);
+  if (rc) return rc;
+  if (n_rows < m || row_len < k || !rows) return RSGPU_INVALID_CUSTOM_MATRIX;  // codec.rs:204-215
+  const int eb = c.field == RSGPU_FIELD_GF16 ? 2 : 1;
+  c.matrix.assign((k + m) * k, 0);
+  for (size_t r = 0; r < k; ++r) c.matrix[r * k + r] = 1;
+  for (size_t r = 0; r < m; ++r)
+    for (size_t col = 0; col < k; ++col) {
+      const uint8_t* e = rows + (r * row_len + col) * eb;
+      c.matrix[(k + r) * k + col] = eb == 2 ? (uint16_t)((e[0] << 8) | e[1]) : e[0];
+    }
+  if (c.fast_one_parity && m == 1)  // codec.rs:593-600
+    for (size_t col = 0; col < k; ++col)
+      if (c.matrix[k * k + col] != 1) return RSGPU_INVALID_CUSTOM_MATRIX;
+  if ((rc = upload_codec(c))) return rc;
+  *out = h.release();
+  return RSGPU_OK;
+}
+
+void rsgpu_codec_free(rsgpu_codec* codec) { delete codec; }
+
+int rsgpu_codec_info(const rsgpu_codec* h, size_t* k, size_t* m, int32_t* field, int32_t* fam) {
+  if (!h) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (k) *k = h->c.k;
+  if (m) *m = h->c.m;
+  if (field) *field = h->c.field;
+  if (fam) *fam = h->c.family;
+  return RSGPU_OK;
+}
+
+int rsgpu_codec_matrix(const rsgpu_codec* h, uint8_t* out, size_t out_len) {
+  if (!h || !out) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  const size_t eb = c.field == RSGPU_FIELD_GF16 ? 2 : 1;
+  if (c.matrix.empty()) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  if (out_len < c.matrix.size() * eb) return RSGPU_ERR_INVALID_ARGUMENT;
+  for (size_t i = 0; i < c.matrix.size(); ++i) {
+    if (eb == 2) {
+      out[2 * i] = (uint8_t)(c.matrix[i] >> 8);
+      out[2 * i + 1] = (uint8_t)c.matrix[i];
+    } else {
+      out[i] = (uint8_t)c.matrix[i];
+    }
+  }
+  return RSGPU_OK;
+}
+
+int rsgpu_encode(const rsgpu_codec* h, const rsgpu_slice* shards, size_t n, void* stream) {
+  if (!h || (!shards && n)) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  std::vector<rsgpu_cslice> data(c.k);
+  for (size_t i = 0; i < c.k; ++i) data[i] = {shards[i].ptr, shards[i].len};
+  return encode_sep_impl(c, data.data(), c.k, shards + c.k, c.m, (hipStream_t)stream);
+}
+
+int rsgpu_encode_sep(const rsgpu_codec* h, const rsgpu_cslice* data, size_t nd,
+                     const rsgpu_slice* parity, size_t np, void* stream) {
+  if (!h || (!data && nd) || (!parity && np)) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (nd == 0) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (np == 0) return h->c.m ? RSGPU_TOO_FEW_PARITY_SHARDS : RSGPU_ERR_INVALID_ARGUMENT;
+  return encode_sep_impl(h->c, data, nd, parity, np, (hipStream_t)stream);
+}
+
+int rsgpu_verify_async(const rsgpu_codec* h, const rsgpu_cslice* shards, size_t n,
+                       int32_t* device_flag, void* stream) {
+  if (!h || !shards || !device_flag) return RSGPU_ERR_INVALID_ARGUMENT;
+  return verify_impl(h->c, shards, n, device_flag, (hipStream_t)stream);
+}
+
+int rsgpu_verify(const rsgpu_codec* h, const rsgpu_cslice* shards, size_t n, int32_t* ok,
+                 void* stream) {
+  if (!h || !shards || !ok) return RSGPU_ERR_INVALID_ARGUMENT;
+  int32_t* flag = thread_flag(h->c.device);
+  if (!flag) return RSGPU_ERR_OUT_OF_MEMORY;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = verify_impl(h->c, shards, n, flag, s);
+  if (rc) return rc;
+  int32_t host = 0;
+  HIP_TRY(hipMemcpyAsync(&host, flag, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *ok = host == 0 ? 1 : 0;
+  return RSGPU_OK;
+}
+
+int rsgpu_verify_with_buffer(const rsgpu_codec* h, const rsgpu_cslice* shards, size_t n,
+                             const rsgpu_slice* buffer, size_t nb, int32_t* ok, void* stream) {
+  if (!h || !shards || !ok || (!buffer && nb)) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  if (nb < c.m) return RSGPU_TOO_FEW_BUFFER_SHARDS;
+  if (nb > c.m) return RSGPU_TOO_MANY_BUFFER_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  if ((rc = check_slices(buffer, nb))) return rc;
+  if (shards[0].len != buffer[0].len) return RSGPU_INCORRECT_SHARD_SIZE;
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = encode_sep_impl(c, shards, c.k, buffer, c.m, s))) return rc;
+  // compare recomputed parity with the stored parity through the identity rows
+  int32_t* flag = thread_flag(c.device);
+  if (!flag) return RSGPU_ERR_OUT_OF_MEMORY;
+  HIP_TRY(launch_fill_i32(flag, 0, 1, s));
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  std::vector<uint16_t> eye(c.m * c.m, 0);
+  for (size_t j = 0; j < c.m; ++j) eye[j * c.m + j] = 1;
+  std::vector<uint32_t> ht;
+  build_tables_host(eye.data(), (int)c.m, (int)c.m, tower, ht);
+  DeviceBuffer dt;
+  HIP_TRY(dt.alloc(ht.size() * 4));
+  HIP_TRY(hipMemcpyAsync(dt.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, s));
+  std::vector<const uint8_t*> in(c.m);
+  std::vector<uint8_t*> out(c.m);
+  for (size_t j = 0; j < c.m; ++j) {
+    in[j] = buffer[j].ptr;
+    out[j] = const_cast<uint8_t*>(shards[c.k + j].ptr);
+  }
+  if (c.m > (size_t)kMaxIn) return RSGPU_ERR_UNSUPPORTED;
+  if ((rc = run_apply((const uint32_t*)dt.p, (int)c.m, (int)c.m, tower, in, out, shards[0].len,
+                      kVerify, flag, s)))
+    return rc;
+  int32_t host = 0;
+  HIP_TRY(hipMemcpyAsync(&host, flag, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *ok = host == 0 ? 1 : 0;
+  return RSGPU_OK;
+}
+
+int rsgpu_reconstruct(const rsgpu_codec* h, rsgpu_slot* shards, size_t n, void* stream) {
+  if (!h || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
+  return reconstruct_impl(h->c, shards, n, 0, nullptr, 0, (hipStream_t)stream);
+}
+
+int rsgpu_reconstruct_data(const rsgpu_codec* h, rsgpu_slot* shards, size_t n, void* stream) {
+  if (!h || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
+  return reconstruct_impl(h->c, shards, n, 1, nullptr, 0, (hipStream_t)stream);
+}
+
+int rsgpu_reconstruct_some(const rsgpu_codec* h, rsgpu_slot* shards, size_t n,
+                           const uint8_t* required, size_t n_required, void* stream) {
+  if (!h || !shards || (!required && n_required)) return RSGPU_ERR_INVALID_ARGUMENT;
+  return reconstruct_impl(h->c, shards, n, 2, required, n_required, (hipStream_t)stream);
+}
+
+int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k, const rsgpu_cslice* in,
+                     const rsgpu_slice* out, void* stream) {
+  if (!rows || !in || !out || r == 0 || k == 0) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (field != RSGPU_FIELD_GF8 && field != RSGPU_FIELD_GF16) return RSGPU_ERR_INVALID_ARGUMENT;
+  // SelectedEncodeKernel::execute validation (codegen.rs:46-79): lengths first
+  const size_t len = in[0].len;
+  if (len == 0) return RSGPU_EMPTY_SHARD;
+  for (size_t i = 0; i < k; ++i)
+    if (in[i].len != len || !in[i].ptr) return RSGPU_INCORRECT_SHARD_SIZE;
+  for (size_t j = 0; j < r; ++j)
+    if (out[j].len != len || !out[j].ptr) return RSGPU_INCORRECT_SHARD_SIZE;
+  const bool tower = field == RSGPU_FIELD_GF16;
+  if (tower && (len & 1)) return RSGPU_INCORRECT_SHARD_SIZE;
+  std::vector<uint16_t> rr(r * k);
+  for (size_t i = 0; i < r * k; ++i) rr[i] = tower ? (uint16_t)((rows[2 * i] << 8) | rows[2 * i + 1]) : rows[i];
+  std::vector<uint32_t> ht;
+  build_tables_host(rr.data(), (int)r, (int)k, tower, ht);
+  hipStream_t s = (hipStream_t)stream;
+  DeviceBuffer dt;
+  HIP_TRY(dt.alloc(ht.size() * 4));
+  HIP_TRY(hipMemcpyAsync(dt.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, s));
+  std::vector<const uint8_t*> vin(k);
+  std::vector<uint8_t*> vout(r);
+  for (size_t i = 0; i < k; ++i) vin[i] = in[i].ptr;
+  for (size_t j = 0; j < r; ++j) vout[j] = out[j].ptr;
+  int rc = run_apply((const uint32_t*)dt.p, (int)k, (int)r, tower, vin, vout, len, kWrite, nullptr, s);
+  HIP_TRY(hipStreamSynchronize(s));  // the table buffer is freed on return
+  return rc;
+}
+
+// ---- host-pointer variants
+static int stage_shards(size_t n, size_t len, DeviceBuffer& buf, std::vector<uint8_t*>& dptr) {
+  HIP_TRY(buf.alloc(n * ((len + 255) & ~size_t(255))));
+  dptr.resize(n);
+  for (size_t i = 0; i < n; ++i) dptr[i] = (uint8_t*)buf.p + i * ((len + 255) & ~size_t(255));
+  return RSGPU_OK;
+}
+
+int rsgpu_encode_host(const rsgpu_codec* h, uint8_t* const* shards, size_t n, size_t len) {
+  if (!h || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  if (n < c.k + c.m) return RSGPU_TOO_FEW_SHARDS;
+  if (n > c.k + c.m) return RSGPU_TOO_MANY_SHARDS;
+  if (len == 0) return RSGPU_EMPTY_SHARD;
+  DeviceBuffer buf;
+  std::vector<uint8_t*> d;
+  int rc = stage_shards(n, len, buf, d);
+  if (rc) return rc;
+  for (size_t i = 0; i < c.k; ++i) HIP_TRY(hipMemcpy(d[i], shards[i], len, hipMemcpyHostToDevice));
+  std::vector<rsgpu_slice> sl(n);
+  for (size_t i = 0; i < n; ++i) sl[i] = {d[i], len};
+  if ((rc = rsgpu_encode(h, sl.data(), n, nullptr))) return rc;
+  for (size_t i = c.k; i < n; ++i) HIP_TRY(hipMemcpy(shards[i], d[i], len, hipMemcpyDeviceToHost));
+  return RSGPU_OK;
+}
+
+int rsgpu_verify_host(const rsgpu_codec* h, const uint8_t* const* shards, size_t n, size_t len,
+                      int32_t* ok) {
+  if (!h || !shards || !ok) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (len == 0) return RSGPU_EMPTY_SHARD;
+  DeviceBuffer buf;
+  std::vector<uint8_t*> d;
+  int rc = stage_shards(n, len, buf, d);
+  if (rc) return rc;
+  for (size_t i = 0; i < n; ++i) HIP_TRY(hipMemcpy(d[i], shards[i], len, hipMemcpyHostToDevice));
+  std::vector<rsgpu_cslice> sl(n);
+  for (size_t i = 0; i < n; ++i) sl[i] = {d[i], len};
+  return rsgpu_verify(h, sl.data(), n, ok, nullptr);
+}
+
+int rsgpu_reconstruct_host(const rsgpu_codec* h, uint8_t* const* shards, uint8_t* present,
+                           size_t n, size_t len, int32_t data_only) {
+  if (!h || !shards || !present) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (len == 0) return RSGPU_EMPTY_SHARD;
+  DeviceBuffer buf;
+  std::vector<uint8_t*> d;
+  int rc = stage_shards(n, len, buf, d);
+  if (rc) return rc;
+  std::vector<rsgpu_slot> sl(n);
+  for (size_t i = 0; i < n; ++i) {
+    sl[i] = {d[i], len, present[i] ? 1 : 0};
+    if (present[i]) HIP_TRY(hipMemcpy(d[i], shards[i], len, hipMemcpyHostToDevice));
+  }
+  rc = data_only ? rsgpu_reconstruct_data(h, sl.data(), n, nullptr)
+                 : rsgpu_reconstruct(h, sl.data(), n, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  for (size_t i = 0; i < n; ++i)
+    if (!present[i] && sl[i].present) {
+      HIP_TRY(hipMemcpy(shards[i], d[i], len, hipMemcpyDeviceToHost));
+      present[i] = 1;
+    }
+  return RSGPU_OK;
+}
+
+int rsgpu_cache_stats(const rsgpu_codec* h, uint64_t* requests, uint64_t* hits, uint64_t* misses,
+                      uint64_t* entries) {
+  if (!h) return RSGPU_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> g(h->c.mu);
+  if (requests) *requests = h->c.requests;
+  if (hits) *hits = h->c.hits;
+  if (misses) *misses = h->c.misses;
+  if (entries) *entries = h->c.index.size();
+  return RSGPU_OK;
+}
+
+const char* rsgpu_status_name(int s) {
+  static const char* names[] = {"Ok",
+                                "TooFewShards",
+                                "TooManyShards",
+                                "TooFewDataShards",
+                                "TooManyDataShards",
+                                "TooFewParityShards",
+                                "TooManyParityShards",
+                                "TooFewBufferShards",
+                                "TooManyBufferShards",
+                                "IncorrectShardSize",
+                                "TooFewShardsPresent",
+                                "EmptyShard",
+                                "InvalidShardFlags",
+                                "InvalidIndex",
+                                "InvalidCustomMatrix",
+                                "UnsupportedCodecFamily",
+                                "UnsupportedLeopardPrototype"};
+  if (s >= 0 && s <= 16) return names[s];
+  switch (s) {
+    case RSGPU_ERR_HIP: return "HipError";
+    case RSGPU_ERR_INVALID_ARGUMENT: return "InvalidArgument";
+    case RSGPU_ERR_NO_DEVICE: return "NoDevice";
+    case RSGPU_ERR_UNSUPPORTED: return "Unsupported";
+    case RSGPU_ERR_OUT_OF_MEMORY: return "OutOfMemory";
+  }
+  return "Unknown";
+}
+
+int rsgpu_init(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return RSGPU_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return RSGPU_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(device));
+  return RSGPU_OK;
+}
+
+const char* rsgpu_last_hip_error(void) { return t_hip_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// internal.h -- types shared by the rsgpu host code and its HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstddef>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rsgpu.h"
+#include "gf.h"
+
+namespace rsgpu {
+
+// Pointers handed to one matrix-apply launch travel in the kernel arguments
+// (scalar-loaded from the kernarg segment), so a launch needs no host-to-device
+// copy.  Wider codecs are applied in column chunks of kMaxIn inputs and row
+// chunks of kMaxOut outputs.
+constexpr int kMaxIn = 64;
+constexpr int kMaxOut = 8;
+constexpr int kBlock = 256;
+
+enum ApplyMode : int { kWrite = 0, kAccumulate = 1, kVerify = 2 };
+
+struct ApplyArgs {
+  const uint8_t* in[kMaxIn];
+  uint8_t* out[kMaxOut];   // destinations, or the stored parity when verifying
+  const uint32_t* tables;  // row-major [row][input][kTabWords]
+  int32_t* flag;           // verify: set nonzero on any mismatch
+  uint64_t n16;            // 16-byte column vectors per shard
+  uint64_t v_begin;        // first vector handled by this launch
+  int32_t k;               // inputs in this launch
+  int32_t tab_row_stride;  // dwords between consecutive table rows
+  int32_t tab_col0;        // table column of in[0]
+  int32_t r;               // outputs in this launch (<= kMaxOut)
+};
+
+struct ByteArgs {
+  const uint8_t* in[kMaxIn];
+  uint8_t* out[kMaxOut];
+  const uint32_t* tables;
+  int32_t* flag;
+  uint64_t begin, end;  // byte range [begin, end)
+  int32_t k, tab_row_stride, tab_col0, r;
+  int32_t mode;
+  int32_t tower;  // 1: GF(2^16) tower elements (byte pairs)
+};
+
+// Launchers (classic_kernels.hip).  `nv` selects the vectors-per-thread
+// variant (1 or 2).  `tower` selects GF((2^8)^2) element pairs; tables are then
+// the 2x2-expanded GF(2^8) coefficients [row][input][4][kTabWords].
+hipError_t launch_apply(const ApplyArgs& a, int mode, int nv, int tower, hipStream_t s);
+hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t s);
+
+// Decode rows on the device (classic_kernels.hip).  One workgroup solves the
+// missing-data equations of the selected survivors and composes the requested
+// rows, then writes their byte-permute tables.
+struct DecodeJob {
+  uint16_t valid[256];    // survivor shard indices (first k present), k entries
+  uint16_t targets[256];  // requested outputs: missing data first, then parity
+  int32_t k, m, n_targets, tower;
+};
+hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
+                              uint32_t* d_tables, int32_t* d_status, hipStream_t s);
+
+// Tables for a host-known row set (encode rows, apply_rows primitive).
+void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out);
+
+struct DeviceBuffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer();
+  hipError_t alloc(size_t n);
+};
+
+// One cached decode plan: the composed rows for a (present, targets) pattern.
+struct DecodeEntry {
+  DeviceBuffer tables;     // [n_targets][k][words]
+  DeviceBuffer rows;       // [n_targets][k] elements
+  std::vector<int> valid;  // survivor indices
+  std::vector<int> targets;
+  hipEvent_t last_use = nullptr;  // signalled after the latest kernel reading it
+  size_t bytes = 0;
+  ~DecodeEntry();
+};
+
+struct Codec {
+  size_t k = 0, m = 0;
+  int field = RSGPU_FIELD_GF8;
+  int family = RSGPU_FAMILY_CLASSIC;
+  int matrix_mode = RSGPU_MATRIX_VANDERMONDE;
+  bool fast_one_parity = false;
+  bool use_cache = true;
+  size_t cache_capacity = 0, cache_max_bytes = 0;
+  int device = 0;
+  std::vector<uint16_t> matrix;  // (k+m) x k elements
+  DeviceBuffer d_matrix;         // same, on the device
+  DeviceBuffer d_parity_tables;  // [m][k][words]
+  std::vector<uint32_t> h_parity_tables;
+  // decode cache (LRU), keyed by present mask + target mask
+  mutable std::mutex mu;
+  mutable std::list<std::pair<std::string, std::shared_ptr<DecodeEntry>>> lru;
+  mutable std::unordered_map<std::string,
+                             std::list<std::pair<std::string, std::shared_ptr<DecodeEntry>>>::iterator>
+      index;
+  mutable size_t cache_bytes = 0;
+  mutable uint64_t requests = 0, hits = 0, misses = 0;
+  // Leopard device state (leopard.hip)
+  std::shared_ptr<void> leopard;
+};
+
+int words_per_coef(int tower);  // kTabWords or 4 * kTabWords
+
+// thread-local last HIP error
+void set_hip_error(hipError_t e, const char* where);
+
+}  // namespace rsgpu
+
+struct rsgpu_codec {
+  rsgpu::Codec c;
+};

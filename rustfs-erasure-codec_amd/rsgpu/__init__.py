@@ -1,0 +1,345 @@
+"""rsgpu -- Python mirror of the rustfs-erasure-codec ``ReedSolomon`` API over
+the MI355X C ABI (include/rsgpu.h, librsgpu.so).
+
+Shards are device tensors (``torch.uint8`` on ``cuda``), one per shard, as the
+reference takes one slice per shard.  Every call validates its arguments in
+the reference's order before any device work is enqueued and raises
+``RsError`` carrying the reference's ``Error`` variant name.
+
+There is no CPU fallback: importing this module fails if librsgpu.so has not
+been built (``python rustfs-erasure-codec_amd/build.py``).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librsgpu.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with rustfs-erasure-codec_amd/build.py")
+
+_lib = C.CDLL(LIB_PATH)
+
+ERROR_NAMES = [
+    "Ok", "TooFewShards", "TooManyShards", "TooFewDataShards", "TooManyDataShards",
+    "TooFewParityShards", "TooManyParityShards", "TooFewBufferShards", "TooManyBufferShards",
+    "IncorrectShardSize", "TooFewShardsPresent", "EmptyShard", "InvalidShardFlags", "InvalidIndex",
+    "InvalidCustomMatrix", "UnsupportedCodecFamily", "UnsupportedLeopardPrototype",
+]
+
+FIELDS = {"gf8": 0, "galois_8": 0, "gf16": 1, "galois_16": 1}
+FAMILIES = {"classic": 0, "leopard_gf8": 1, "leopard_gf16": 2}
+MATRIX_MODES = {"vandermonde": 0, "cauchy": 1, "jerasure_like": 2, "custom": 3}
+LEOPARD_MODES = {"disabled": 0, "as_needed": 1, "prefer_gf16": 2, "prefer_leopard": 3}
+
+
+class RsError(Exception):
+    """A validation or device error; ``name`` is the reference Error variant."""
+
+    def __init__(self, code):
+        self.code = code
+        self.name = _lib.rsgpu_status_name(code).decode()
+        detail = _lib.rsgpu_last_hip_error().decode() if code >= 100 else ""
+        super().__init__(f"{self.name}" + (f" ({detail})" if detail else ""))
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("field", C.c_int32), ("codec_family", C.c_int32), ("matrix_mode", C.c_int32),
+        ("leopard_mode", C.c_int32), ("fast_one_parity", C.c_int32), ("inversion_cache", C.c_int32),
+        ("inversion_cache_capacity", C.c_uint64), ("inversion_cache_max_bytes", C.c_uint64),
+    ]
+
+
+class CSlice(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("len", C.c_size_t)]
+
+
+class Slice(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("len", C.c_size_t)]
+
+
+class Slot(C.Structure):
+    _fields_ = [("ptr", C.c_void_p), ("len", C.c_size_t), ("present", C.c_int32)]
+
+
+_vp = C.c_void_p
+_sz = C.c_size_t
+_i32p = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_lib.rsgpu_options_default.argtypes = [C.POINTER(Options)]
+_lib.rsgpu_codec_new.argtypes = [_sz, _sz, C.POINTER(Options), C.POINTER(_vp)]
+_lib.rsgpu_codec_with_custom_matrix.argtypes = [_sz, _sz, _u8p, _sz, _sz, C.POINTER(Options), C.POINTER(_vp)]
+_lib.rsgpu_codec_free.argtypes = [_vp]
+_lib.rsgpu_codec_info.argtypes = [_vp, C.POINTER(_sz), C.POINTER(_sz), _i32p, _i32p]
+_lib.rsgpu_codec_matrix.argtypes = [_vp, _u8p, _sz]
+_lib.rsgpu_encode.argtypes = [_vp, C.POINTER(Slice), _sz, _vp]
+_lib.rsgpu_encode_sep.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _vp]
+_lib.rsgpu_verify.argtypes = [_vp, C.POINTER(CSlice), _sz, _i32p, _vp]
+_lib.rsgpu_verify_async.argtypes = [_vp, C.POINTER(CSlice), _sz, _vp, _vp]
+_lib.rsgpu_verify_with_buffer.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _i32p, _vp]
+_lib.rsgpu_reconstruct.argtypes = [_vp, C.POINTER(Slot), _sz, _vp]
+_lib.rsgpu_reconstruct_data.argtypes = [_vp, C.POINTER(Slot), _sz, _vp]
+_lib.rsgpu_reconstruct_some.argtypes = [_vp, C.POINTER(Slot), _sz, _u8p, _sz, _vp]
+_lib.rsgpu_encode_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz]
+_lib.rsgpu_verify_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz, _i32p]
+_lib.rsgpu_reconstruct_host.argtypes = [_vp, C.POINTER(_vp), _u8p, _sz, _sz, C.c_int32]
+_lib.rsgpu_apply_rows.argtypes = [C.c_int32, _u8p, _sz, _sz, C.POINTER(CSlice), C.POINTER(Slice), _vp]
+_lib.rsgpu_cache_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
+_lib.rsgpu_status_name.restype = C.c_char_p
+_lib.rsgpu_status_name.argtypes = [C.c_int]
+_lib.rsgpu_last_hip_error.restype = C.c_char_p
+_lib.rsgpu_init.argtypes = [C.c_int]
+
+
+def _check(rc):
+    if rc != 0:
+        raise RsError(rc)
+
+
+def lib():
+    return _lib
+
+
+def _stream(stream):
+    if stream is not None:
+        return C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+    import torch
+
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr_len(t):
+    """(device pointer, byte length) of a uint8 device tensor view."""
+    if t is None:
+        return 0, 0
+    if not t.is_contiguous():
+        raise ValueError("shard views must be contiguous")
+    return t.data_ptr(), t.numel() * t.element_size()
+
+
+class ShardSlot:
+    """A shard buffer kept across calls with a presence flag (src/lib.rs:250-304)."""
+
+    def __init__(self, data, present=True):
+        self.data = data
+        self.present = bool(present)
+
+    @classmethod
+    def new_present(cls, data):
+        return cls(data, True)
+
+    @classmethod
+    def new_missing(cls, data):
+        return cls(data, False)
+
+    def is_present(self):
+        return self.present
+
+    def mark_missing(self):
+        self.present = False
+
+    def mark_present(self):
+        self.present = True
+
+
+class ReedSolomon:
+    """ReedSolomon<F> with CodecOptions (src/core/mod.rs:89, src/core/options.rs:99-147)."""
+
+    def __init__(self, data_shards, parity_shards, *, field="gf8", codec_family="classic",
+                 matrix_mode="vandermonde", leopard_mode="disabled", fast_one_parity=False,
+                 inversion_cache=True, inversion_cache_capacity=0, inversion_cache_max_bytes=0,
+                 _custom=None):
+        opts = Options()
+        _lib.rsgpu_options_default(C.byref(opts))
+        opts.field = FIELDS[field]
+        opts.codec_family = FAMILIES[codec_family]
+        opts.matrix_mode = MATRIX_MODES[matrix_mode]
+        opts.leopard_mode = LEOPARD_MODES[leopard_mode]
+        opts.fast_one_parity = int(bool(fast_one_parity))
+        opts.inversion_cache = int(bool(inversion_cache))
+        opts.inversion_cache_capacity = inversion_cache_capacity
+        opts.inversion_cache_max_bytes = inversion_cache_max_bytes
+        h = C.c_void_p()
+        self._h = None
+        if _custom is not None:
+            rows = np.ascontiguousarray(_custom, np.uint8)
+            eb = 2 if opts.field == 1 else 1
+            n_rows = rows.shape[0]
+            row_len = rows.shape[1] // eb if rows.ndim == 2 else 0
+            rc = _lib.rsgpu_codec_with_custom_matrix(data_shards, parity_shards,
+                                                     rows.ctypes.data_as(_u8p), n_rows, row_len,
+                                                     C.byref(opts), C.byref(h))
+        else:
+            rc = _lib.rsgpu_codec_new(data_shards, parity_shards, C.byref(opts), C.byref(h))
+        _check(rc)
+        self._h = h
+        k, m, f, fam = _sz(), _sz(), C.c_int32(), C.c_int32()
+        _lib.rsgpu_codec_info(h, C.byref(k), C.byref(m), C.byref(f), C.byref(fam))
+        self._k, self._m, self._field, self._family = k.value, m.value, f.value, fam.value
+
+    @classmethod
+    def new(cls, data_shards, parity_shards):
+        return cls(data_shards, parity_shards)
+
+    @classmethod
+    def with_custom_matrix(cls, data_shards, parity_shards, rows, **opts):
+        """rows: (n_rows, row_len * elem_bytes) uint8 array, GF16 elements as [x, c] pairs."""
+        return cls(data_shards, parity_shards, matrix_mode="vandermonde", _custom=rows, **opts)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rsgpu_codec_free(self._h)
+            self._h = None
+
+    # ---- shape
+    def data_shard_count(self):
+        return self._k
+
+    def parity_shard_count(self):
+        return self._m
+
+    def total_shard_count(self):
+        return self._k + self._m
+
+    def codec_family(self):
+        return {0: "classic", 1: "leopard_gf8", 2: "leopard_gf16"}[self._family]
+
+    def field(self):
+        return "gf16" if self._field == 1 else "gf8"
+
+    def matrix(self):
+        eb = 2 if self._field == 1 else 1
+        out = np.zeros(((self._k + self._m) * self._k * eb,), np.uint8)
+        _check(_lib.rsgpu_codec_matrix(self._h, out.ctypes.data_as(_u8p), out.size))
+        if eb == 2:
+            out = (out[0::2].astype(np.uint16) << 8) | out[1::2]
+        return out.reshape(self._k + self._m, self._k)
+
+    # ---- encode
+    def encode(self, shards, stream=None):
+        arr = (Slice * len(shards))(*[Slice(*_ptr_len(s)) for s in shards])
+        _check(_lib.rsgpu_encode(self._h, arr, len(shards), _stream(stream)))
+
+    def encode_sep(self, data, parity, stream=None):
+        d = (CSlice * len(data))(*[CSlice(*_ptr_len(s)) for s in data])
+        p = (Slice * len(parity))(*[Slice(*_ptr_len(s)) for s in parity])
+        _check(_lib.rsgpu_encode_sep(self._h, d, len(data), p, len(parity), _stream(stream)))
+
+    # ---- verify
+    def verify(self, shards, stream=None):
+        arr = (CSlice * len(shards))(*[CSlice(*_ptr_len(s)) for s in shards])
+        ok = C.c_int32(0)
+        _check(_lib.rsgpu_verify(self._h, arr, len(shards), C.byref(ok), _stream(stream)))
+        return bool(ok.value)
+
+    def verify_with_workspace(self, shards, workspace=None, stream=None):
+        """The classic GPU verify compares in registers and needs no workspace."""
+        return self.verify(shards, stream)
+
+    def verify_async(self, shards, device_flag, stream=None):
+        arr = (CSlice * len(shards))(*[CSlice(*_ptr_len(s)) for s in shards])
+        _check(_lib.rsgpu_verify_async(self._h, arr, len(shards), device_flag.data_ptr(), _stream(stream)))
+
+    def verify_with_buffer(self, shards, buffer, stream=None):
+        arr = (CSlice * len(shards))(*[CSlice(*_ptr_len(s)) for s in shards])
+        buf = (Slice * len(buffer))(*[Slice(*_ptr_len(s)) for s in buffer])
+        ok = C.c_int32(0)
+        _check(_lib.rsgpu_verify_with_buffer(self._h, arr, len(shards), buf, len(buffer),
+                                             C.byref(ok), _stream(stream)))
+        return bool(ok.value)
+
+    # ---- reconstruct
+    def _slots(self, shards):
+        """Accepts a list of Optional[tensor] or of ShardSlot; returns (ctypes slots, fixups)."""
+        import torch
+
+        shard_len, device = None, None
+        for s in shards:
+            t = s.data if isinstance(s, ShardSlot) else s
+            present = s.present if isinstance(s, ShardSlot) else s is not None
+            if present and t is not None:
+                shard_len = t.numel()
+                device = t.device
+                break
+        slots = (Slot * len(shards))()
+        allocated = {}
+        for i, s in enumerate(shards):
+            if isinstance(s, ShardSlot):
+                p, n = _ptr_len(s.data)
+                slots[i] = Slot(p, n if s.present else n, int(s.present))
+            elif s is not None:
+                p, n = _ptr_len(s)
+                slots[i] = Slot(p, n, 1)
+            else:
+                if shard_len is not None:
+                    t = torch.empty(shard_len, dtype=torch.uint8, device=device)
+                    allocated[i] = t
+                    slots[i] = Slot(t.data_ptr(), shard_len, 0)
+                else:
+                    slots[i] = Slot(None, 0, 0)
+        return slots, allocated
+
+    def _finish(self, shards, slots, allocated):
+        for i, s in enumerate(shards):
+            if isinstance(s, ShardSlot):
+                s.present = bool(slots[i].present)
+            elif s is None and i in allocated and slots[i].present:
+                shards[i] = allocated[i]
+
+    def reconstruct(self, shards, stream=None):
+        slots, alloc = self._slots(shards)
+        _check(_lib.rsgpu_reconstruct(self._h, slots, len(shards), _stream(stream)))
+        self._finish(shards, slots, alloc)
+
+    def reconstruct_data(self, shards, stream=None):
+        slots, alloc = self._slots(shards)
+        _check(_lib.rsgpu_reconstruct_data(self._h, slots, len(shards), _stream(stream)))
+        self._finish(shards, slots, alloc)
+
+    def reconstruct_some(self, shards, required, stream=None):
+        slots, alloc = self._slots(shards)
+        req = np.ascontiguousarray(required, np.uint8)
+        _check(_lib.rsgpu_reconstruct_some(self._h, slots, len(shards), req.ctypes.data_as(_u8p),
+                                           req.size, _stream(stream)))
+        self._finish(shards, slots, alloc)
+
+    # ---- host buffers (PCIe-inclusive path)
+    def encode_host(self, shards):
+        arrs = [np.ascontiguousarray(s, np.uint8) for s in shards]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        _check(_lib.rsgpu_encode_host(self._h, ptrs, len(arrs), arrs[0].size))
+        for s, a in zip(shards, arrs):
+            if s is not a:
+                s[...] = a
+
+    def reconstruct_host(self, shards, present, data_only=False):
+        arrs = [np.ascontiguousarray(s, np.uint8) for s in shards]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        pres = np.ascontiguousarray(present, np.uint8).copy()
+        _check(_lib.rsgpu_reconstruct_host(self._h, ptrs, pres.ctypes.data_as(_u8p), len(arrs),
+                                           arrs[0].size, int(data_only)))
+        return arrs, pres
+
+    def cache_stats(self):
+        vals = [C.c_uint64() for _ in range(4)]
+        _check(_lib.rsgpu_cache_stats(self._h, *[C.byref(v) for v in vals]))
+        return dict(zip(["requests", "hits", "misses", "entries"], [v.value for v in vals]))
+
+
+def apply_rows(rows, inputs, outputs, field="gf8", stream=None):
+    """out_j = XOR_i rows[j][i] * in_i (SelectedEncodeKernel::execute semantics)."""
+    rows = np.asarray(rows)
+    f = FIELDS[field]
+    r, k = rows.shape
+    if f == 1:
+        rb = np.zeros((r, k, 2), np.uint8)
+        rb[..., 0] = (rows.astype(np.uint32) >> 8) & 0xFF
+        rb[..., 1] = rows.astype(np.uint32) & 0xFF
+        rows_b = np.ascontiguousarray(rb.reshape(-1))
+    else:
+        rows_b = np.ascontiguousarray(rows.astype(np.uint8).reshape(-1))
+    ins = (CSlice * k)(*[CSlice(*_ptr_len(t)) for t in inputs])
+    outs = (Slice * r)(*[Slice(*_ptr_len(t)) for t in outputs])
+    _check(_lib.rsgpu_apply_rows(f, rows_b.ctypes.data_as(_u8p), r, k, ins, outs, _stream(stream)))
