@@ -1,12 +1,20 @@
 """Benchmark of the Reed-Solomon data path on one MI355X.
 
-Workload (one step): 512 stripes of RS 10+4 with 1 MiB shards, device-resident uniform
-random bytes, each stripe encoded by its own launch ("plain" per-stripe launches).
-The same run also measures verify and reconstruct (1..4 random erasures per stripe),
-the 8+3 shape, per-stripe latency (p50/p99), the roofline of the encode kernel and a
-CPU baseline (the oracle's restatement of the reference algorithm) on a bounded sample.
+Workload: 512 stripes of RS 10+4 (GF(2^8), Vandermonde) with 1 MiB shards of uniform random
+bytes, device-resident.  One step encodes all 512 stripes.
 
-Prints ONE JSON line.  `value` = data bytes (k * shard) encoded per second.
+  value               : batched encode -- the 512 stripes in ONE persistent-kernel launch
+                        replayed from a hipGraph (data bytes k*S per second)
+  encode_plain_gbs    : the same stripes launched one stripe at a time
+  verify_*/reconstruct_*: verify (register compare) and reconstruct with 1..4 random erasures
+                        per stripe (data and parity mixed), plain and batched
+  encode_8x3_*        : the 8+3 x 1 MiB shape
+  stripe_latency_ms   : p50/p99 device time of one plain stripe encode
+  roofline            : the batch kernel (dominant kernel) vs the 8 TB/s HBM peak
+  cpu_baseline        : the oracle (scalar C restatement of the reference algorithm) on a
+                        bounded sample, one host thread
+
+Prints ONE JSON line.
 """
 import argparse
 import ctypes as C
@@ -44,55 +52,59 @@ class Stripes:
         self.buf[:, :k].random_(0, 256, generator=g)
         self.rs = rsgpu.ReedSolomon(k, m)
         self.lib = rsgpu.lib()
-        base, stride_s, stride_i = self.buf.data_ptr(), (k + m) * shard, shard
-        self.enc = []
-        self.ver = []
-        for s in range(n):
-            arr = (rsgpu.Slice * (k + m))(*[rsgpu.Slice(base + s * stride_s + i * stride_i, shard)
-                                            for i in range(k + m)])
-            self.enc.append(arr)
-            carr = (rsgpu.CSlice * (k + m))(*[rsgpu.CSlice(base + s * stride_s + i * stride_i, shard)
-                                              for i in range(k + m)])
-            self.ver.append(carr)
         self.rsgpu = rsgpu
+        self.enc, self.ver = [], []
+        for s in range(n):
+            ptrs = [self.ptr(s, i) for i in range(k + m)]
+            self.enc.append((rsgpu.Slice * (k + m))(*[rsgpu.Slice(p, shard) for p in ptrs]))
+            self.ver.append((rsgpu.CSlice * (k + m))(*[rsgpu.CSlice(p, shard) for p in ptrs]))
+
+    def ptr(self, s, i):
+        return self.buf.data_ptr() + (s * (self.k + self.m) + i) * self.shard
+
+    def _ok(self, rc):
+        if rc:
+            raise self.rsgpu.RsError(rc)
 
     def encode(self, s, stream):
-        rc = self.lib.rsgpu_encode(self.rs._h, self.enc[s], self.k + self.m, stream)
-        if rc:
-            raise self.rsgpu.RsError(rc)
+        self._ok(self.lib.rsgpu_encode(self.rs._h, self.enc[s], self.k + self.m, stream))
 
     def verify_async(self, s, flag_ptr, stream):
-        rc = self.lib.rsgpu_verify_async(self.rs._h, self.ver[s], self.k + self.m, flag_ptr, stream)
-        if rc:
-            raise self.rsgpu.RsError(rc)
+        self._ok(self.lib.rsgpu_verify_async(self.rs._h, self.ver[s], self.k + self.m, flag_ptr, stream))
 
-    def erasure_slots(self, seed):
-        """Per-stripe slot arrays with 1..m erasures at random positions (data and parity
-        mixed); the erased shards are rebuilt into a separate output buffer."""
+    def erasures(self, seed):
+        """Per stripe: 1..m erased positions at random (data and parity mixed)."""
         import numpy as np
 
         rng = np.random.default_rng(seed)
-        self.rbuf = None
-        slots = []
-        base = self.buf.data_ptr()
-        n_erased = 0
-        for s in range(self.n):
+        out = []
+        for _ in range(self.n):
             e = int(rng.integers(1, self.m + 1))
-            missing = set(rng.choice(self.k + self.m, e, replace=False).tolist())
-            n_erased += e
-            arr = (self.rsgpu.Slot * (self.k + self.m))()
-            for i in range(self.k + self.m):
-                ptr = base + (s * (self.k + self.m) + i) * self.shard
-                arr[i] = self.rsgpu.Slot(ptr, self.shard, 0 if i in missing else 1)
-            slots.append((arr, missing))
-        return slots, n_erased
+            out.append(set(rng.choice(self.k + self.m, e, replace=False).tolist()))
+        return out
+
+    def slots(self, s, missing):
+        arr = (self.rsgpu.Slot * (self.k + self.m))()
+        for i in range(self.k + self.m):
+            arr[i] = self.rsgpu.Slot(self.ptr(s, i), self.shard, 0 if i in missing else 1)
+        return arr
 
     def reconstruct(self, arr, stream):
-        for i in range(self.k + self.m):
-            pass
-        rc = self.lib.rsgpu_reconstruct(self.rs._h, arr, self.k + self.m, stream)
-        if rc:
-            raise self.rsgpu.RsError(rc)
+        self._ok(self.lib.rsgpu_reconstruct(self.rs._h, arr, self.k + self.m, stream))
+
+    def batch(self, op, stream, erasures=None, flags=None):
+        b = self.rsgpu.Batch()
+        lib = self.lib
+        for s in range(self.n):
+            if op == "encode":
+                self._ok(lib.rsgpu_batch_add_encode(b._h, self.rs._h, self.enc[s], self.k + self.m))
+            elif op == "verify":
+                self._ok(lib.rsgpu_batch_add_verify(b._h, self.rs._h, self.ver[s], self.k + self.m,
+                                                    flags.data_ptr() + 4 * s))
+            else:
+                self._ok(lib.rsgpu_batch_add_reconstruct(b._h, self.rs._h, self.slots(s, erasures[s]),
+                                                         self.k + self.m, 0, stream))
+        return b
 
 
 def timed(torch, fn, steps, warmup):
@@ -106,19 +118,20 @@ def timed(torch, fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def per_launch_events(torch, fn, count):
-    """Average / p50 / p99 device duration of single launches via events on torch's stream."""
+def event_durations(torch, fn, count):
+    """Device durations (ms) of `count` calls of fn(i), events recorded on torch's stream
+    (the stream every launch here is enqueued on)."""
     import numpy as np
 
     stream = torch.cuda.current_stream()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(count)]
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(count)]
     for i in range(count):
         evs[i][0].record(stream)
         fn(i)
         evs[i][1].record(stream)
     torch.cuda.synchronize()
-    d = np.array([a.elapsed_time(b) for a, b in evs])
-    return float(d.mean()), float(np.percentile(d, 50)), float(np.percentile(d, 99))
+    return np.array([a.elapsed_time(b) for a, b in evs])
 
 
 def cpu_baseline(k, m, shard, stripes):
@@ -137,7 +150,8 @@ def cpu_baseline(k, m, shard, stripes):
     dt = time.perf_counter() - t0
     return {"value": round(stripes * k * shard / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
             "kind": "port",
-            "sample": f"{stripes} stripes of RS {k}+{m} x {shard // 1024} KiB encode, scalar oracle, 1 thread"}
+            "sample": f"{stripes} stripes of RS {k}+{m} x {shard // 1024} KiB encode, scalar oracle, "
+                      "1 thread"}
 
 
 def main():
@@ -155,88 +169,109 @@ def main():
     stream = torch.cuda.current_stream()
     sp = C.c_void_p(stream.cuda_stream)
     k, m, S, N = 10, 4, args.shard, args.stripes
-
     st = Stripes(rsgpu, torch, k, m, N, S, seed=0x00EC5EED)
+    data_bytes = N * k * S
 
-    def enc_all():
-        for s in range(N):
-            st.encode(s, sp)
+    # ---- plain: one launch per stripe
+    t_enc_plain = timed(torch, lambda: [st.encode(s, sp) for s in range(N)], args.steps, args.warmup)
+    lat = event_durations(torch, lambda i: st.encode(i % N, sp), min(N, 256))
+    ref_parity = st.buf[:, k:].clone()
 
-    t_enc = timed(torch, enc_all, args.steps, args.warmup)
-    enc_gbs = N * k * S / t_enc / 1e9
+    # ---- batched encode: one persistent launch, replayed from a hipGraph
+    b_enc = st.batch("encode", sp)
+    tune = b_enc.autotune(reps=3)
+    st.buf[:, k:].zero_()
+    b_enc.graph_launch()
+    torch.cuda.synchronize()
+    assert torch.equal(st.buf[:, k:], ref_parity), "batched encode differs from plain encode"
+    t_enc_batch = timed(torch, b_enc.graph_launch, args.steps, args.warmup)
+    d_batch = event_durations(torch, lambda i: b_enc.graph_launch(), 3)
+    bytes_per_launch = N * (k + m) * S
+    achieved = bytes_per_launch / (float(d_batch.mean()) * 1e-3) / 1e9
 
-    # roofline + latency of single encode launches (events on the launch stream)
-    avg_ms, p50_ms, p99_ms = per_launch_events(torch, lambda i: st.encode(i % N, sp), min(N, 256))
-    bytes_per_launch = (k + m) * S
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-
-    # verify (register compare, no parity writes)
+    # ---- verify
     flags = torch.zeros(N, dtype=torch.int32, device="cuda")
+    t_ver_plain = timed(torch, lambda: [st.verify_async(s, flags.data_ptr() + 4 * s, sp)
+                                        for s in range(N)], max(1, args.steps // 2), 1)
+    assert int(flags.sum()) == 0
+    b_ver = st.batch("verify", sp, flags=flags)
+    b_ver.autotune(reps=2)
 
-    def ver_all():
-        for s in range(N):
-            st.verify_async(s, flags.data_ptr() + 4 * s, sp)
+    def ver_batch():
+        flags.zero_()
+        b_ver.graph_launch()
 
-    t_ver = timed(torch, ver_all, max(1, args.steps // 2), 1)
+    t_ver_batch = timed(torch, ver_batch, args.steps, 1)
     assert int(flags.sum()) == 0, "verify reported a mismatch on freshly encoded stripes"
-    ver_gbs = N * k * S / t_ver / 1e9
 
-    # reconstruct 1..m random erasures per stripe (in place: survivors read, erased rewritten)
-    slots, n_erased = st.erasure_slots(7)
-    ref = st.buf[: min(N, 8)].clone()
+    # ---- reconstruct 1..4 erasures per stripe, in place (survivors read, erased rewritten)
+    er = st.erasures(7)
+    n_erased = sum(len(e) for e in er)
+    slot_arrays = [st.slots(s, er[s]) for s in range(N)]
 
-    def rec_all():
-        for arr, _ in slots:
+    def rec_plain():
+        for s in range(N):
+            arr = slot_arrays[s]
             for i in range(k + m):
-                arr[i].present = 0 if i in _ else 1
+                arr[i].present = 0 if i in er[s] else 1
             st.reconstruct(arr, sp)
 
-    t_rec = timed(torch, rec_all, max(1, args.steps // 2), 1)
-    assert torch.equal(ref, st.buf[: ref.shape[0]]), "reconstruct changed stripe bytes"
-    rec_gbs = N * k * S / t_rec / 1e9
-    rec_moved = (N * k * S + n_erased * S) / t_rec / 1e9
+    ref = st.buf.clone()
+    t_rec_plain = timed(torch, rec_plain, max(1, args.steps // 2), 1)
+    b_rec = st.batch("reconstruct", sp, erasures=er)
+    b_rec.autotune(reps=2)
+    t_rec_batch = timed(torch, b_rec.graph_launch, args.steps, 1)
+    assert torch.equal(ref, st.buf), "reconstruct changed stripe bytes"
+    del ref
 
-    # 8+3 plain encode
-    del slots
+    # ---- 8+3
+    del b_enc, b_ver, b_rec
     st83 = Stripes(rsgpu, torch, 8, 3, N, S, seed=83)
-
-    def enc83():
-        for s in range(N):
-            st83.encode(s, sp)
-
-    t83 = timed(torch, enc83, max(1, args.steps // 2), 1)
-    enc83_gbs = N * 8 * S / t83 / 1e9
-    del st83
+    t83_plain = timed(torch, lambda: [st83.encode(s, sp) for s in range(N)], max(1, args.steps // 2), 1)
+    b83 = st83.batch("encode", sp)
+    b83.autotune(reps=2)
+    t83_batch = timed(torch, b83.graph_launch, args.steps, 1)
+    del b83, st83
 
     cpu = None if args.no_cpu else cpu_baseline(k, m, S, args.cpu_stripes)
-
+    gbs = lambda nbytes, t: round(nbytes / t / 1e9, 2)
+    rec_moved = N * k * S + n_erased * S
     out = {
         "metric": "data GB/s RS 10+4 encode/reconstruct, batched vs plain, bit-exact; p99 ms",
-        "value": round(enc_gbs, 2),
+        "value": gbs(data_bytes, t_enc_batch),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_enc * 1e3, 3),
+        "ms_per_step": round(t_enc_batch * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": f"RS 10+4 GF(2^8) encode, {N} stripes x {S // 1024} KiB shards, "
-                               "device-resident, one launch per stripe",
+        "config": {"workload": f"RS 10+4 GF(2^8) encode of {N} stripes x {S // 1024} KiB shards, "
+                               "device-resident, batched: one persistent launch per step "
+                               "(hipGraph replay); plain per-stripe launches reported alongside",
                    "stripes": N, "shard_bytes": S, "data_shards": k, "parity_shards": m},
-        "encode_plain_gbs": round(enc_gbs, 2),
-        "encode_plain_bytes_moved_gbs": round(enc_gbs * (k + m) / k, 2),
-        "verify_plain_gbs": round(ver_gbs, 2),
-        "reconstruct_plain_gbs": round(rec_gbs, 2),
-        "reconstruct_plain_bytes_moved_gbs": round(rec_moved, 2),
-        "encode_8x3_plain_gbs": round(enc83_gbs, 2),
-        "stripe_latency_ms": {"p50": round(p50_ms, 4), "p99": round(p99_ms, 4)},
+        "encode_batched_gbs": gbs(data_bytes, t_enc_batch),
+        "encode_plain_gbs": gbs(data_bytes, t_enc_plain),
+        "encode_batched_bytes_moved_gbs": gbs(N * (k + m) * S, t_enc_batch),
+        "encode_batched_vs_plain": round(t_enc_plain / t_enc_batch, 2),
+        "verify_batched_gbs": gbs(data_bytes, t_ver_batch),
+        "verify_plain_gbs": gbs(data_bytes, t_ver_plain),
+        "reconstruct_batched_gbs": gbs(data_bytes, t_rec_batch),
+        "reconstruct_plain_gbs": gbs(data_bytes, t_rec_plain),
+        "reconstruct_batched_bytes_moved_gbs": gbs(rec_moved, t_rec_batch),
+        "encode_8x3_batched_gbs": gbs(N * 8 * S, t83_batch),
+        "encode_8x3_plain_gbs": gbs(N * 8 * S, t83_plain),
+        "stripe_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                              "p99": round(float(np.percentile(lat, 99)), 4)},
+        "batch_tune": tune,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "apply_kernel (encode 10+4 x 1 MiB, one stripe per launch)",
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": round(avg_ms, 5)},
+                     "kernel": "batch_kernel (512 x RS 10+4 x 1 MiB encode, one launch)",
+                     "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": round(float(d_batch.mean()), 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

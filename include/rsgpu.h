@@ -156,6 +156,37 @@ int rsgpu_reconstruct_host(const rsgpu_codec* codec, uint8_t* const* shards, uin
 int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k,
                      const rsgpu_cslice* in, const rsgpu_slice* out, void* stream);
 
+/* ---- batches: many stripe operations in one persistent launch.
+ * The reference runs each stripe call separately and parallelises inside it
+ * (encode_sep_par encode.rs:1177, execute_classic_plan_parallel
+ * reconstruct.rs:229, ParallelPolicy parallel.rs:65-264); a batch is that
+ * parallel policy for many stripes at once.  Each add_* validates exactly like
+ * the single-stripe call of the same name and records the op; classic GF(2^8)
+ * ops run in one persistent kernel, tower-field and Leopard ops through their
+ * own kernels on the same stream.  Reconstruct ops enqueue any decode-row solve
+ * on `stream` when added and mark their targets present.  Verify ops OR a
+ * nonzero value into `device_flag` on mismatch (the caller zeroes it). ---- */
+typedef struct rsgpu_batch rsgpu_batch;
+int rsgpu_batch_new(rsgpu_batch** out);
+void rsgpu_batch_free(rsgpu_batch* batch);
+int rsgpu_batch_clear(rsgpu_batch* batch);
+int rsgpu_batch_add_encode(rsgpu_batch* batch, const rsgpu_codec* codec, const rsgpu_slice* shards,
+                           size_t n_shards);
+int rsgpu_batch_add_verify(rsgpu_batch* batch, const rsgpu_codec* codec, const rsgpu_cslice* shards,
+                           size_t n_shards, int32_t* device_flag);
+int rsgpu_batch_add_reconstruct(rsgpu_batch* batch, const rsgpu_codec* codec, rsgpu_slot* shards,
+                                size_t n_shards, int32_t data_only, void* stream);
+/* Tile shape: vectors of 16 bytes per lane (1 or 2) and workgroups per CU. */
+int rsgpu_batch_set_tune(rsgpu_batch* batch, int32_t vectors_per_lane, int32_t blocks_per_cu);
+/* Times every tile shape on this batch (`reps` runs each) and keeps the fastest. */
+int rsgpu_batch_autotune(rsgpu_batch* batch, void* stream, int32_t reps, int32_t* vectors_per_lane,
+                         int32_t* blocks_per_cu, float* ms);
+int rsgpu_batch_run(rsgpu_batch* batch, void* stream);
+/* Same work, captured once into a hipGraph and replayed on later calls. */
+int rsgpu_batch_graph_launch(rsgpu_batch* batch, void* stream);
+int rsgpu_batch_stats(const rsgpu_batch* batch, uint64_t* n_ops, uint64_t* bytes_moved,
+                      uint64_t* data_bytes, uint64_t* tiles);
+
 /* ---- decode-matrix cache statistics (ReconstructionCacheStats, codec.rs:401-426) ---- */
 int rsgpu_cache_stats(const rsgpu_codec* codec, uint64_t* requests, uint64_t* hits,
                       uint64_t* misses, uint64_t* entries);

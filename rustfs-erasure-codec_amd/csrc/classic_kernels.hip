@@ -11,59 +11,18 @@
 // x[7:6] (tables from make_perm_table, held in SGPRs), folded with 3-input
 // XORs (v_bitop3_b32).  The bit-field selectors depend only on the input and
 // are shared by all r outputs.
-#include "internal.h"
+#include "kernel_common.h"
 
 namespace rsgpu {
 
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-
-struct Sel {
-  uint32_t s0, s1, s2;
-};
-
-__device__ __forceinline__ Sel selectors(uint32_t x) {
-  Sel s;
-  s.s0 = x & 0x07070707u;
-  s.s1 = (x >> 3) & 0x07070707u;
-  s.s2 = (x >> 6) & 0x03030303u;
-  return s;
-}
-
-// c * x for the four bytes of x, with c's table t[0..4].
-__device__ __forceinline__ uint32_t gmul(const uint32_t* t, const Sel& s) {
-  uint32_t a = __builtin_amdgcn_perm(t[1], t[0], s.s0);
-  uint32_t b = __builtin_amdgcn_perm(t[3], t[2], s.s1);
-  uint32_t c = __builtin_amdgcn_perm(t[4], t[4], s.s2);
-  return xor3(a, b, c);
-}
-
-__device__ __forceinline__ uint4 load16(const uint8_t* p) {
-  return *reinterpret_cast<const uint4*>(p);
-}
-__device__ __forceinline__ void store16(uint8_t* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
-
-// Tower elements [x, c] interleaved in a 16-byte vector -> planes (X0, X1, C0, C1).
-__device__ __forceinline__ void deinterleave(uint4 v, uint32_t* w) {
-  w[0] = __builtin_amdgcn_perm(v.y, v.x, 0x06040200u);
-  w[2] = __builtin_amdgcn_perm(v.y, v.x, 0x07050301u);
-  w[1] = __builtin_amdgcn_perm(v.w, v.z, 0x06040200u);
-  w[3] = __builtin_amdgcn_perm(v.w, v.z, 0x07050301u);
-}
-__device__ __forceinline__ uint4 interleave(const uint32_t* w) {
-  uint4 v;
-  v.x = __builtin_amdgcn_perm(w[2], w[0], 0x05010400u);
-  v.y = __builtin_amdgcn_perm(w[2], w[0], 0x07030602u);
-  v.z = __builtin_amdgcn_perm(w[3], w[1], 0x05010400u);
-  v.w = __builtin_amdgcn_perm(w[3], w[1], 0x07030602u);
-  return v;
-}
 
 // R outputs, NV vectors per lane, TOWER selects GF(2^16) tower element pairs.
 template <int R, int NV, int MODE, bool TOWER>
 __global__ __launch_bounds__(kBlock) void apply_kernel(const ApplyArgs a) {
   constexpr int W = TOWER ? 4 * kTabWords : kTabWords;  // table dwords per coefficient
+  // Inputs per load group: every lane issues KB*NV 128-bit loads back to back
+  // before the arithmetic, so a k <= 16 stripe has all its input in flight.
+  constexpr int KB = NV == 1 ? 16 : 8;
   const uint64_t tile = (uint64_t)kBlock * NV;
   const uint64_t n_tiles = (a.n16 + tile - 1) / tile;
   for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
@@ -83,19 +42,17 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(const ApplyArgs a) {
 #pragma unroll
         for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
 
-    // Inputs are consumed in groups of 4 so that each lane keeps 4*NV 128-bit
-    // loads in flight before the arithmetic that needs them.
-    for (int i0 = 0; i0 < a.k; i0 += 4) {
-      uint4 x[4][NV];
+    for (int i0 = 0; i0 < a.k; i0 += KB) {
+      uint4 x[KB][NV];
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
+      for (int b = 0; b < KB; ++b)
 #pragma unroll
         for (int j = 0; j < NV; ++j)
           x[b][j] = (i0 + b < a.k && ok[j]) ? load16(a.in[i0 + b] + v[j]) : make_uint4(0, 0, 0, 0);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        if (i0 + b >= a.k) break;
-        const uint32_t* tcol = a.tables + (size_t)(a.tab_col0 + i0 + b) * W;
+      for (int b = 0; b < KB; ++b) {
+        if (i0 + b >= a.k) continue;  // uniform; `continue` keeps the loop fully unrollable
+        ctab_t tcol = as_const(a.tables) + (size_t)(a.tab_col0 + i0 + b) * W;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
           uint32_t w[4];
@@ -112,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(const ApplyArgs a) {
           for (int d = 0; d < 4; ++d) s[d] = selectors(w[d]);
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const uint32_t* tc = tcol + (size_t)r * a.tab_row_stride;
+            ctab_t tc = tcol + (size_t)r * a.tab_row_stride;
             if constexpr (TOWER) {
               // [A B; C D] per tower_coefficients: x' = A x ^ B c, c' = C x ^ D c
 #pragma unroll

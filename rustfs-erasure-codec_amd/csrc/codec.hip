@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "batch.h"
 #include "internal.h"
 #include "leopard.h"
 
@@ -37,10 +38,6 @@ hipError_t DeviceBuffer::alloc(size_t n) {
   hipError_t e = hipMalloc(&p, n);
   if (e == hipSuccess) bytes = n;
   return e;
-}
-
-DecodeEntry::~DecodeEntry() {
-  if (last_use) (void)hipEventDestroy(last_use);
 }
 
 // ------------------------------------------------------- field abstraction
@@ -224,7 +221,8 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
   if (mode == kVerify && k_total > kMaxIn) return RSGPU_ERR_UNSUPPORTED;  // handled by caller
   const uint64_t n16 = vec_ok ? len / 16 : 0;
   const size_t tail_begin = (size_t)n16 * 16;
-  if (nv == 0) nv = 2;
+  // Two vectors per lane only when that still leaves >= 8 workgroups per CU.
+  if (nv == 0) nv = n16 >= (uint64_t)kBlock * 2 * 2048 ? 2 : 1;
   for (int r0 = 0; r0 < r_total; r0 += kMaxOut) {
     const int rn = std::min(kMaxOut, r_total - r0);
     for (int k0 = 0; k0 < k_total; k0 += kMaxIn) {
@@ -464,7 +462,6 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
   const size_t tab_bytes = nt * c.k * words_per_coef(tower) * 4;
   HIP_TRY(e->tables.alloc(tab_bytes));
   HIP_TRY(e->rows.alloc(nt * c.k * 2));
-  HIP_TRY(hipEventCreateWithFlags(&e->last_use, hipEventDisableTiming));
   e->bytes = tab_bytes + nt * c.k * 2 + key.size();
   DecodeJob job{};
   job.k = (int)c.k;
@@ -517,10 +514,112 @@ static int reconstruct_impl(const Codec& c, rsgpu_slot* shards, size_t n, int go
   rc = run_apply((const uint32_t*)e->tables.p, (int)c.k, (int)plan.targets.size(),
                  c.field == RSGPU_FIELD_GF16, in, out, plan.shard_len, kWrite, nullptr, s);
   if (rc) return rc;
-  HIP_TRY(hipEventRecord(e->last_use, s));
-  // An entry evicted meanwhile is released here; hipFree in its destructor
-  // waits for the queued kernels that still read it.
+  // An entry evicted meanwhile is released when `e` goes out of scope; hipFree
+  // in its destructor waits for the queued kernels that still read it.
   for (int t : plan.targets) shards[t].present = 1;
+  return RSGPU_OK;
+}
+
+// ------------------------------------------------------------ batch stripes
+// Adds one stripe operation to a persistent-kernel batch: descriptors of at
+// most kMaxOut outputs each, plus byte-kernel jobs for unaligned views and for
+// the length % 16 tail.
+static int add_stripe(BatchPlan& plan, const std::vector<const uint8_t*>& in,
+                      const std::vector<uint8_t*>& out, const uint32_t* tables, size_t len, int mode,
+                      int32_t* flag) {
+  const int k = (int)in.size(), r = (int)out.size();
+  bool vec_ok = true;
+  for (auto p : in) vec_ok &= aligned16(p);
+  for (auto p : out) vec_ok &= aligned16(p);
+  const uint64_t n16 = vec_ok ? len / 16 : 0;
+  const size_t tail = (size_t)n16 * 16;
+  if (tail < len && k > kMaxIn) return RSGPU_ERR_UNSUPPORTED;
+  for (int r0 = 0; r0 < r; r0 += kMaxOut) {
+    const int rn = std::min(kMaxOut, r - r0);
+    if (n16) {
+      BatchStripe s{};
+      s.n16 = n16;
+      s.tables = tables + (size_t)r0 * k * kTabWords;
+      s.flag = flag;
+      s.ptr0 = (uint32_t)plan.ptrs.size();
+      s.tab_row_stride = (uint32_t)(k * kTabWords);
+      s.k = (uint16_t)k;
+      s.r = (uint16_t)rn;
+      s.mode = (uint16_t)mode;
+      s.tower = 0;
+      for (auto p : in) plan.ptrs.push_back(p);
+      for (int j = 0; j < rn; ++j) plan.ptrs.push_back(out[r0 + j]);
+      plan.stripes.push_back(s);
+    }
+    if (tail < len) {
+      ByteArgs b{};
+      for (int i = 0; i < k; ++i) b.in[i] = in[i];
+      for (int j = 0; j < rn; ++j) b.out[j] = out[r0 + j];
+      b.tables = tables + (size_t)r0 * k * kTabWords;
+      b.flag = flag;
+      b.begin = tail;
+      b.end = len;
+      b.k = k;
+      b.tab_row_stride = k * kTabWords;
+      b.tab_col0 = 0;
+      b.r = rn;
+      b.mode = mode;
+      b.tower = 0;
+      plan.byte_jobs.push_back(b);
+    }
+  }
+  plan.bytes_moved += (uint64_t)(k + r) * len;
+  plan.data_bytes += (uint64_t)k * len;
+  plan.n_ops++;
+  return RSGPU_OK;
+}
+
+int batch_encode_stripe(const Codec& c, const rsgpu_slice* shards, size_t n, BatchPlan& plan) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  std::vector<const uint8_t*> in(c.k);
+  std::vector<uint8_t*> out(c.m);
+  for (size_t i = 0; i < c.k; ++i) in[i] = shards[i].ptr;
+  for (size_t j = 0; j < c.m; ++j) out[j] = shards[c.k + j].ptr;
+  for (size_t i = 0; i < n; ++i)
+    if (!shards[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  return add_stripe(plan, in, out, (const uint32_t*)c.d_parity_tables.p, shards[0].len, kWrite, nullptr);
+}
+
+int batch_verify_stripe(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* flag,
+                        BatchPlan& plan) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  std::vector<const uint8_t*> in(c.k);
+  std::vector<uint8_t*> out(c.m);
+  for (size_t i = 0; i < c.k; ++i) in[i] = shards[i].ptr;
+  for (size_t j = 0; j < c.m; ++j) out[j] = const_cast<uint8_t*>(shards[c.k + j].ptr);
+  for (size_t i = 0; i < n; ++i)
+    if (!shards[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  return add_stripe(plan, in, out, (const uint32_t*)c.d_parity_tables.p, shards[0].len, kVerify, flag);
+}
+
+int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
+                             BatchPlan& plan) {
+  Plan p;
+  int rc = make_plan(c, shards, n, goal, nullptr, 0, p);
+  if (rc || p.all_present || p.targets.empty()) return rc;
+  std::shared_ptr<DecodeEntry> e;
+  if ((rc = get_decode_entry(c, shards, p, s, e))) return rc;
+  std::vector<const uint8_t*> in(c.k);
+  std::vector<uint8_t*> out(p.targets.size());
+  for (size_t i = 0; i < c.k; ++i) in[i] = shards[p.valid[i]].ptr;
+  for (size_t t = 0; t < p.targets.size(); ++t) out[t] = shards[p.targets[t]].ptr;
+  if ((rc = add_stripe(plan, in, out, (const uint32_t*)e->tables.p, p.shard_len, kWrite, nullptr)))
+    return rc;
+  plan.keep.push_back(e);
+  for (int t : p.targets) shards[t].present = 1;
   return RSGPU_OK;
 }
 

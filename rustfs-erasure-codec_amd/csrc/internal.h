@@ -87,9 +87,7 @@ struct DecodeEntry {
   DeviceBuffer rows;       // [n_targets][k] elements
   std::vector<int> valid;  // survivor indices
   std::vector<int> targets;
-  hipEvent_t last_use = nullptr;  // signalled after the latest kernel reading it
   size_t bytes = 0;
-  ~DecodeEntry();
 };
 
 struct Codec {

@@ -1,0 +1,61 @@
+// kernel_common.h -- device helpers shared by the matrix-apply kernels:
+// byte-permute GF(2^8) multiply, constant-address-space table reads and the
+// tower-element (de)interleave.
+#pragma once
+#include "internal.h"
+
+namespace rsgpu {
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+struct Sel {
+  uint32_t s0, s1, s2;
+};
+
+__device__ __forceinline__ Sel selectors(uint32_t x) {
+  Sel s;
+  s.s0 = x & 0x07070707u;
+  s.s1 = (x >> 3) & 0x07070707u;
+  s.s2 = (x >> 6) & 0x03030303u;
+  return s;
+}
+
+// Coefficient tables are read through the constant address space: the loads are
+// then known invariant and, being wave-uniform, become scalar (s_load) loads
+// into SGPRs instead of vector loads that would share vmcnt with the shard data.
+typedef const __attribute__((address_space(4))) uint32_t* ctab_t;
+
+__device__ __forceinline__ ctab_t as_const(const uint32_t* p) { return (ctab_t)p; }
+
+// c * x for the four bytes of x, with c's table t[0..4].
+__device__ __forceinline__ uint32_t gmul(ctab_t t, const Sel& s) {
+  uint32_t a = __builtin_amdgcn_perm(t[1], t[0], s.s0);
+  uint32_t b = __builtin_amdgcn_perm(t[3], t[2], s.s1);
+  uint32_t c = __builtin_amdgcn_perm(t[4], t[4], s.s2);
+  return xor3(a, b, c);
+}
+
+__device__ __forceinline__ uint4 load16(const uint8_t* p) {
+  return *reinterpret_cast<const uint4*>(p);
+}
+__device__ __forceinline__ void store16(uint8_t* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
+
+// Tower elements [x, c] interleaved in a 16-byte vector -> planes (X0, X1, C0, C1).
+__device__ __forceinline__ void deinterleave(uint4 v, uint32_t* w) {
+  w[0] = __builtin_amdgcn_perm(v.y, v.x, 0x06040200u);
+  w[2] = __builtin_amdgcn_perm(v.y, v.x, 0x07050301u);
+  w[1] = __builtin_amdgcn_perm(v.w, v.z, 0x06040200u);
+  w[3] = __builtin_amdgcn_perm(v.w, v.z, 0x07050301u);
+}
+__device__ __forceinline__ uint4 interleave(const uint32_t* w) {
+  uint4 v;
+  v.x = __builtin_amdgcn_perm(w[2], w[0], 0x05010400u);
+  v.y = __builtin_amdgcn_perm(w[2], w[0], 0x07030602u);
+  v.z = __builtin_amdgcn_perm(w[3], w[1], 0x05010400u);
+  v.w = __builtin_amdgcn_perm(w[3], w[1], 0x07030602u);
+  return v;
+}
+
+}  // namespace rsgpu

@@ -328,6 +328,77 @@ class ReedSolomon:
         return dict(zip(["requests", "hits", "misses", "entries"], [v.value for v in vals]))
 
 
+_lib.rsgpu_batch_new.argtypes = [C.POINTER(_vp)]
+_lib.rsgpu_batch_free.argtypes = [_vp]
+_lib.rsgpu_batch_clear.argtypes = [_vp]
+_lib.rsgpu_batch_add_encode.argtypes = [_vp, _vp, C.POINTER(Slice), _sz]
+_lib.rsgpu_batch_add_verify.argtypes = [_vp, _vp, C.POINTER(CSlice), _sz, _vp]
+_lib.rsgpu_batch_add_reconstruct.argtypes = [_vp, _vp, C.POINTER(Slot), _sz, C.c_int32, _vp]
+_lib.rsgpu_batch_set_tune.argtypes = [_vp, C.c_int32, C.c_int32]
+_lib.rsgpu_batch_autotune.argtypes = [_vp, _vp, C.c_int32, _i32p, _i32p, C.POINTER(C.c_float)]
+_lib.rsgpu_batch_run.argtypes = [_vp, _vp]
+_lib.rsgpu_batch_graph_launch.argtypes = [_vp, _vp]
+_lib.rsgpu_batch_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
+
+
+class Batch:
+    """Many stripe operations executed by one persistent kernel launch."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(_lib.rsgpu_batch_new(C.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rsgpu_batch_free(self._h)
+            self._h = None
+
+    def clear(self):
+        self._keep = []
+        _check(_lib.rsgpu_batch_clear(self._h))
+
+    def add_encode(self, rs, shards):
+        arr = (Slice * len(shards))(*[Slice(*_ptr_len(s)) for s in shards])
+        _check(_lib.rsgpu_batch_add_encode(self._h, rs._h, arr, len(shards)))
+        self._keep.append(shards)
+
+    def add_verify(self, rs, shards, device_flag):
+        arr = (CSlice * len(shards))(*[CSlice(*_ptr_len(s)) for s in shards])
+        _check(_lib.rsgpu_batch_add_verify(self._h, rs._h, arr, len(shards), device_flag.data_ptr()))
+        self._keep.append((shards, device_flag))
+
+    def add_reconstruct(self, rs, slots, data_only=False, stream=None):
+        """slots: list of ShardSlot (buffers for missing shards must be allocated)."""
+        arr = (Slot * len(slots))(*[Slot(*_ptr_len(s.data), int(s.present)) for s in slots])
+        _check(_lib.rsgpu_batch_add_reconstruct(self._h, rs._h, arr, len(slots), int(data_only),
+                                                _stream(stream)))
+        for s, a in zip(slots, arr):
+            s.present = bool(a.present)
+        self._keep.append(slots)
+
+    def set_tune(self, vectors_per_lane, blocks_per_cu):
+        _check(_lib.rsgpu_batch_set_tune(self._h, vectors_per_lane, blocks_per_cu))
+
+    def autotune(self, reps=3, stream=None):
+        nv, bpc, ms = C.c_int32(), C.c_int32(), C.c_float()
+        _check(_lib.rsgpu_batch_autotune(self._h, _stream(stream), reps, C.byref(nv), C.byref(bpc),
+                                         C.byref(ms)))
+        return {"vectors_per_lane": nv.value, "blocks_per_cu": bpc.value, "ms": ms.value}
+
+    def run(self, stream=None):
+        _check(_lib.rsgpu_batch_run(self._h, _stream(stream)))
+
+    def graph_launch(self, stream=None):
+        _check(_lib.rsgpu_batch_graph_launch(self._h, _stream(stream)))
+
+    def stats(self):
+        vals = [C.c_uint64() for _ in range(4)]
+        _check(_lib.rsgpu_batch_stats(self._h, *[C.byref(v) for v in vals]))
+        return dict(zip(["ops", "bytes_moved", "data_bytes", "tiles"], [v.value for v in vals]))
+
+
 def apply_rows(rows, inputs, outputs, field="gf8", stream=None):
     """out_j = XOR_i rows[j][i] * in_i (SelectedEncodeKernel::execute semantics)."""
     rows = np.asarray(rows)

@@ -1,0 +1,132 @@
+"""GPU parity of the persistent ragged-batch kernel: every stripe of a mixed batch
+(shapes, shard sizes, encode/verify/reconstruct, erasures) equals the oracle,
+also when the batch is replayed from a hipGraph and after autotuning."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import announce, to_host
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+rsgpu = pytest.importorskip("rsgpu")
+
+SHAPES = [(4, 2), (6, 3), (8, 3), (8, 4), (10, 4), (12, 4), (16, 4), (20, 4)]
+
+
+def _mixed_batch(seed, n_stripes, max_len=1 << 16, tails=True):
+    rng = np.random.default_rng(seed)
+    codecs = {s: rsgpu.ReedSolomon(*s) for s in SHAPES}
+    mats = {s: O.build_matrix(O.GF8, O.VANDERMONDE, *s)[1] for s in SHAPES}
+    items = []
+    for i in range(n_stripes):
+        k, m = SHAPES[rng.integers(len(SHAPES))]
+        length = int(rng.integers(1, max_len // 64 + 1)) * 64
+        if tails and i % 7 == 3:
+            length += int(rng.integers(1, 16))
+        op = ["encode", "encode", "encode", "verify", "reconstruct"][rng.integers(5)]
+        data = [rng.integers(0, 256, length, dtype=np.uint8) for _ in range(k)]
+        parity = O.encode(O.GF8, mats[(k, m)], k, m, data)
+        items.append(dict(k=k, m=m, length=length, op=op, data=data, parity=parity))
+    return rng, codecs, mats, items
+
+
+def test_mixed_batch_matches_oracle_and_graph_replays():
+    announce("mixed batch 120 stripes")
+    rng, codecs, mats, items = _mixed_batch(1, 120)
+    batch = rsgpu.Batch()
+    flags = torch.zeros(len(items), dtype=torch.int32, device="cuda")
+    views = []
+    for idx, it in enumerate(items):
+        k, m, n = it["k"], it["m"], it["length"]
+        rs = codecs[(k, m)]
+        full = it["data"] + it["parity"]
+        if it["op"] == "encode":
+            sh = [torch.from_numpy(d).cuda() for d in it["data"]] + \
+                 [torch.zeros(n, dtype=torch.uint8, device="cuda") for _ in range(m)]
+            batch.add_encode(rs, sh)
+            views.append(("encode", sh, full, k))
+        elif it["op"] == "verify":
+            sh = [torch.from_numpy(x).cuda() for x in full]
+            corrupt = idx % 2 == 0
+            if corrupt:
+                sh[k + int(rng.integers(m))][int(rng.integers(n))] ^= 0x5A
+            batch.add_verify(rs, sh, flags[idx:idx + 1])
+            views.append(("verify", idx, corrupt))
+        else:
+            e = int(rng.integers(1, m + 1))
+            missing = set(rng.choice(k + m, e, replace=False).tolist())
+            slots = [rsgpu.ShardSlot(torch.from_numpy(x).cuda() if i not in missing
+                                     else torch.zeros(n, dtype=torch.uint8, device="cuda"),
+                                     i not in missing) for i, x in enumerate(full)]
+            batch.add_reconstruct(rs, slots)
+            views.append(("reconstruct", slots, full, missing))
+
+    def check():
+        torch.cuda.synchronize()
+        for v in views:
+            if v[0] in ("encode", "reconstruct"):
+                tensors = v[1] if v[0] == "encode" else [s.data for s in v[1]]
+                for i, t in enumerate(tensors):
+                    assert (to_host(t) == v[2][i]).all(), (v[0], i)
+            else:
+                assert (int(flags[v[1]]) != 0) == v[2], v
+
+    batch.run()
+    check()
+    # graph replays, with eager work and host syncs between them; every replay re-checked
+    for rep in range(3):
+        for v in views:  # clear every output so that the replay must rewrite it
+            if v[0] == "encode":
+                for t in v[1][v[3]:]:
+                    t.zero_()
+            elif v[0] == "reconstruct":
+                for i in v[3]:
+                    v[1][i].data.zero_()
+        flags.zero_()
+        batch.graph_launch()
+        check()
+        torch.cuda.synchronize()
+
+
+def test_autotune_keeps_outputs_bit_exact():
+    announce("batch autotune")
+    _, codecs, mats, items = _mixed_batch(2, 40, max_len=1 << 15, tails=False)
+    batch = rsgpu.Batch()
+    outs = []
+    for it in items:
+        k, m, n = it["k"], it["m"], it["length"]
+        sh = [torch.from_numpy(d).cuda() for d in it["data"]] + \
+             [torch.zeros(n, dtype=torch.uint8, device="cuda") for _ in range(m)]
+        batch.add_encode(codecs[(k, m)], sh)
+        outs.append((sh, it))
+    best = batch.autotune(reps=2)
+    assert best["vectors_per_lane"] in (1, 2)
+    for sh, it in outs:
+        for t in sh[it["k"]:]:
+            t.zero_()
+    batch.run()
+    torch.cuda.synchronize()
+    for sh, it in outs:
+        for j in range(it["m"]):
+            assert (to_host(sh[it["k"] + j]) == it["parity"][j]).all()
+    st = batch.stats()
+    assert st["ops"] == len(items)
+
+
+def test_batch_with_tower_and_leopard_codecs_defers():
+    announce("batch deferred tower op")
+    k, m, n = 5, 3, 256
+    rs16 = rsgpu.ReedSolomon(k, m, field="gf16")
+    _, omat = O.build_matrix(O.GF16, O.VANDERMONDE, k, m)
+    rng = np.random.default_rng(3)
+    data = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k)]
+    sh = [torch.from_numpy(d).cuda() for d in data] + [torch.zeros(n, dtype=torch.uint8, device="cuda")
+                                                       for _ in range(m)]
+    b = rsgpu.Batch()
+    b.add_encode(rs16, sh)
+    b.run()
+    torch.cuda.synchronize()
+    exp = O.encode(O.GF16, omat, k, m, data)
+    assert all((to_host(sh[k + j]) == exp[j]).all() for j in range(m))
