@@ -43,12 +43,18 @@ def parse():
 
 
 class Stripes:
-    """n stripes of k+m shards in one device allocation, with prebuilt C-ABI views."""
+    """n stripes of k+m shards in one device allocation, with prebuilt C-ABI views.
 
-    def __init__(self, rsgpu, torch, k, m, n, shard, seed):
+    Shards sit at a pitch of shard + `pad` bytes.  An exact power-of-two pitch puts the
+    k+m streams of a column on the same HBM channel bits; independent allocations
+    would not, so the default pitch adds 64 KiB + 256 B (both pitches are reported)."""
+
+    def __init__(self, rsgpu, torch, k, m, n, shard, seed, pad=65536 + 256):
         self.k, self.m, self.n, self.shard = k, m, n, shard
+        self.pitch = shard + pad
         g = torch.Generator(device="cuda").manual_seed(seed)
-        self.buf = torch.empty((n, k + m, shard), dtype=torch.uint8, device="cuda")
+        self.store = torch.empty((n, k + m, self.pitch), dtype=torch.uint8, device="cuda")
+        self.buf = self.store[:, :, :shard]
         self.buf[:, :k].random_(0, 256, generator=g)
         self.rs = rsgpu.ReedSolomon(k, m)
         self.lib = rsgpu.lib()
@@ -60,7 +66,7 @@ class Stripes:
             self.ver.append((rsgpu.CSlice * (k + m))(*[rsgpu.CSlice(p, shard) for p in ptrs]))
 
     def ptr(self, s, i):
-        return self.buf.data_ptr() + (s * (self.k + self.m) + i) * self.shard
+        return self.store.data_ptr() + (s * (self.k + self.m) + i) * self.pitch
 
     def _ok(self, rc):
         if rc:
@@ -224,8 +230,16 @@ def main():
     assert torch.equal(ref, st.buf), "reconstruct changed stripe bytes"
     del ref
 
+    # ---- the same encode batch with shards at an exact 1 MiB pitch
+    del b_rec
+    st_p2 = Stripes(rsgpu, torch, k, m, N, S, seed=5, pad=0)
+    b_p2 = st_p2.batch("encode", sp)
+    b_p2.autotune(reps=2)
+    t_enc_p2 = timed(torch, b_p2.graph_launch, args.steps, 1)
+    del b_p2, st_p2
+
     # ---- 8+3
-    del b_enc, b_ver, b_rec
+    del b_enc, b_ver
     st83 = Stripes(rsgpu, torch, 8, 3, N, S, seed=83)
     t83_plain = timed(torch, lambda: [st83.encode(s, sp) for s in range(N)], max(1, args.steps // 2), 1)
     b83 = st83.batch("encode", sp)
@@ -252,10 +266,12 @@ def main():
         "config": {"workload": f"RS 10+4 GF(2^8) encode of {N} stripes x {S // 1024} KiB shards, "
                                "device-resident, batched: one persistent launch per step "
                                "(hipGraph replay); plain per-stripe launches reported alongside",
-                   "stripes": N, "shard_bytes": S, "data_shards": k, "parity_shards": m},
+                   "stripes": N, "shard_bytes": S, "data_shards": k, "parity_shards": m,
+                   "shard_pitch_bytes": st.pitch},
         "encode_batched_gbs": gbs(data_bytes, t_enc_batch),
         "encode_plain_gbs": gbs(data_bytes, t_enc_plain),
         "encode_batched_bytes_moved_gbs": gbs(N * (k + m) * S, t_enc_batch),
+        "encode_batched_gbs_pow2_pitch": gbs(data_bytes, t_enc_p2),
         "encode_batched_vs_plain": round(t_enc_plain / t_enc_batch, 2),
         "verify_batched_gbs": gbs(data_bytes, t_ver_batch),
         "verify_plain_gbs": gbs(data_bytes, t_ver_plain),

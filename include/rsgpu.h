@@ -176,11 +176,13 @@ int rsgpu_batch_add_verify(rsgpu_batch* batch, const rsgpu_codec* codec, const r
                            size_t n_shards, int32_t* device_flag);
 int rsgpu_batch_add_reconstruct(rsgpu_batch* batch, const rsgpu_codec* codec, rsgpu_slot* shards,
                                 size_t n_shards, int32_t data_only, void* stream);
-/* Tile shape: vectors of 16 bytes per lane (1 or 2) and workgroups per CU. */
-int rsgpu_batch_set_tune(rsgpu_batch* batch, int32_t vectors_per_lane, int32_t blocks_per_cu);
-/* Times every tile shape on this batch (`reps` runs each) and keeps the fastest. */
-int rsgpu_batch_autotune(rsgpu_batch* batch, void* stream, int32_t reps, int32_t* vectors_per_lane,
-                         int32_t* blocks_per_cu, float* ms);
+/* Kernel shape: inputs per pipelined load group (4, 6, 8 or 12), 16-byte vectors
+ * per lane (1 or 2; 1 for 12-input groups) and resident workgroups per CU (1..64). */
+int rsgpu_batch_set_tune(rsgpu_batch* batch, int32_t inputs_per_group, int32_t vectors_per_lane,
+                         int32_t blocks_per_cu);
+/* Times every kernel shape on this batch (`reps` runs each) and keeps the fastest. */
+int rsgpu_batch_autotune(rsgpu_batch* batch, void* stream, int32_t reps, int32_t* inputs_per_group,
+                         int32_t* vectors_per_lane, int32_t* blocks_per_cu, float* ms);
 int rsgpu_batch_run(rsgpu_batch* batch, void* stream);
 /* Same work, captured once into a hipGraph and replayed on later calls. */
 int rsgpu_batch_graph_launch(rsgpu_batch* batch, void* stream);

@@ -17,10 +17,15 @@ struct BatchStripe {
   uint16_t mode, tower;
 };
 
-// Tile shape of the persistent kernel: vectors per lane (nv) and workgroups per CU.
+// Shape of the persistent kernel: inputs per pipelined load group (kb), 16-byte
+// vectors per lane (nv; tiles are kBlock * nv vectors per shard), resident
+// workgroups per CU, and the accumulator rows compiled in (4 or 8, from the
+// widest descriptor in the batch).
 struct BatchTune {
+  int kb = 8;
   int nv = 1;
   int blocks_per_cu = 8;
+  int rmax = 4;
 };
 
 struct BatchPlan {

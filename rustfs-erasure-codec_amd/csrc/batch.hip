@@ -3,12 +3,13 @@
 // Thousands of classic GF(2^8) stripe operations (encode, verify, reconstruct;
 // any mix of k, r and shard length) are flattened into one table of stripe
 // descriptors plus one table of shard pointers in device memory.  Every stripe
-// is cut into tiles of kBlock * NV 16-byte vectors per shard; the tiles of all
+// is cut into tiles of kBlock 16-byte vectors per shard; the tiles of all
 // stripes form one global index space that a persistent grid strides through.
 // A workgroup maps its tile to the stripe by a wave-uniform binary search over
 // the stripes' first-tile offsets (scalar loads), then runs the same
 // register-accumulator matrix apply as the single-stripe kernel with runtime
-// k and r (r <= 8 outputs per descriptor).  The launch sequence (decode solves
+// k and r (r <= 8 outputs per descriptor), software-pipelined so that the
+// loads of the next input group are in flight while the current one computes.  The launch sequence (decode solves
 // are enqueued when an op is added; the run is a single kernel) can be captured
 // in a hipGraph and replayed.
 #include <algorithm>
@@ -23,86 +24,213 @@ namespace rsgpu {
 typedef const __attribute__((address_space(4))) BatchStripe* cstripe_t;
 typedef const __attribute__((address_space(4))) uint64_t* cptr_t;
 
+// Wave-uniform position of a workgroup in the global chunk sequence: a chunk is
+// one group of KB inputs of one tile of one stripe.
+struct Cursor {
+  uint64_t g;      // global tile
+  uint64_t vtile;  // first 16-byte vector of the tile within its stripe
+  uint64_t n16;
+  ctab_t tables;
+  uint32_t p0, stride;
+  int k, R, mode, stripe, i0;
+};
+
 template <int NV>
+__device__ __forceinline__ bool cursor_at(Cursor& c, uint64_t g, cstripe_t stripes, int n_stripes,
+                                          uint64_t total_tiles) {
+  if (g >= total_tiles) return false;
+  int lo = 0, hi = n_stripes - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (stripes[mid].tile0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  c.g = g;
+  c.stripe = lo;
+  c.vtile = (g - stripes[lo].tile0) * (uint64_t)(kBlock * NV);
+  c.n16 = stripes[lo].n16;
+  c.tables = as_const(stripes[lo].tables);
+  c.p0 = stripes[lo].ptr0;
+  c.stride = stripes[lo].tab_row_stride;
+  c.k = stripes[lo].k;
+  c.R = stripes[lo].r;
+  c.mode = stripes[lo].mode;
+  c.i0 = 0;
+  return true;
+}
+
+template <int KB, int NV>
+__device__ __forceinline__ bool cursor_next(Cursor& c, cstripe_t stripes, int n_stripes,
+                                            uint64_t total_tiles) {
+  if (c.i0 + KB < c.k) {
+    c.i0 += KB;
+    return true;
+  }
+  return cursor_at<NV>(c, c.g + gridDim.x, stripes, n_stripes, total_tiles);
+}
+
+template <int KB, int NV>
+__device__ __forceinline__ void issue(uint4 (&x)[KB][NV], const Cursor& c, cptr_t ptrs) {
+#pragma unroll
+  for (int b = 0; b < KB; ++b) {
+    const int i = c.i0 + b;
+    const uint8_t* src = (const uint8_t*)ptrs[c.p0 + (i < c.k ? i : 0)];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const uint64_t v = c.vtile + j * kBlock + threadIdx.x;
+      x[b][j] = (i < c.k && v < c.n16) ? load16(src + v * 16) : make_uint4(0, 0, 0, 0);
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t word(const uint4& v, int d) {
+  return d == 0 ? v.x : d == 1 ? v.y : d == 2 ? v.z : v.w;
+}
+
+// Inputs are taken in pairs so that the six byte lookups of two coefficients
+// fold into the accumulator with three 3-input XORs.
+template <int KB, int RMAX, int NV>
+__device__ __forceinline__ void compute(const uint4 (&x)[KB][NV], const Cursor& c,
+                                        uint32_t (&acc)[RMAX][NV][4]) {
+#pragma unroll
+  for (int b = 0; b < KB; b += 2) {
+    const bool has0 = c.i0 + b < c.k, has1 = c.i0 + b + 1 < c.k;  // uniform
+    if (!has0) continue;
+    Sel s0[NV][4], s1[NV][4];
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        s0[j][d] = selectors(word(x[b][j], d));
+        s1[j][d] = selectors(word(x[b + 1][j], d));
+      }
+    ctab_t ta_col = c.tables + (size_t)(c.i0 + b) * kTabWords;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      if (r >= c.R) continue;
+      ctab_t ta = ta_col + (size_t)r * c.stride;
+      ctab_t tb = ta + kTabWords;
+      if (has1) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            uint32_t p0 = __builtin_amdgcn_perm(ta[1], ta[0], s0[j][d].s0);
+            uint32_t p1 = __builtin_amdgcn_perm(ta[3], ta[2], s0[j][d].s1);
+            uint32_t p2 = __builtin_amdgcn_perm(ta[4], ta[4], s0[j][d].s2);
+            uint32_t q0 = __builtin_amdgcn_perm(tb[1], tb[0], s1[j][d].s0);
+            uint32_t q1 = __builtin_amdgcn_perm(tb[3], tb[2], s1[j][d].s1);
+            uint32_t q2 = __builtin_amdgcn_perm(tb[4], tb[4], s1[j][d].s2);
+            uint32_t t = xor3(acc[r][j][d], p0, p1);
+            t = xor3(t, p2, q0);
+            acc[r][j][d] = xor3(t, q1, q2);
+          }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            uint32_t p0 = __builtin_amdgcn_perm(ta[1], ta[0], s0[j][d].s0);
+            uint32_t p1 = __builtin_amdgcn_perm(ta[3], ta[2], s0[j][d].s1);
+            uint32_t p2 = __builtin_amdgcn_perm(ta[4], ta[4], s0[j][d].s2);
+            acc[r][j][d] = xor3(acc[r][j][d], p0, p1) ^ p2;
+          }
+      }
+    }
+  }
+}
+
+// Writes (or compares) the accumulators when the chunk closes its tile.
+template <int KB, int RMAX, int NV>
+__device__ __forceinline__ void finish(const Cursor& c, uint32_t (&acc)[RMAX][NV][4], cptr_t ptrs,
+                                       const BatchStripe* stripes_g) {
+  if (c.i0 + KB < c.k) return;
+  bool mismatch = false;
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r) {
+    if (r >= c.R) continue;
+    uint8_t* base = (uint8_t*)ptrs[c.p0 + c.k + r];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const uint64_t v = c.vtile + j * kBlock + threadIdx.x;
+      if (v >= c.n16) continue;
+      uint8_t* dst = base + v * 16;
+      uint4 o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
+      if (c.mode == kWrite) {
+        store16(dst, o);
+      } else {
+        uint4 p = load16(dst);
+        mismatch |= ((p.x ^ o.x) | (p.y ^ o.y) | (p.z ^ o.z) | (p.w ^ o.w)) != 0;
+      }
+    }
+  }
+  if (mismatch) atomicOr(stripes_g[c.stripe].flag, 1);
+#pragma unroll
+  for (int r = 0; r < RMAX; ++r)
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
+}
+
+// Software-pipelined persistent kernel: while a wave computes one chunk (KB
+// inputs of its current tile) the 128-bit loads of its next chunk -- the next
+// input group, or the first group of its next tile, possibly in another
+// stripe -- are already in flight in the second register buffer.
+template <int KB, int RMAX, int NV>
 __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchStripe* stripes_g, int n_stripes,
                                                        const void* const* ptrs_g,
                                                        uint64_t total_tiles) {
-  constexpr int KB = NV == 1 ? 16 : 8;
   cstripe_t stripes = (cstripe_t)stripes_g;
   cptr_t ptrs = (cptr_t)ptrs_g;
-  const uint64_t tile = (uint64_t)kBlock * NV;
-  for (uint64_t g = blockIdx.x; g < total_tiles; g += gridDim.x) {
-    int lo = 0, hi = n_stripes - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (stripes[mid].tile0 <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    const uint64_t n16 = stripes[lo].n16;
-    const int k = stripes[lo].k, R = stripes[lo].r, mode = stripes[lo].mode;
-    const uint32_t p0 = stripes[lo].ptr0, stride = stripes[lo].tab_row_stride;
-    ctab_t tables = as_const(stripes[lo].tables);
-    const uint64_t t = g - stripes[lo].tile0;
-    uint64_t v[NV];
-    bool ok[NV];
+  Cursor cur;
+  if (!cursor_at<NV>(cur, blockIdx.x, stripes, n_stripes, total_tiles)) return;
+  uint32_t acc[RMAX][NV][4];
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      v[j] = t * tile + (uint64_t)j * kBlock + threadIdx.x;
-      ok[j] = v[j] < n16;
-      v[j] *= 16;
-    }
-    uint32_t acc[kMaxOut][NV][4];
+  for (int r = 0; r < RMAX; ++r)
 #pragma unroll
-    for (int r = 0; r < kMaxOut; ++r)
+    for (int j = 0; j < NV; ++j)
 #pragma unroll
-      for (int j = 0; j < NV; ++j)
-#pragma unroll
-        for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
-    for (int i0 = 0; i0 < k; i0 += KB) {
-      uint4 x[KB][NV];
-#pragma unroll
-      for (int b = 0; b < KB; ++b) {
-        const uint8_t* src = (const uint8_t*)ptrs[p0 + ((i0 + b < k) ? i0 + b : 0)];
-#pragma unroll
-        for (int j = 0; j < NV; ++j)
-          x[b][j] = (i0 + b < k && ok[j]) ? load16(src + v[j]) : make_uint4(0, 0, 0, 0);
-      }
-#pragma unroll
-      for (int b = 0; b < KB; ++b) {
-        if (i0 + b >= k) continue;  // uniform; keeps the loop fully unrollable
-        ctab_t tcol = tables + (size_t)(i0 + b) * kTabWords;
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-          Sel s[4] = {selectors(x[b][j].x), selectors(x[b][j].y), selectors(x[b][j].z),
-                      selectors(x[b][j].w)};
-#pragma unroll
-          for (int r = 0; r < kMaxOut; ++r) {
-            if (r >= R) continue;
-            ctab_t tc = tcol + (size_t)r * stride;
-#pragma unroll
-            for (int d = 0; d < 4; ++d) acc[r][j][d] ^= gmul(tc, s[d]);
-          }
-        }
-      }
-    }
-    bool mismatch = false;
-#pragma unroll
-    for (int r = 0; r < kMaxOut; ++r) {
-      if (r >= R) continue;
-      uint8_t* dst = (uint8_t*)ptrs[p0 + k + r];
-#pragma unroll
-      for (int j = 0; j < NV; ++j) {
-        if (!ok[j]) continue;
-        uint4 o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
-        if (mode == kWrite) {
-          store16(dst + v[j], o);
-        } else {
-          uint4 p = load16(dst + v[j]);
-          mismatch |= ((p.x ^ o.x) | (p.y ^ o.y) | (p.z ^ o.z) | (p.w ^ o.w)) != 0;
-        }
-      }
-    }
-    if (mismatch) atomicOr(stripes_g[lo].flag, 1);
+      for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
+  uint4 xa[KB][NV], xb[KB][NV];
+  issue<KB, NV>(xa, cur, ptrs);
+  while (true) {
+    Cursor nxt = cur;
+    bool more = cursor_next<KB, NV>(nxt, stripes, n_stripes, total_tiles);
+    if (more) issue<KB, NV>(xb, nxt, ptrs);
+    compute<KB, RMAX, NV>(xa, cur, acc);
+    finish<KB, RMAX, NV>(cur, acc, ptrs, stripes_g);
+    if (!more) break;
+    cur = nxt;
+    nxt = cur;
+    more = cursor_next<KB, NV>(nxt, stripes, n_stripes, total_tiles);
+    if (more) issue<KB, NV>(xa, nxt, ptrs);
+    compute<KB, RMAX, NV>(xb, cur, acc);
+    finish<KB, RMAX, NV>(cur, acc, ptrs, stripes_g);
+    if (!more) break;
+    cur = nxt;
+  }
+}
+
+template <int KB, int RMAX>
+static void launch_nv(int nv, unsigned grid, hipStream_t s, const BatchStripe* d_stripes,
+                      int n_stripes, const void* const* d_ptrs, uint64_t total_tiles) {
+  if (nv == 2)
+    hipLaunchKernelGGL((batch_kernel<KB, RMAX, 2>), dim3(grid), dim3(kBlock), 0, s, d_stripes,
+                       n_stripes, d_ptrs, total_tiles);
+  else
+    hipLaunchKernelGGL((batch_kernel<KB, RMAX, 1>), dim3(grid), dim3(kBlock), 0, s, d_stripes,
+                       n_stripes, d_ptrs, total_tiles);
+}
+
+template <int RMAX>
+static void launch_kb(int kb, int nv, unsigned grid, hipStream_t s, const BatchStripe* d_stripes,
+                      int n_stripes, const void* const* d_ptrs, uint64_t total_tiles) {
+  switch (kb) {
+    case 4: launch_nv<4, RMAX>(nv, grid, s, d_stripes, n_stripes, d_ptrs, total_tiles); break;
+    case 6: launch_nv<6, RMAX>(nv, grid, s, d_stripes, n_stripes, d_ptrs, total_tiles); break;
+    case 12: launch_nv<12, RMAX>(1, grid, s, d_stripes, n_stripes, d_ptrs, total_tiles); break;
+    default: launch_nv<8, RMAX>(nv, grid, s, d_stripes, n_stripes, d_ptrs, total_tiles);
   }
 }
 
@@ -111,12 +239,10 @@ hipError_t launch_batch(const BatchStripe* d_stripes, int n_stripes, const void*
   if (total_tiles == 0 || n_stripes == 0) return hipSuccess;
   uint64_t grid = (uint64_t)n_cu * tune.blocks_per_cu;
   if (grid > total_tiles) grid = total_tiles;
-  if (tune.nv == 2)
-    hipLaunchKernelGGL(batch_kernel<2>, dim3((unsigned)grid), dim3(kBlock), 0, s, d_stripes,
-                       n_stripes, d_ptrs, total_tiles);
+  if (tune.rmax <= 4)
+    launch_kb<4>(tune.kb, tune.nv, (unsigned)grid, s, d_stripes, n_stripes, d_ptrs, total_tiles);
   else
-    hipLaunchKernelGGL(batch_kernel<1>, dim3((unsigned)grid), dim3(kBlock), 0, s, d_stripes,
-                       n_stripes, d_ptrs, total_tiles);
+    launch_kb<8>(tune.kb, tune.nv, (unsigned)grid, s, d_stripes, n_stripes, d_ptrs, total_tiles);
   return hipGetLastError();
 }
 
@@ -150,6 +276,7 @@ struct rsgpu_batch {
   ~rsgpu_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (graph_stream) (void)hipStreamDestroy(graph_stream);
   }
 };
 
@@ -170,13 +297,17 @@ static void invalidate_graph(rsgpu_batch* b) {
 }
 
 static void retile(rsgpu_batch* b) {
+  if (b->tune.kb == 12) b->tune.nv = 1;  // the 12-input group is built for one vector per lane
   const uint64_t tile = (uint64_t)kBlock * b->tune.nv;
   uint64_t t0 = 0;
+  int rmax = 1;
   for (auto& s : b->plan.stripes) {
     s.tile0 = t0;
     t0 += (s.n16 + tile - 1) / tile;
+    rmax = std::max<int>(rmax, s.r);
   }
   b->plan.total_tiles = t0;
+  b->tune.rmax = rmax <= 4 ? 4 : 8;
 }
 
 static int upload(rsgpu_batch* b, hipStream_t s) {
@@ -308,11 +439,15 @@ int rsgpu_batch_add_reconstruct(rsgpu_batch* b, const rsgpu_codec* c, rsgpu_slot
   return rc;
 }
 
-int rsgpu_batch_set_tune(rsgpu_batch* b, int32_t nv, int32_t blocks_per_cu) {
-  if (!b || (nv != 1 && nv != 2) || blocks_per_cu < 1 || blocks_per_cu > 64)
+int rsgpu_batch_set_tune(rsgpu_batch* b, int32_t inputs_per_group, int32_t vectors_per_lane,
+                         int32_t blocks_per_cu) {
+  const int kb = inputs_per_group, nv = vectors_per_lane;
+  if (!b || (kb != 4 && kb != 6 && kb != 8 && kb != 12) || (nv != 1 && nv != 2) ||
+      (kb == 12 && nv != 1) || blocks_per_cu < 1 || blocks_per_cu > 64)
     return RSGPU_ERR_INVALID_ARGUMENT;
   if (nv != b->tune.nv) b->dirty = true;
   b->tune.nv = nv;
+  b->tune.kb = kb;
   b->tune.blocks_per_cu = blocks_per_cu;
   invalidate_graph(b);
   return RSGPU_OK;
@@ -333,11 +468,24 @@ int rsgpu_batch_graph_launch(rsgpu_batch* b, void* stream) {
   int rc = upload(b, s);
   if (rc) return rc;
   if (!b->deferred.empty()) return rsgpu_batch_run(b, stream);  // deferred ops sync on the host
-  if (!b->exec || b->graph_stream != s) {
-    invalidate_graph(b);
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return RSGPU_ERR_HIP;
-    rc = launch_main(b, s);
-    hipError_t e = hipStreamEndCapture(s, &b->graph);
+  if (!b->exec) {
+    // Capture on the batch's own stream (the legacy default stream cannot be
+    // captured); the instantiated graph is then launched on the caller's stream.
+    if (!b->graph_stream) {
+      hipError_t e = hipStreamCreateWithFlags(&b->graph_stream, hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        set_hip_error(e, "hipStreamCreateWithFlags");
+        return RSGPU_ERR_HIP;
+      }
+    }
+    hipStream_t cs = b->graph_stream;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      set_hip_error(e, "hipStreamBeginCapture");
+      return RSGPU_ERR_HIP;
+    }
+    rc = launch_main(b, cs);
+    e = hipStreamEndCapture(cs, &b->graph);
     if (rc) return rc;
     if (e != hipSuccess) {
       set_hip_error(e, "hipStreamEndCapture");
@@ -348,7 +496,6 @@ int rsgpu_batch_graph_launch(rsgpu_batch* b, void* stream) {
       set_hip_error(e, "hipGraphInstantiate");
       return RSGPU_ERR_HIP;
     }
-    b->graph_stream = s;
   }
   hipError_t e = hipGraphLaunch(b->exec, s);
   if (e != hipSuccess) {
@@ -375,20 +522,20 @@ int rsgpu_batch_stats(const rsgpu_batch* b, uint64_t* n_ops, uint64_t* bytes_mov
 
 // Exhaustive tile-shape search: every candidate is bit-exact, so the fastest
 // (device time over `reps` runs) wins.  Leaves the batch tuned to the winner.
-int rsgpu_batch_autotune(rsgpu_batch* b, void* stream, int32_t reps, int32_t* best_nv,
-                         int32_t* best_blocks_per_cu, float* best_ms) {
+int rsgpu_batch_autotune(rsgpu_batch* b, void* stream, int32_t reps, int32_t* best_kb,
+                         int32_t* best_nv, int32_t* best_blocks_per_cu, float* best_ms) {
   if (!b || reps < 1) return RSGPU_ERR_INVALID_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
-  static const int nvs[] = {1, 2};
-  static const int bpcs[] = {2, 4, 8, 16, 32};
+  static const int shapes[][2] = {{4, 1}, {6, 1}, {8, 1}, {12, 1}, {4, 2}, {6, 2}};
+  static const int bpcs[] = {2, 4, 6, 8, 12, 16};
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return RSGPU_ERR_HIP;
   float best = 1e30f;
   BatchTune winner = b->tune;
   int rc = RSGPU_OK;
-  for (int nv : nvs)
+  for (auto& sh : shapes)
     for (int bpc : bpcs) {
-      if ((rc = rsgpu_batch_set_tune(b, nv, bpc))) break;
+      if ((rc = rsgpu_batch_set_tune(b, sh[0], sh[1], bpc))) break;
       if ((rc = upload(b, s))) break;
       if ((rc = launch_main(b, s))) break;  // warm
       (void)hipEventRecord(e0, s);
@@ -406,7 +553,8 @@ int rsgpu_batch_autotune(rsgpu_batch* b, void* stream, int32_t reps, int32_t* be
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  rsgpu_batch_set_tune(b, winner.nv, winner.blocks_per_cu);
+  rsgpu_batch_set_tune(b, winner.kb, winner.nv, winner.blocks_per_cu);
+  if (best_kb) *best_kb = winner.kb;
   if (best_nv) *best_nv = winner.nv;
   if (best_blocks_per_cu) *best_blocks_per_cu = winner.blocks_per_cu;
   if (best_ms) *best_ms = best;

@@ -334,8 +334,8 @@ _lib.rsgpu_batch_clear.argtypes = [_vp]
 _lib.rsgpu_batch_add_encode.argtypes = [_vp, _vp, C.POINTER(Slice), _sz]
 _lib.rsgpu_batch_add_verify.argtypes = [_vp, _vp, C.POINTER(CSlice), _sz, _vp]
 _lib.rsgpu_batch_add_reconstruct.argtypes = [_vp, _vp, C.POINTER(Slot), _sz, C.c_int32, _vp]
-_lib.rsgpu_batch_set_tune.argtypes = [_vp, C.c_int32, C.c_int32]
-_lib.rsgpu_batch_autotune.argtypes = [_vp, _vp, C.c_int32, _i32p, _i32p, C.POINTER(C.c_float)]
+_lib.rsgpu_batch_set_tune.argtypes = [_vp, C.c_int32, C.c_int32, C.c_int32]
+_lib.rsgpu_batch_autotune.argtypes = [_vp, _vp, C.c_int32, _i32p, _i32p, _i32p, C.POINTER(C.c_float)]
 _lib.rsgpu_batch_run.argtypes = [_vp, _vp]
 _lib.rsgpu_batch_graph_launch.argtypes = [_vp, _vp]
 _lib.rsgpu_batch_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
@@ -378,14 +378,15 @@ class Batch:
             s.present = bool(a.present)
         self._keep.append(slots)
 
-    def set_tune(self, vectors_per_lane, blocks_per_cu):
-        _check(_lib.rsgpu_batch_set_tune(self._h, vectors_per_lane, blocks_per_cu))
+    def set_tune(self, inputs_per_group, vectors_per_lane, blocks_per_cu):
+        _check(_lib.rsgpu_batch_set_tune(self._h, inputs_per_group, vectors_per_lane, blocks_per_cu))
 
     def autotune(self, reps=3, stream=None):
-        nv, bpc, ms = C.c_int32(), C.c_int32(), C.c_float()
-        _check(_lib.rsgpu_batch_autotune(self._h, _stream(stream), reps, C.byref(nv), C.byref(bpc),
-                                         C.byref(ms)))
-        return {"vectors_per_lane": nv.value, "blocks_per_cu": bpc.value, "ms": ms.value}
+        kb, nv, bpc, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
+        _check(_lib.rsgpu_batch_autotune(self._h, _stream(stream), reps, C.byref(kb), C.byref(nv),
+                                         C.byref(bpc), C.byref(ms)))
+        return {"inputs_per_group": kb.value, "vectors_per_lane": nv.value,
+                "blocks_per_cu": bpc.value, "ms": ms.value}
 
     def run(self, stream=None):
         _check(_lib.rsgpu_batch_run(self._h, _stream(stream)))
