@@ -1,8 +1,883 @@
-// leopard.hip -- placeholder until the FFT kernels land.
+// leopard.hip -- Leopard GF(2^8) and GF(2^16) FFT codecs on gfx950.
+//
+// Field tables, FFT skews and the transform schedules follow
+// src/core/leopard_gf8/{tables,mod,encode,decode}.rs and
+// src/core/leopard_gf16/{tables,mod,encode,decode,ops}.rs.
+//
+// Device mapping.  Every Leopard operation is independent per column: a byte
+// (GF8) or a split-layout element (GF16, ops.rs:520-568: in each 64-byte block
+// element i = byte[i] | byte[i+32] << 8).  A workgroup takes a column tile of C
+// "words" -- a dword of 4 bytes (GF8), or the pair (low-byte dword, high-byte
+// dword) of 4 elements (GF16) -- from every work lane, keeps the whole lane x
+// tile matrix in LDS, and runs all transform layers of the plan on it with one
+// barrier per layer.  Items of a layer are (butterfly, word) pairs with the word
+// fastest, so the lanes of a wave share a butterfly's twiddle tables.
+//
+// Multiplication by a fixed element is GF(2)-linear in the element bits (the
+// Cantor-basis representation is a linear change of basis), so it uses the same
+// three-v_perm byte tables as the classic kernels: one table per byte map
+// (GF16: low->low, low->high, high->low, high->high).  The reference's
+// butterfly shortcut for log_m == 2^n - 1 (ops.rs:466-498 GF8, :382-452 GF16)
+// is exactly a multiplication by zero, so such twiddles get an all-zero table.
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "kernel_common.h"
 #include "leopard.h"
+
 namespace rsgpu {
-int init_leopard(Codec&) { return RSGPU_OK; }
-int leopard_encode(const Codec&, const rsgpu_cslice*, const rsgpu_slice*, size_t, hipStream_t) { return RSGPU_ERR_UNSUPPORTED; }
-int leopard_verify(const Codec&, const rsgpu_cslice*, size_t, int32_t*, hipStream_t) { return RSGPU_ERR_UNSUPPORTED; }
-int leopard_reconstruct(const Codec&, rsgpu_slot*, size_t, int, const uint8_t*, size_t, hipStream_t) { return RSGPU_ERR_UNSUPPORTED; }
+
+// ------------------------------------------------------------- host tables
+struct LeoField {
+  bool gf16;
+  unsigned order, modulus;
+  std::vector<uint32_t> log, exp;  // exp has 2*order entries for GF16 (ops.rs index range)
+  std::vector<uint32_t> skew, walsh;
+  uint32_t add_mod(uint32_t a, uint32_t b) const {
+    uint32_t sum = a + b;
+    return (sum + (sum >> (gf16 ? 16 : 8))) & (order - 1);
+  }
+  uint32_t mul_log(uint32_t a, uint32_t log_b) const {
+    if (a == 0) return 0;
+    return exp[add_mod(log[a], log_b)];
+  }
+};
+
+// init_luts8/16 + init_fft_skew8/16 (leopard_gf8/tables.rs:25-104,
+// leopard_gf16/tables.rs:22-116).
+static void build_field(LeoField& F, bool gf16) {
+  F.gf16 = gf16;
+  const unsigned bits = gf16 ? 16 : 8;
+  F.order = 1u << bits;
+  F.modulus = F.order - 1;
+  const uint32_t poly = gf16 ? 0x1002Du : 0x11Du;
+  static const uint32_t cantor8[8] = {1, 214, 152, 146, 86, 200, 88, 230};
+  static const uint32_t cantor16[16] = {0x0001, 0xACCA, 0x3C0E, 0x163E, 0xC582, 0xED2E, 0x914C, 0x4012,
+                                        0x6C98, 0x10D8, 0x6A72, 0xB900, 0xFDB8, 0xFB34, 0xFF38, 0x991E};
+  const uint32_t* cantor = gf16 ? cantor16 : cantor8;
+  F.exp.assign(gf16 ? 2 * F.order : F.order, 0);
+  F.log.assign(F.order, 0);
+  uint32_t state = 1;
+  for (uint32_t i = 0; i < F.modulus; ++i) {
+    F.exp[state] = i;
+    state <<= 1;
+    if (state >= F.order) state ^= poly;
+  }
+  F.exp[0] = F.modulus;
+  F.log[0] = 0;
+  for (unsigned i = 0; i < bits; ++i) {
+    uint32_t width = 1u << i;
+    for (uint32_t j = 0; j < width; ++j) F.log[j + width] = F.log[j] ^ cantor[i];
+  }
+  for (uint32_t i = 0; i < F.order; ++i) F.log[i] = F.exp[F.log[i]];
+  for (uint32_t i = 0; i < F.order; ++i) F.exp[F.log[i]] = i;
+  F.exp[F.modulus] = F.exp[0];
+
+  std::vector<uint32_t> temp(bits - 1);
+  for (unsigned i = 1; i < bits; ++i) temp[i - 1] = 1u << i;
+  F.skew.assign(F.modulus, 0);
+  for (unsigned m = 0; m + 1 < bits; ++m) {
+    uint32_t step = 1u << (m + 1);
+    F.skew[(1u << m) - 1] = 0;
+    for (unsigned i = m; i + 1 < bits; ++i) {
+      uint32_t s = 1u << (i + 1);
+      for (uint32_t j = (1u << m) - 1; j < s; j += step) F.skew[j + s] = F.skew[j] ^ temp[i];
+    }
+    uint32_t prod = F.mul_log(temp[m], F.log[temp[m] ^ 1]);
+    temp[m] = F.modulus - F.log[prod];
+    for (unsigned i = m + 1; i + 1 < bits; ++i) {
+      uint32_t sum = F.add_mod(F.log[temp[i] ^ 1], temp[m]);
+      temp[i] = F.mul_log(temp[i], sum);
+    }
+  }
+  for (uint32_t i = 0; i < F.modulus; ++i) F.skew[i] = F.log[F.skew[i]];
 }
+
+static const LeoField& field8() {
+  static LeoField f;
+  static std::once_flag once;
+  std::call_once(once, [] { build_field(f, false); });
+  return f;
+}
+static const LeoField& field16() {
+  static LeoField f;
+  static std::once_flag once;
+  std::call_once(once, [] { build_field(f, true); });
+  return f;
+}
+
+// Byte-permute tables of "multiply by g^log_m": GF8 one 8-dword table, GF16 four
+// (low->low, low->high, high->low, high->high).  `twiddle` applies the
+// butterfly shortcut (log_m == modulus -> zero).
+constexpr int kLeoTabWords8 = kTabWords;
+constexpr int kLeoTabWords16 = 4 * kTabWords;
+
+static void pack_table(const uint8_t f[256], uint32_t* w) {
+  uint32_t v[5] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < 4; ++i) {
+    v[0] |= (uint32_t)f[i] << (8 * i);
+    v[1] |= (uint32_t)f[i + 4] << (8 * i);
+    v[2] |= (uint32_t)f[8 * i] << (8 * i);
+    v[3] |= (uint32_t)f[8 * (i + 4)] << (8 * i);
+    v[4] |= (uint32_t)f[64 * i] << (8 * i);
+  }
+  for (int i = 0; i < 5; ++i) w[i] = v[i];
+  w[5] = w[6] = w[7] = 0;
+}
+
+static void leo_table(const LeoField& F, uint32_t log_m, bool twiddle, uint32_t* out) {
+  const bool zero = twiddle && log_m == F.modulus;
+  if (!F.gf16) {
+    uint8_t f[256];
+    for (int v = 0; v < 256; ++v) f[v] = zero ? 0 : (uint8_t)F.mul_log((uint32_t)v, log_m);
+    pack_table(f, out);
+    return;
+  }
+  uint8_t ll[256], lh[256], hl[256], hh[256];
+  for (int v = 0; v < 256; ++v) {
+    uint32_t lo = zero ? 0 : F.mul_log((uint32_t)v, log_m);
+    uint32_t hi = zero ? 0 : F.mul_log((uint32_t)v << 8, log_m);
+    ll[v] = lo & 0xff;
+    lh[v] = lo >> 8;
+    hl[v] = hi & 0xff;
+    hh[v] = hi >> 8;
+  }
+  pack_table(ll, out);
+  pack_table(lh, out + kTabWords);
+  pack_table(hl, out + 2 * kTabWords);
+  pack_table(hh, out + 3 * kTabWords);
+}
+
+// ------------------------------------------------------------ transform plans
+struct LeoBlock {
+  uint32_t r, cnt, t01, t23, t02, pad0, pad1, pad2;
+};
+struct LeoLayer {
+  uint32_t dist, n_blocks, block0, cnt, radix, inverse, pad0, pad1;
+};
+struct LeoGroup {
+  uint32_t first, mtrunc, layer0, n_layers;
+};
+
+struct PlanBuilder {
+  const LeoField& F;
+  std::vector<LeoLayer> layers;
+  std::vector<LeoBlock> blocks;
+  std::vector<uint32_t> tables;  // [n][words]
+  std::map<uint32_t, uint32_t> tab_index;
+  explicit PlanBuilder(const LeoField& f) : F(f) {}
+  uint32_t tab(uint32_t log_m) {
+    auto it = tab_index.find(log_m);
+    if (it != tab_index.end()) return it->second;
+    const int W = F.gf16 ? kLeoTabWords16 : kLeoTabWords8;
+    uint32_t idx = (uint32_t)(tables.size() / W);
+    tables.resize(tables.size() + W);
+    leo_table(F, log_m, true, tables.data() + (size_t)idx * W);
+    tab_index[log_m] = idx;
+    return idx;
+  }
+  void begin_layer(uint32_t dist, uint32_t cnt, uint32_t radix, bool inverse) {
+    LeoLayer L{};
+    L.dist = dist;
+    L.cnt = cnt;
+    L.radix = radix;
+    L.inverse = inverse ? 1 : 0;
+    L.block0 = (uint32_t)blocks.size();
+    layers.push_back(L);
+  }
+  void block(uint32_t r, uint32_t cnt, uint32_t l01, uint32_t l23, uint32_t l02) {
+    LeoBlock b{};
+    b.r = r;
+    b.cnt = cnt;
+    b.t01 = tab(l01);
+    b.t23 = tab(l23);
+    b.t02 = tab(l02);
+    blocks.push_back(b);
+    layers.back().n_blocks++;
+  }
+  // IFFT, encoder form (skew offset base; build_ifft_dit8_plan mod.rs:376-447) or
+  // decoder form (full skew with i_end - 1; build_ifft_decode_dit8_plan :455-526).
+  uint32_t ifft(uint32_t mtrunc, uint32_t m, const uint32_t* skew, bool decoder) {
+    const uint32_t first = (uint32_t)layers.size();
+    const uint32_t o = decoder ? 1 : 0;
+    uint32_t dist = 1, dist4 = 4;
+    while (dist4 <= m) {
+      begin_layer(dist, dist, 4, true);
+      for (uint32_t r = 0; r < mtrunc; r += dist4) {
+        uint32_t iend = r + dist;
+        block(r, dist, skew[iend - o], skew[iend + 2 * dist - o], skew[iend + dist - o]);
+      }
+      dist = dist4;
+      dist4 <<= 2;
+    }
+    if (dist < m) {
+      begin_layer(dist, dist, 2, true);
+      uint32_t l = skew[dist - o];
+      block(0, dist, l, l, l);
+    }
+    return (uint32_t)layers.size() - first;
+  }
+  // FFT (build_fft_dit8_plan mod.rs:334-374): full skew, i_end - 1 indices.
+  uint32_t fft(uint32_t mtrunc, uint32_t m, const uint32_t* skew) {
+    const uint32_t first = (uint32_t)layers.size();
+    uint32_t dist4 = m, dist = m >> 2;
+    while (dist != 0) {
+      begin_layer(dist, dist, 4, false);
+      for (uint32_t r = 0; r < mtrunc; r += dist4) {
+        uint32_t iend = r + dist;
+        block(r, dist, skew[iend - 1], skew[iend + 2 * dist - 1], skew[iend + dist - 1]);
+      }
+      dist4 = dist;
+      dist >>= 2;
+    }
+    if (dist4 == 2) {
+      begin_layer(1, 1, 2, false);
+      for (uint32_t r = 0; r < mtrunc; r += 2) block(r, 1, skew[r], skew[r], skew[r]);
+    }
+    return (uint32_t)layers.size() - first;
+  }
+};
+
+static uint32_t ceil_pow2(uint32_t v) {
+  uint32_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+struct LeoDevPlan {
+  DeviceBuffer layers, blocks, tables;
+  uint32_t n_layers = 0;
+};
+
+static int upload_plan(const PlanBuilder& b, LeoDevPlan& p) {
+  if (p.layers.alloc(std::max<size_t>(1, b.layers.size()) * sizeof(LeoLayer)) != hipSuccess ||
+      p.blocks.alloc(std::max<size_t>(1, b.blocks.size()) * sizeof(LeoBlock)) != hipSuccess ||
+      p.tables.alloc(std::max<size_t>(1, b.tables.size()) * 4) != hipSuccess)
+    return RSGPU_ERR_OUT_OF_MEMORY;
+  if (!b.layers.empty() && hipMemcpy(p.layers.p, b.layers.data(), b.layers.size() * sizeof(LeoLayer),
+                                     hipMemcpyHostToDevice) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  if (!b.blocks.empty() && hipMemcpy(p.blocks.p, b.blocks.data(), b.blocks.size() * sizeof(LeoBlock),
+                                     hipMemcpyHostToDevice) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  if (!b.tables.empty() &&
+      hipMemcpy(p.tables.p, b.tables.data(), b.tables.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  p.n_layers = (uint32_t)b.layers.size();
+  return RSGPU_OK;
+}
+
+// Per-pattern decode state: transform plan, per-lane multiply tables (error
+// locator and its inverse) built on the device.
+struct LeoDecodeEntry {
+  LeoDevPlan plan;
+  uint32_t n_ifft = 0, n_fft = 0, input_count = 0;
+  DeviceBuffer err_tabs;  // [n][words] multiply-by-err_loc tables (lanes < work_size)
+  DeviceBuffer inv_tabs;  // [n][words] multiply-by-(modulus - err_loc)
+  DeviceBuffer err_locs;  // [work_size] u32, kept for inspection
+};
+
+struct LeoState {
+  bool gf16 = false;
+  uint32_t k = 0, parity = 0, m = 0, n = 0, work_size = 0;
+  // encode
+  LeoDevPlan enc;
+  std::vector<LeoGroup> groups;
+  uint32_t fft_layer0 = 0, fft_layers = 0;
+  DeviceBuffer d_groups;
+  // device copies of the field log/exp tables (for the error locator and tables)
+  DeviceBuffer d_log, d_exp;
+  size_t lds_limit = 65536;
+  std::mutex mu;
+  std::map<std::string, std::shared_ptr<LeoDecodeEntry>> decode_cache;
+};
+
+// ---------------------------------------------------------------- kernels
+struct LeoArgs {
+  const LeoLayer* layers;
+  const LeoBlock* blocks;
+  const uint32_t* tables;
+  const uint8_t* const* ptrs;  // k data then parity shard pointers
+  const LeoGroup* groups;
+  const uint32_t* err_tabs;
+  const uint32_t* inv_tabs;
+  const uint8_t* present;      // decode: per shard (device copy), required in bit 1
+  uint64_t words;              // words per shard (GF8: len/4, GF16: len/8)
+  uint32_t C;                  // words per tile
+  uint32_t rows;               // LDS lanes
+  uint32_t k, parity, m, n, work_size;
+  uint32_t n_groups, fft_layer0, fft_layers, ifft_layer0, ifft_layers;
+  int32_t* flag;               // verify
+  uint32_t mode;               // 0 encode, 1 verify
+};
+
+template <bool G16>
+struct Word;
+template <>
+struct Word<false> {
+  uint32_t v;
+};
+template <>
+struct Word<true> {
+  uint32_t lo, hi;
+};
+
+template <bool G16>
+__device__ __forceinline__ Word<G16> lds_get(const uint32_t* lds, uint32_t idx) {
+  Word<G16> w;
+  if constexpr (G16) {
+    uint2 t = reinterpret_cast<const uint2*>(lds)[idx];
+    w.lo = t.x;
+    w.hi = t.y;
+  } else {
+    w.v = lds[idx];
+  }
+  return w;
+}
+template <bool G16>
+__device__ __forceinline__ void lds_put(uint32_t* lds, uint32_t idx, Word<G16> w) {
+  if constexpr (G16) reinterpret_cast<uint2*>(lds)[idx] = make_uint2(w.lo, w.hi);
+  else lds[idx] = w.v;
+}
+
+__device__ __forceinline__ uint32_t tmul(const uint32_t* t, const Sel& s) {
+  uint32_t a = __builtin_amdgcn_perm(t[1], t[0], s.s0);
+  uint32_t b = __builtin_amdgcn_perm(t[3], t[2], s.s1);
+  uint32_t c = __builtin_amdgcn_perm(t[4], t[4], s.s2);
+  return xor3(a, b, c);
+}
+
+// y * g^log for the table t
+template <bool G16>
+__device__ __forceinline__ Word<G16> wmul(const uint32_t* t, Word<G16> y) {
+  Word<G16> r;
+  if constexpr (G16) {
+    Sel sl = selectors(y.lo), sh = selectors(y.hi);
+    r.lo = tmul(t, sl) ^ tmul(t + 2 * kTabWords, sh);
+    r.hi = tmul(t + kTabWords, sl) ^ tmul(t + 3 * kTabWords, sh);
+  } else {
+    r.v = tmul(t, selectors(y.v));
+  }
+  return r;
+}
+template <bool G16>
+__device__ __forceinline__ void wxor(Word<G16>& a, Word<G16> b) {
+  if constexpr (G16) {
+    a.lo ^= b.lo;
+    a.hi ^= b.hi;
+  } else {
+    a.v ^= b.v;
+  }
+}
+// fft_dit2: x ^= y * t; y ^= x      ifft_dit2: y ^= x; x ^= y * t
+template <bool G16>
+__device__ __forceinline__ void fwd2(Word<G16>& x, Word<G16>& y, const uint32_t* t) {
+  wxor(x, wmul<G16>(t, y));
+  wxor(y, x);
+}
+template <bool G16>
+__device__ __forceinline__ void inv2(Word<G16>& x, Word<G16>& y, const uint32_t* t) {
+  wxor(y, x);
+  wxor(x, wmul<G16>(t, y));
+}
+
+// Runs layers [l0, l0 + nl) over LDS lanes [base, base + rows).
+template <bool G16>
+__device__ void run_layers(uint32_t* lds, uint32_t lane_base, uint32_t l0, uint32_t nl,
+                           const LeoArgs& a) {
+  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
+  const uint32_t C = a.C;
+  for (uint32_t l = l0; l < l0 + nl; ++l) {
+    const LeoLayer L = a.layers[l];
+    const uint32_t items = L.n_blocks * L.cnt * C;
+    for (uint32_t it = threadIdx.x; it < items; it += blockDim.x) {
+      const uint32_t w = it % C, bf = it / C;
+      const LeoBlock B = a.blocks[L.block0 + bf / L.cnt];
+      const uint32_t i = B.r + bf % L.cnt;
+      const uint32_t d = L.dist;
+      const uint32_t* t01 = a.tables + (size_t)B.t01 * TW;
+      if (L.radix == 2) {
+        Word<G16> x = lds_get<G16>(lds, (lane_base + i) * C + w);
+        Word<G16> y = lds_get<G16>(lds, (lane_base + i + d) * C + w);
+        if (L.inverse) inv2<G16>(x, y, t01);
+        else fwd2<G16>(x, y, t01);
+        lds_put<G16>(lds, (lane_base + i) * C + w, x);
+        lds_put<G16>(lds, (lane_base + i + d) * C + w, y);
+      } else {
+        const uint32_t* t23 = a.tables + (size_t)B.t23 * TW;
+        const uint32_t* t02 = a.tables + (size_t)B.t02 * TW;
+        Word<G16> x0 = lds_get<G16>(lds, (lane_base + i) * C + w);
+        Word<G16> x1 = lds_get<G16>(lds, (lane_base + i + d) * C + w);
+        Word<G16> x2 = lds_get<G16>(lds, (lane_base + i + 2 * d) * C + w);
+        Word<G16> x3 = lds_get<G16>(lds, (lane_base + i + 3 * d) * C + w);
+        if (L.inverse) {  // ifft_dit4 (ops.rs:556-586)
+          inv2<G16>(x0, x1, t01);
+          inv2<G16>(x2, x3, t23);
+          inv2<G16>(x1, x3, t02);
+          inv2<G16>(x0, x2, t02);
+        } else {  // fft_dit4 (ops.rs:516-548)
+          fwd2<G16>(x0, x2, t02);
+          fwd2<G16>(x1, x3, t02);
+          fwd2<G16>(x0, x1, t01);
+          fwd2<G16>(x2, x3, t23);
+        }
+        lds_put<G16>(lds, (lane_base + i) * C + w, x0);
+        lds_put<G16>(lds, (lane_base + i + d) * C + w, x1);
+        lds_put<G16>(lds, (lane_base + i + 2 * d) * C + w, x2);
+        lds_put<G16>(lds, (lane_base + i + 3 * d) * C + w, x3);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Global addresses of word w of tile `tile` of a shard.
+template <bool G16>
+__device__ __forceinline__ Word<G16> gload(const uint8_t* shard, uint64_t word) {
+  Word<G16> r;
+  if constexpr (G16) {
+    const uint64_t b = word >> 3, j = word & 7;
+    r.lo = *reinterpret_cast<const uint32_t*>(shard + b * 64 + j * 4);
+    r.hi = *reinterpret_cast<const uint32_t*>(shard + b * 64 + 32 + j * 4);
+  } else {
+    r.v = *reinterpret_cast<const uint32_t*>(shard + word * 4);
+  }
+  return r;
+}
+template <bool G16>
+__device__ __forceinline__ void gstore(uint8_t* shard, uint64_t word, Word<G16> v) {
+  if constexpr (G16) {
+    const uint64_t b = word >> 3, j = word & 7;
+    *reinterpret_cast<uint32_t*>(shard + b * 64 + j * 4) = v.lo;
+    *reinterpret_cast<uint32_t*>(shard + b * 64 + 32 + j * 4) = v.hi;
+  } else {
+    *reinterpret_cast<uint32_t*>(shard + word * 4) = v.v;
+  }
+}
+template <bool G16>
+__device__ __forceinline__ bool wequal(Word<G16> a, Word<G16> b) {
+  if constexpr (G16) return a.lo == b.lo && a.hi == b.hi;
+  else return a.v == b.v;
+}
+
+// encode_with_tables (leopard_gf8/encode.rs:194-372; leopard_gf16/encode.rs:109-253):
+// groups of m data lanes are IFFT'd (skew offset m-1+group*m) and XORed into the
+// first m lanes, which are then FFT'd; lanes 0..parity are the parity.
+template <bool G16>
+__global__ __launch_bounds__(256) void leo_encode_kernel(const LeoArgs a) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t C = a.C, m = a.m;
+  const uint64_t w0 = (uint64_t)blockIdx.x * C;
+  uint32_t* A = lds;
+  uint32_t* Bv = lds + (size_t)m * C * (G16 ? 2 : 1);
+  for (uint32_t g = 0; g < a.n_groups; ++g) {
+    const LeoGroup G = a.groups[g];
+    uint32_t* dst = g == 0 ? A : Bv;
+    for (uint32_t it = threadIdx.x; it < m * C; it += blockDim.x) {
+      const uint32_t lane = it / C, w = it % C;
+      Word<G16> v{};
+      if (lane < G.mtrunc && w0 + w < a.words) v = gload<G16>(a.ptrs[G.first + lane], w0 + w);
+      lds_put<G16>(dst, lane * C + w, v);
+    }
+    __syncthreads();
+    run_layers<G16>(dst, 0, G.layer0, G.n_layers, a);
+    if (g > 0) {
+      for (uint32_t it = threadIdx.x; it < m * C; it += blockDim.x) {
+        Word<G16> x = lds_get<G16>(A, it), y = lds_get<G16>(Bv, it);
+        wxor(x, y);
+        lds_put<G16>(A, it, x);
+      }
+      __syncthreads();
+    }
+  }
+  run_layers<G16>(A, 0, a.fft_layer0, a.fft_layers, a);
+  bool mismatch = false;
+  for (uint32_t it = threadIdx.x; it < a.parity * C; it += blockDim.x) {
+    const uint32_t lane = it / C, w = it % C;
+    if (w0 + w >= a.words) continue;
+    uint8_t* out = const_cast<uint8_t*>(a.ptrs[a.k + lane]);
+    Word<G16> v = lds_get<G16>(A, lane * C + w);
+    if (a.mode == 0) gstore<G16>(out, w0 + w, v);
+    else mismatch |= !wequal<G16>(v, gload<G16>(out, w0 + w));
+  }
+  if (mismatch) atomicOr(a.flag, 1);
+}
+
+// reconstruct_with_resources (leopard_gf8/decode.rs:319-429; leopard_gf16/decode.rs:816-905).
+template <bool G16>
+__global__ __launch_bounds__(256) void leo_decode_kernel(const LeoArgs a) {
+  extern __shared__ uint32_t lds[];
+  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
+  const uint32_t C = a.C, n = a.n, m = a.m;
+  const uint64_t w0 = (uint64_t)blockIdx.x * C;
+  // lanes: 0..parity parity shards, m..m+k data shards, scaled by the error locator
+  for (uint32_t it = threadIdx.x; it < n * C; it += blockDim.x) {
+    const uint32_t lane = it / C, w = it % C;
+    Word<G16> v{};
+    int shard = -1;
+    if (lane < a.parity) shard = (int)(a.k + lane);
+    else if (lane >= m && lane < m + a.k) shard = (int)(lane - m);
+    if (shard >= 0 && (a.present[shard] & 1) && w0 + w < a.words) {
+      v = gload<G16>(a.ptrs[shard], w0 + w);
+      v = wmul<G16>(a.err_tabs + (size_t)lane * TW, v);
+    }
+    lds_put<G16>(lds, it, v);
+  }
+  __syncthreads();
+  run_layers<G16>(lds, 0, a.ifft_layer0, a.ifft_layers, a);
+  // formal derivative (decode.rs:537-551): in closed form every lane j takes
+  // lane j ^ XOR_{b : bit b of j clear, j + 2^b < n} lane (j + 2^b), all of
+  // them before the update.
+  for (uint32_t it0 = 0; it0 < n * C; it0 += blockDim.x) {
+    const uint32_t it = it0 + threadIdx.x;
+    Word<G16> acc{};
+    if (it < n * C) {
+      const uint32_t j = it / C, w = it % C;
+      acc = lds_get<G16>(lds, it);
+      for (uint32_t bit = 1; bit < n; bit <<= 1)
+        if (!(j & bit) && j + bit < n) wxor(acc, lds_get<G16>(lds, (j + bit) * C + w));
+    }
+    __syncthreads();
+    if (it < n * C) lds_put<G16>(lds, it, acc);
+    __syncthreads();
+  }
+  run_layers<G16>(lds, 0, a.fft_layer0, a.fft_layers, a);
+  for (uint32_t it = threadIdx.x; it < (a.k + a.parity) * C; it += blockDim.x) {
+    const uint32_t s = it / C, w = it % C;
+    if ((a.present[s] & 3) != 2 || w0 + w >= a.words) continue;  // missing and required
+    const uint32_t lane = s >= a.k ? s - a.k : m + s;
+    Word<G16> v = wmul<G16>(a.inv_tabs + (size_t)lane * TW, lds_get<G16>(lds, lane * C + w));
+    gstore<G16>(const_cast<uint8_t*>(a.ptrs[s]), w0 + w, v);
+  }
+}
+
+// Error locator in closed form: err[i] = sum_{j erased} log[i ^ j] mod (2^n - 1)
+// over the erased positions of compute_error_locs (leopard_gf8/decode.rs:68-114:
+// missing parity p -> p, padding parity..m, missing data d -> m + d).  This is
+// the FWHT convolution of the reference evaluated directly (the Walsh transform
+// is its own inverse up to 2^n == 1 mod 2^n - 1, the identity
+// leopard_gf16/decode.rs:412-436 relies on).  Both representations of zero
+// (0 and 2^n - 1) multiply as the identity, so outputs are unchanged.  Then
+// writes the per-lane multiply tables for err and (modulus - err).
+struct ErrJob {
+  uint32_t work_size, n_pos, gf16, modulus;
+};
+
+__device__ __forceinline__ uint32_t d_mul_log(const uint32_t* lg, const uint32_t* ex, uint32_t a,
+                                              uint32_t log_b, uint32_t bits, uint32_t order) {
+  if (a == 0) return 0;
+  uint32_t sum = lg[a] + log_b;
+  return ex[(sum + (sum >> bits)) & (order - 1)];
+}
+
+__device__ void d_pack(const uint32_t* lg, const uint32_t* ex, uint32_t log_m, uint32_t shift,
+                       uint32_t byte_sel, uint32_t bits, uint32_t order, uint32_t* w) {
+  uint32_t v[5] = {0, 0, 0, 0, 0};
+  auto f = [&](uint32_t x) -> uint32_t {
+    return (d_mul_log(lg, ex, x << shift, log_m, bits, order) >> (8 * byte_sel)) & 0xff;
+  };
+  for (int i = 0; i < 4; ++i) {
+    v[0] |= f(i) << (8 * i);
+    v[1] |= f(i + 4) << (8 * i);
+    v[2] |= f(8 * i) << (8 * i);
+    v[3] |= f(8 * (i + 4)) << (8 * i);
+    v[4] |= f(64 * i) << (8 * i);
+  }
+  for (int i = 0; i < 5; ++i) w[i] = v[i];
+  w[5] = w[6] = w[7] = 0;
+}
+
+__global__ void leo_errloc_kernel(const uint32_t* __restrict__ lg, const uint32_t* __restrict__ ex,
+                                  const uint32_t* __restrict__ pos, ErrJob job, uint32_t* err,
+                                  uint32_t* err_tabs, uint32_t* inv_tabs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= job.work_size) return;
+  uint64_t sum = 0;
+  for (uint32_t t = 0; t < job.n_pos; ++t) sum += lg[i ^ pos[t]];
+  const uint32_t e = (uint32_t)(sum % job.modulus);
+  err[i] = e;
+  const uint32_t bits = job.gf16 ? 16 : 8, order = job.modulus + 1;
+  const uint32_t inv = job.modulus - e;
+  if (job.gf16) {
+    uint32_t* te = err_tabs + (size_t)i * kLeoTabWords16;
+    uint32_t* ti = inv_tabs + (size_t)i * kLeoTabWords16;
+    d_pack(lg, ex, e, 0, 0, bits, order, te);
+    d_pack(lg, ex, e, 0, 1, bits, order, te + kTabWords);
+    d_pack(lg, ex, e, 8, 0, bits, order, te + 2 * kTabWords);
+    d_pack(lg, ex, e, 8, 1, bits, order, te + 3 * kTabWords);
+    d_pack(lg, ex, inv, 0, 0, bits, order, ti);
+    d_pack(lg, ex, inv, 0, 1, bits, order, ti + kTabWords);
+    d_pack(lg, ex, inv, 8, 0, bits, order, ti + 2 * kTabWords);
+    d_pack(lg, ex, inv, 8, 1, bits, order, ti + 3 * kTabWords);
+  } else {
+    d_pack(lg, ex, e, 0, 0, bits, order, err_tabs + (size_t)i * kLeoTabWords8);
+    d_pack(lg, ex, inv, 0, 0, bits, order, inv_tabs + (size_t)i * kLeoTabWords8);
+  }
+}
+
+// ------------------------------------------------------------------- host
+static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopard.get()); }
+
+static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words) {
+  const uint32_t wb = s.gf16 ? 8 : 4;
+  uint64_t C = s.lds_limit / ((uint64_t)rows * wb);
+  if (C > 64) C = 64;
+  if (C < 1) C = 1;
+  if (C > words) C = (uint32_t)words;
+  return (uint32_t)C;
+}
+
+int init_leopard(Codec& c) {
+  auto st = std::make_shared<LeoState>();
+  st->gf16 = c.family == RSGPU_FAMILY_LEOPARD_GF16;
+  const LeoField& F = st->gf16 ? field16() : field8();
+  st->k = (uint32_t)c.k;
+  st->parity = (uint32_t)c.m;
+  st->m = ceil_pow2(st->parity);
+  if (st->m > F.modulus) return RSGPU_TOO_MANY_SHARDS;
+  st->work_size = st->m + st->k;
+  st->n = ceil_pow2(st->work_size);
+  int dev = 0, lds = 65536;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+    st->lds_limit = (size_t)lds;
+  if ((size_t)st->n * (st->gf16 ? 8 : 4) > st->lds_limit ||
+      (size_t)2 * st->m * (st->gf16 ? 8 : 4) > st->lds_limit)
+    return RSGPU_ERR_UNSUPPORTED;
+  // encode plan (EncodePlans::new, leopard_gf8/encode.rs:109-147)
+  PlanBuilder pb(F);
+  const uint32_t* skew = F.skew.data();
+  const uint32_t m = st->m, k = st->k;
+  const uint32_t mtrunc = std::min(k, m);
+  LeoGroup g0{0, mtrunc, 0, 0};
+  g0.n_layers = pb.ifft(mtrunc, m, skew + (m - 1), false);
+  st->groups.push_back(g0);
+  uint32_t i = m;
+  if (m < k) {
+    for (; i + m <= k; i += m) {
+      LeoGroup g{i, m, (uint32_t)pb.layers.size(), 0};
+      g.n_layers = pb.ifft(m, m, skew + (m - 1) + i, false);
+      st->groups.push_back(g);
+    }
+    if (k % m) {
+      LeoGroup g{i, k % m, (uint32_t)pb.layers.size(), 0};
+      g.n_layers = pb.ifft(k % m, m, skew + (m - 1) + i, false);
+      st->groups.push_back(g);
+    }
+  }
+  st->fft_layer0 = (uint32_t)pb.layers.size();
+  st->fft_layers = pb.fft(st->parity, m, skew);
+  int rc = upload_plan(pb, st->enc);
+  if (rc) return rc;
+  if (st->d_groups.alloc(st->groups.size() * sizeof(LeoGroup)) != hipSuccess) return RSGPU_ERR_OUT_OF_MEMORY;
+  if (hipMemcpy(st->d_groups.p, st->groups.data(), st->groups.size() * sizeof(LeoGroup),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  std::vector<uint32_t> ex(F.exp.begin(), F.exp.begin() + F.order);
+  if (st->d_log.alloc(F.order * 4) != hipSuccess || st->d_exp.alloc(F.order * 4) != hipSuccess)
+    return RSGPU_ERR_OUT_OF_MEMORY;
+  if (hipMemcpy(st->d_log.p, F.log.data(), F.order * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(st->d_exp.p, ex.data(), F.order * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  c.leopard = st;
+  return RSGPU_OK;
+}
+
+static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs, size_t len, int mode,
+                         int32_t* flag, hipStream_t s, DeviceBuffer& dptrs) {
+  LeoState& st = state(c);
+  const uint64_t words = len / (st.gf16 ? 8 : 4);
+  const uint32_t C = tile_words(st, 2 * st.m, words);
+  if (dptrs.alloc(ptrs.size() * sizeof(void*)) != hipSuccess) return RSGPU_ERR_OUT_OF_MEMORY;
+  if (hipMemcpyAsync(dptrs.p, ptrs.data(), ptrs.size() * sizeof(void*), hipMemcpyHostToDevice, s) !=
+      hipSuccess)
+    return RSGPU_ERR_HIP;
+  LeoArgs a{};
+  a.layers = (const LeoLayer*)st.enc.layers.p;
+  a.blocks = (const LeoBlock*)st.enc.blocks.p;
+  a.tables = (const uint32_t*)st.enc.tables.p;
+  a.ptrs = (const uint8_t* const*)dptrs.p;
+  a.groups = (const LeoGroup*)st.d_groups.p;
+  a.words = words;
+  a.C = C;
+  a.k = st.k;
+  a.parity = st.parity;
+  a.m = st.m;
+  a.n_groups = (uint32_t)st.groups.size();
+  a.fft_layer0 = st.fft_layer0;
+  a.fft_layers = st.fft_layers;
+  a.flag = flag;
+  a.mode = (uint32_t)mode;
+  const size_t lds = (size_t)2 * st.m * C * (st.gf16 ? 8 : 4);
+  const unsigned grid = (unsigned)((words + C - 1) / C);
+  if (st.gf16)
+    hipLaunchKernelGGL(leo_encode_kernel<true>, dim3(grid), dim3(256), lds, s, a);
+  else
+    hipLaunchKernelGGL(leo_encode_kernel<false>, dim3(grid), dim3(256), lds, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_hip_error(e, "leo_encode_kernel");
+    return RSGPU_ERR_HIP;
+  }
+  // the pointer table is freed on return: wait for the kernel that reads it
+  if (hipStreamSynchronize(s) != hipSuccess) return RSGPU_ERR_HIP;
+  return RSGPU_OK;
+}
+
+int leopard_encode(const Codec& c, const rsgpu_cslice* data, const rsgpu_slice* parity, size_t len,
+                   hipStream_t s) {
+  if (len == 0 || len % 64) return RSGPU_INCORRECT_SHARD_SIZE;  // leopard.rs:50-56
+  std::vector<const uint8_t*> ptrs;
+  for (size_t i = 0; i < c.k; ++i) ptrs.push_back(data[i].ptr);
+  for (size_t i = 0; i < c.m; ++i) ptrs.push_back(parity[i].ptr);
+  DeviceBuffer dptrs;
+  return launch_encode(c, ptrs, len, 0, nullptr, s, dptrs);
+}
+
+int leopard_verify(const Codec& c, const rsgpu_cslice* shards, size_t len, int32_t* d_flag,
+                   hipStream_t s) {
+  if (len == 0 || len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
+  std::vector<const uint8_t*> ptrs;
+  for (size_t i = 0; i < c.k + c.m; ++i) ptrs.push_back(shards[i].ptr);
+  DeviceBuffer dptrs;
+  return launch_encode(c, ptrs, len, 1, d_flag, s, dptrs);
+}
+
+// Decode plan per erasure pattern (DecodePlan::new, leopard_gf8/decode.rs:130-152;
+// build_decode_plan, leopard_gf16/decode.rs:507-560).
+static int decode_entry(const Codec& c, const std::vector<uint8_t>& present, hipStream_t s,
+                        std::shared_ptr<LeoDecodeEntry>& out) {
+  LeoState& st = state(c);
+  std::string key(present.begin(), present.end());
+  std::lock_guard<std::mutex> g(st.mu);
+  auto it = st.decode_cache.find(key);
+  if (it != st.decode_cache.end()) {
+    out = it->second;
+    return RSGPU_OK;
+  }
+  const LeoField& F = st.gf16 ? field16() : field8();
+  auto e = std::make_shared<LeoDecodeEntry>();
+  uint32_t input_count = st.m;
+  for (uint32_t i = 0; i < st.k; ++i)
+    if (present[i]) input_count = st.m + i + 1;
+  e->input_count = input_count;
+  PlanBuilder pb(F);
+  e->n_ifft = pb.ifft(input_count, st.n, F.skew.data(), true);
+  e->n_fft = pb.fft(st.work_size, st.n, F.skew.data());
+  int rc = upload_plan(pb, e->plan);
+  if (rc) return rc;
+  // erased positions of compute_error_locs
+  std::vector<uint32_t> pos;
+  for (uint32_t i = 0; i < st.parity; ++i)
+    if (!present[st.k + i]) pos.push_back(i);
+  for (uint32_t i = st.parity; i < st.m; ++i) pos.push_back(i);
+  for (uint32_t i = 0; i < st.k; ++i)
+    if (!present[i]) pos.push_back(st.m + i);
+  const int TW = st.gf16 ? kLeoTabWords16 : kLeoTabWords8;
+  DeviceBuffer dpos;
+  if (dpos.alloc(std::max<size_t>(1, pos.size()) * 4) != hipSuccess ||
+      e->err_tabs.alloc((size_t)st.n * TW * 4) != hipSuccess ||
+      e->inv_tabs.alloc((size_t)st.n * TW * 4) != hipSuccess ||
+      e->err_locs.alloc((size_t)st.work_size * 4) != hipSuccess)
+    return RSGPU_ERR_OUT_OF_MEMORY;
+  if (!pos.empty() && hipMemcpyAsync(dpos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  ErrJob job{st.work_size, (uint32_t)pos.size(), st.gf16 ? 1u : 0u, F.modulus};
+  hipLaunchKernelGGL(leo_errloc_kernel, dim3((st.work_size + 255) / 256), dim3(256), 0, s,
+                     (const uint32_t*)st.d_log.p, (const uint32_t*)st.d_exp.p, (const uint32_t*)dpos.p,
+                     job, (uint32_t*)e->err_locs.p, (uint32_t*)e->err_tabs.p, (uint32_t*)e->inv_tabs.p);
+  hipError_t he = hipGetLastError();
+  if (he != hipSuccess) {
+    set_hip_error(he, "leo_errloc_kernel");
+    return RSGPU_ERR_HIP;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) return RSGPU_ERR_HIP;  // dpos freed on return
+  if (st.decode_cache.size() >= 64) st.decode_cache.clear();
+  st.decode_cache[key] = e;
+  out = e;
+  return RSGPU_OK;
+}
+
+// validate_leopard_reconstruction + reconstruct_leopard_impl (reconstruct.rs:795-1040)
+int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
+                        const uint8_t* required, size_t n_required, hipStream_t s) {
+  const size_t total = c.k + c.m;
+  if (goal == 2 && n_required != c.k && n_required != total) return RSGPU_INVALID_SHARD_FLAGS;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  size_t len = 0, present = 0;
+  bool need = false;
+  for (size_t i = 0; i < n; ++i) {
+    if (shards[i].present) {
+      if (shards[i].len == 0) return RSGPU_EMPTY_SHARD;
+      if ((len && shards[i].len != len) || shards[i].len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
+      len = shards[i].len;
+      ++present;
+    }
+  }
+  if (present < c.k) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  std::vector<uint8_t> flags(total, 0);
+  for (size_t i = 0; i < total; ++i) {
+    bool req = goal == 2 ? (i < n_required && required[i]) : (goal == 0 || i < c.k);
+    flags[i] = (uint8_t)((shards[i].present ? 1 : 0) | (!shards[i].present && req ? 2 : 0));
+    if (!shards[i].present && req) {
+      need = true;
+      if (!shards[i].ptr || shards[i].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+    }
+  }
+  if (!need) return RSGPU_OK;
+  if (total - present > c.m) return RSGPU_TOO_FEW_SHARDS_PRESENT;  // decode.rs:744-747
+  LeoState& st = state(c);
+  std::vector<uint8_t> pres(total);
+  for (size_t i = 0; i < total; ++i) pres[i] = shards[i].present ? 1 : 0;
+  std::shared_ptr<LeoDecodeEntry> e;
+  int rc = decode_entry(c, pres, s, e);
+  if (rc) return rc;
+  const uint64_t words = len / (st.gf16 ? 8 : 4);
+  const uint32_t C = tile_words(st, st.n, words);
+  std::vector<const uint8_t*> ptrs(total);
+  for (size_t i = 0; i < total; ++i) ptrs[i] = shards[i].ptr;
+  DeviceBuffer dptrs, dflags;
+  if (dptrs.alloc(total * sizeof(void*)) != hipSuccess || dflags.alloc(total) != hipSuccess)
+    return RSGPU_ERR_OUT_OF_MEMORY;
+  if (hipMemcpyAsync(dptrs.p, ptrs.data(), total * sizeof(void*), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(dflags.p, flags.data(), total, hipMemcpyHostToDevice, s) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  LeoArgs a{};
+  a.layers = (const LeoLayer*)e->plan.layers.p;
+  a.blocks = (const LeoBlock*)e->plan.blocks.p;
+  a.tables = (const uint32_t*)e->plan.tables.p;
+  a.ptrs = (const uint8_t* const*)dptrs.p;
+  a.err_tabs = (const uint32_t*)e->err_tabs.p;
+  a.inv_tabs = (const uint32_t*)e->inv_tabs.p;
+  a.present = (const uint8_t*)dflags.p;
+  a.words = words;
+  a.C = C;
+  a.k = st.k;
+  a.parity = st.parity;
+  a.m = st.m;
+  a.n = st.n;
+  a.work_size = st.work_size;
+  a.ifft_layer0 = 0;
+  a.ifft_layers = e->n_ifft;
+  a.fft_layer0 = e->n_ifft;
+  a.fft_layers = e->n_fft;
+  const size_t lds = (size_t)st.n * C * (st.gf16 ? 8 : 4);
+  const unsigned grid = (unsigned)((words + C - 1) / C);
+  if (st.gf16)
+    hipLaunchKernelGGL(leo_decode_kernel<true>, dim3(grid), dim3(256), lds, s, a);
+  else
+    hipLaunchKernelGGL(leo_decode_kernel<false>, dim3(grid), dim3(256), lds, s, a);
+  hipError_t he = hipGetLastError();
+  if (he != hipSuccess) {
+    set_hip_error(he, "leo_decode_kernel");
+    return RSGPU_ERR_HIP;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) return RSGPU_ERR_HIP;  // dptrs/dflags freed on return
+  for (size_t i = 0; i < total; ++i)
+    if (flags[i] & 2) shards[i].present = 1;
+  return RSGPU_OK;
+}
+
+}  // namespace rsgpu
