@@ -39,7 +39,35 @@ def parse():
     ap.add_argument("--shard", type=int, default=1 << 20)
     ap.add_argument("--cpu-stripes", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the tuned batch encode launches plus a 1 GiB device copy "
+                         "(the reference kernel for scaling rocprofv3 byte counters)")
     return ap.parse_args()
+
+
+def roofline_only(args):
+    """Short run for counter collection: a 1 GiB device copy of known size, then the tuned
+    10+4 batch encode launched `steps` times."""
+    import torch
+
+    import rsgpu
+
+    torch.cuda.init()
+    src = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    k, m, S, N = 10, 4, args.shard, args.stripes
+    st = Stripes(rsgpu, torch, k, m, N, S, seed=0x00EC5EED)
+    b = st.batch("encode", sp)
+    tune = b.autotune(reps=2)
+    for _ in range(args.steps):
+        b.graph_launch()
+    torch.cuda.synchronize()
+    print(json.dumps({"roofline_only": True, "tune": tune, "bytes_per_launch": N * (k + m) * S,
+                      "copy_bytes": 1 << 30}))
 
 
 class Stripes:
@@ -140,6 +168,21 @@ def event_durations(torch, fn, count):
     return np.array([a.elapsed_time(b) for a, b in evs])
 
 
+def measured_traffic(bytes_per_launch):
+    """HBM bytes per launch of the encode batch kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a 1 GiB copy of known size); None when the
+    workload differs from the profiled one."""
+    path = os.path.join(ROOT, "profiles", "batch_kernel_encode_10x4_1MiB_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("bytes_per_launch_algorithmic") != bytes_per_launch:
+        return None
+    return d["traffic_bytes_per_launch"]
+
+
 def cpu_baseline(k, m, shard, stripes):
     """Oracle (scalar C restatement of the reference's code_some_slices) on a bounded sample."""
     import numpy as np
@@ -166,6 +209,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(stripes shard across GPUs with no exchange; see DESIGN.md)"}))
         sys.exit(2)
+    if args.roofline_only:
+        roofline_only(args)
+        return
     import numpy as np
     import torch
 
@@ -284,7 +330,8 @@ def main():
                               "p99": round(float(np.percentile(lat, 99)), 4)},
         "batch_tune": tune,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(bytes_per_launch),
                      "kernel": "batch_kernel (512 x RS 10+4 x 1 MiB encode, one launch)",
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": round(float(d_batch.mean()), 4)},
