@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--shard", type=int, default=1 << 20)
     ap.add_argument("--cpu-stripes", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ragged", type=int, default=8192,
+                    help="stripes in the ragged mix (0 skips it)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the tuned batch encode launches plus a 1 GiB device copy "
                          "(the reference kernel for scaling rocprofv3 byte counters)")
@@ -223,6 +225,7 @@ def main():
     k, m, S, N = 10, 4, args.shard, args.stripes
     st = Stripes(rsgpu, torch, k, m, N, S, seed=0x00EC5EED)
     data_bytes = N * k * S
+    pitch = st.pitch
 
     # ---- plain: one launch per stripe
     t_enc_plain = timed(torch, lambda: [st.encode(s, sp) for s in range(N)], args.steps, args.warmup)
@@ -293,8 +296,38 @@ def main():
     t83_batch = timed(torch, b83.graph_launch, args.steps, 1)
     del b83, st83
 
-    cpu = None if args.no_cpu else cpu_baseline(k, m, S, args.cpu_stripes)
     gbs = lambda nbytes, t: round(nbytes / t / 1e9, 2)
+
+    # ---- ragged mix: mixed (k, m), 4 KiB-4 MiB shards, 60/20/20 encode/verify/reconstruct
+    ragged = None
+    if args.ragged:
+        from rsgpu import workload
+
+        del st
+        spec = workload.ragged_spec(args.ragged)
+        rg = workload.RaggedSet(rsgpu, torch, spec)
+        rg.make_consistent(sp)
+        torch.cuda.synchronize()
+        t_rg_plain = timed(torch, lambda: rg.run_plain(sp), 1, 1)
+        b_rg = rg.make_batch(sp)
+        rg_tune = b_rg.autotune(reps=2)
+
+        def rg_batch():
+            rg.flags.zero_()
+            b_rg.graph_launch()
+
+        t_rg_batch = timed(torch, rg_batch, args.steps, 1)
+        assert int(rg.flags.sum()) == 0, "ragged verify reported a mismatch"
+        ragged = {"stripes": len(spec), "data_bytes": rg.data_bytes, "moved_bytes": rg.moved_bytes,
+                  "plain_gbs": gbs(rg.data_bytes, t_rg_plain),
+                  "batched_gbs": gbs(rg.data_bytes, t_rg_batch),
+                  "batched_bytes_moved_gbs": gbs(rg.moved_bytes, t_rg_batch),
+                  "batched_vs_plain": round(t_rg_plain / t_rg_batch, 2),
+                  "batched_ms": round(t_rg_batch * 1e3, 3), "plain_ms": round(t_rg_plain * 1e3, 3),
+                  "tune": rg_tune}
+        del b_rg, rg
+
+    cpu = None if args.no_cpu else cpu_baseline(k, m, S, args.cpu_stripes)
     rec_moved = N * k * S + n_erased * S
     out = {
         "metric": "data GB/s RS 10+4 encode/reconstruct, batched vs plain, bit-exact; p99 ms",
@@ -313,7 +346,7 @@ def main():
                                "device-resident, batched: one persistent launch per step "
                                "(hipGraph replay); plain per-stripe launches reported alongside",
                    "stripes": N, "shard_bytes": S, "data_shards": k, "parity_shards": m,
-                   "shard_pitch_bytes": st.pitch},
+                   "shard_pitch_bytes": pitch},
         "encode_batched_gbs": gbs(data_bytes, t_enc_batch),
         "encode_plain_gbs": gbs(data_bytes, t_enc_plain),
         "encode_batched_bytes_moved_gbs": gbs(N * (k + m) * S, t_enc_batch),
@@ -326,6 +359,7 @@ def main():
         "reconstruct_batched_bytes_moved_gbs": gbs(rec_moved, t_rec_batch),
         "encode_8x3_batched_gbs": gbs(N * 8 * S, t83_batch),
         "encode_8x3_plain_gbs": gbs(N * 8 * S, t83_plain),
+        "ragged": ragged,
         "stripe_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                               "p99": round(float(np.percentile(lat, 99)), 4)},
         "batch_tune": tune,
