@@ -189,6 +189,62 @@ int rsgpu_batch_graph_launch(rsgpu_batch* batch, void* stream);
 int rsgpu_batch_stats(const rsgpu_batch* batch, uint64_t* n_ops, uint64_t* bytes_moved,
                       uint64_t* data_bytes, uint64_t* tiles);
 
+/* ---- kernel-variant dispatch.
+ * The reference picks a CPU kernel per call from the ISA and the shard shape
+ * (SelectedEncodeKernel::select, src/galois_8/codegen.rs:10-44; ParallelPolicy
+ * chunking, parallel.rs:65-127; Leopard prototype selection leopard.rs:191-223).
+ * Here every call can run one of rsgpu_variant_count() kernel variants that all
+ * produce the same bytes:
+ *   kind 0: matrix apply, `lanes` per workgroup, `vectors_per_lane` 16-byte
+ *           vectors per lane, `rows_per_group` outputs per workgroup (0 = all);
+ *           on a Leopard GF(2^8) codec this applies its generator matrix
+ *           (encode, verify, and reconstruct with exactly k shards present);
+ *   kind 1: Leopard FFT with a column tile of `leopard_tile_words` words.
+ * A codec uses its forced variant if set, else its variant table, else the
+ * default tiling; an entry that is not valid for a call falls back to the
+ * default.  Setting either while calls run on the codec is not synchronised
+ * with those calls (each call reads one consistent table). ---- */
+int rsgpu_variant_count(void);
+/* Table geometry: ops (0 encode, 1 verify, 2 reconstruct), size buckets, and the
+ * bucket offset: bucket = clamp(round(2 * log2(shard_len)) - bucket0). */
+int rsgpu_dispatch_dims(int32_t* n_ops, int32_t* n_buckets, int32_t* bucket0);
+int rsgpu_variant_info(int32_t variant, int32_t* kind, int32_t* lanes, int32_t* vectors_per_lane,
+                       int32_t* rows_per_group, int32_t* leopard_tile_words);
+/* 1 if `variant` is exact for `op` with `erasures` missing shards on this codec. */
+int rsgpu_variant_valid(const rsgpu_codec* codec, int32_t op, size_t erasures, int32_t variant);
+/* Forces one variant for every call (-1 clears). */
+int rsgpu_codec_set_variant(rsgpu_codec* codec, int32_t variant);
+/* Installs table[op][e][bucket] (n_ops x n_erasures x n_buckets int16, -1 =
+ * default tiling; erasure counts >= n_erasures use the last row); NULL clears. */
+int rsgpu_codec_set_variant_table(rsgpu_codec* codec, const int16_t* table, size_t n_erasures);
+/* The variant a call with these parameters would run (-1 = default tiling). */
+int rsgpu_codec_variant_for(const rsgpu_codec* codec, int32_t op, size_t erasures, size_t shard_len,
+                            int32_t* variant);
+
+/* ---- the dispatcher's network: fp32 x[n_in <= 16] -> 64 ReLU -> 64 ReLU ->
+ * y[n_out <= 32], trained on the device.  It predicts per-variant log-times from
+ * call features so a dispatcher can fill variant tables without timing every
+ * variant (the reference's selection is the fixed cascade of codegen.rs:10-44).
+ * Parameters (host layout): W1[64][n_in] b1[64] W2[64][64] b2[64]
+ * W3[n_out][64] b3[n_out].  x, y, t, w are device arrays, row-major per sample.
+ * Loss = sum w (y - t)^2 / sum w; training takes full-batch Adam steps
+ * (beta 0.9 / 0.999). ---- */
+typedef struct rsgpu_mlp rsgpu_mlp;
+int rsgpu_mlp_new(int32_t n_in, int32_t n_out, uint64_t seed, rsgpu_mlp** out);
+void rsgpu_mlp_free(rsgpu_mlp* mlp);
+int rsgpu_mlp_dims(const rsgpu_mlp* mlp, int32_t* n_in, int32_t* n_hidden, int32_t* n_out,
+                   size_t* n_params);
+int rsgpu_mlp_get_params(const rsgpu_mlp* mlp, float* params, size_t n_params);
+/* Also resets the Adam moments. */
+int rsgpu_mlp_set_params(rsgpu_mlp* mlp, const float* params, size_t n_params);
+int rsgpu_mlp_forward(rsgpu_mlp* mlp, const float* x, size_t batch, float* y, void* stream);
+/* Loss and its gradient (host outputs) at the current parameters. */
+int rsgpu_mlp_loss_grad(rsgpu_mlp* mlp, const float* x, const float* t, const float* w, size_t batch,
+                        float* loss, float* grad, size_t n_params, void* stream);
+/* `steps` Adam steps with step size lr; *loss (optional) = loss before the last step. */
+int rsgpu_mlp_train(rsgpu_mlp* mlp, const float* x, const float* t, const float* w, size_t batch,
+                    int32_t steps, float lr, float* loss, void* stream);
+
 /* ---- decode-matrix cache statistics (ReconstructionCacheStats, codec.rs:401-426) ---- */
 int rsgpu_cache_stats(const rsgpu_codec* codec, uint64_t* requests, uint64_t* hits,
                       uint64_t* misses, uint64_t* entries);

@@ -24,13 +24,27 @@ FLAGS = [
 ]
 
 
+# apply_inst.hip holds one instantiation set of the matrix-apply kernel; it is
+# compiled once per (tile shape, element type) pair.
+INST = "apply_inst.hip"
+N_TILES = 5
+
+
 def sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    units = [(f, f.replace(".hip", ".o"), []) for f in sorted(os.listdir(CSRC))
+             if f.endswith(".hip") and f != INST]
+    for tower in (0, 1):
+        for tile in range(N_TILES):
+            units.append((INST, f"apply_t{tile}_w{tower}.o",
+                          [f"-DAPPLY_TILE={tile}", f"-DAPPLY_TOWER={tower}"]))
+    # longest first: the instantiation sets dominate the build
+    return sorted(units, key=lambda u: u[0] != INST)
 
 
-def compile_one(src):
-    obj = os.path.join(OBJ, src.replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def compile_one(unit):
+    src, obj_name, defs = unit
+    obj = os.path.join(OBJ, obj_name)
+    cmd = [HIPCC, *FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

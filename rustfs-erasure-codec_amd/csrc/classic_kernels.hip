@@ -1,120 +1,11 @@
 // classic_kernels.hip -- gfx950 kernels of the classic (matrix) Reed-Solomon
-// data path: the r x k matrix apply that serves encode, verify and
-// reconstruct, and the device-side solve that builds decode rows.
-//
-// Matrix apply: a wavefront owns a 1 KiB (NV = 1) or 2 KiB (NV = 2) column tile
-// of every shard.  Each lane reads its 16-byte vector of all k inputs with
-// 128-bit loads, keeps the r output accumulators in VGPRs and writes every
-// output once (encode, reconstruct), or compares it in registers against the
-// stored parity without writing (verify).  A GF(2^8) product c*x of a packed
-// dword is three v_perm_b32 byte lookups on the bit fields x[2:0], x[5:3],
-// x[7:6] (tables from make_perm_table, held in SGPRs), folded with 3-input
-// XORs (v_bitop3_b32).  The bit-field selectors depend only on the input and
-// are shared by all r outputs.
+// data path: the GF(2^8) instantiations of the tiled r x k matrix apply
+// (apply_kernel.h) that serves encode, verify and reconstruct, the byte kernel
+// for tails and unaligned views, and the device-side solve that builds decode
+// rows.
 #include "kernel_common.h"
 
 namespace rsgpu {
-
-
-// R outputs, NV vectors per lane, TOWER selects GF(2^16) tower element pairs.
-template <int R, int NV, int MODE, bool TOWER>
-__global__ __launch_bounds__(kBlock) void apply_kernel(const ApplyArgs a) {
-  constexpr int W = TOWER ? 4 * kTabWords : kTabWords;  // table dwords per coefficient
-  // Inputs per load group: every lane issues KB*NV 128-bit loads back to back
-  // before the arithmetic, so a k <= 16 stripe has all its input in flight.
-  constexpr int KB = NV == 1 ? 16 : 8;
-  const uint64_t tile = (uint64_t)kBlock * NV;
-  const uint64_t n_tiles = (a.n16 + tile - 1) / tile;
-  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    uint64_t v[NV];
-    bool ok[NV];
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      v[j] = t * tile + (uint64_t)j * kBlock + threadIdx.x;
-      ok[j] = v[j] < a.n16;
-      v[j] = (a.v_begin + v[j]) * 16;  // byte offset
-    }
-    uint32_t acc[R][NV][4];
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-      for (int j = 0; j < NV; ++j)
-#pragma unroll
-        for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
-
-    for (int i0 = 0; i0 < a.k; i0 += KB) {
-      uint4 x[KB][NV];
-#pragma unroll
-      for (int b = 0; b < KB; ++b)
-#pragma unroll
-        for (int j = 0; j < NV; ++j)
-          x[b][j] = (i0 + b < a.k && ok[j]) ? load16(a.in[i0 + b] + v[j]) : make_uint4(0, 0, 0, 0);
-#pragma unroll
-      for (int b = 0; b < KB; ++b) {
-        if (i0 + b >= a.k) continue;  // uniform; `continue` keeps the loop fully unrollable
-        ctab_t tcol = as_const(a.tables) + (size_t)(a.tab_col0 + i0 + b) * W;
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
-          uint32_t w[4];
-          if constexpr (TOWER) {
-            deinterleave(x[b][j], w);
-          } else {
-            w[0] = x[b][j].x;
-            w[1] = x[b][j].y;
-            w[2] = x[b][j].z;
-            w[3] = x[b][j].w;
-          }
-          Sel s[4];
-#pragma unroll
-          for (int d = 0; d < 4; ++d) s[d] = selectors(w[d]);
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            ctab_t tc = tcol + (size_t)r * a.tab_row_stride;
-            if constexpr (TOWER) {
-              // [A B; C D] per tower_coefficients: x' = A x ^ B c, c' = C x ^ D c
-#pragma unroll
-              for (int d = 0; d < 2; ++d) {
-                acc[r][j][d] ^= gmul(tc + 0 * kTabWords, s[d]) ^ gmul(tc + 1 * kTabWords, s[d + 2]);
-                acc[r][j][d + 2] ^= gmul(tc + 2 * kTabWords, s[d]) ^ gmul(tc + 3 * kTabWords, s[d + 2]);
-              }
-            } else {
-#pragma unroll
-              for (int d = 0; d < 4; ++d) acc[r][j][d] ^= gmul(tc, s[d]);
-            }
-          }
-        }
-      }
-    }
-
-    bool mismatch = false;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      if (!ok[j]) continue;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        uint4 o;
-        if constexpr (TOWER) {
-          o = interleave(acc[r][j]);
-        } else {
-          o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
-        }
-        uint8_t* dst = a.out[r] + v[j];
-        if constexpr (MODE == kWrite) {
-          store16(dst, o);
-        } else if constexpr (MODE == kAccumulate) {
-          uint4 p = load16(dst);
-          store16(dst, make_uint4(p.x ^ o.x, p.y ^ o.y, p.z ^ o.z, p.w ^ o.w));
-        } else {
-          uint4 p = load16(dst);
-          mismatch |= ((p.x ^ o.x) | (p.y ^ o.y) | (p.z ^ o.z) | (p.w ^ o.w)) != 0;
-        }
-      }
-    }
-    if constexpr (MODE == kVerify) {
-      if (mismatch) atomicOr(a.flag, 1);
-    }
-  }
-}
 
 // Byte-granular variant for shard tails (length % 16) and unaligned views.
 __device__ __forceinline__ uint8_t gmul_byte(const uint32_t* t, uint8_t x) {
@@ -170,44 +61,22 @@ __global__ void fill_i32_kernel(int32_t* p, int32_t v, int n) {
   if (i < n) p[i] = v;
 }
 
-template <int R, int NV, int MODE, bool TOWER>
-static hipError_t launch_t(const ApplyArgs& a, hipStream_t s) {
-  const uint64_t tile = (uint64_t)kBlock * NV;
-  uint64_t tiles = (a.n16 + tile - 1) / tile;
-  unsigned grid = (unsigned)(tiles < 1048576 ? (tiles ? tiles : 1) : 1048576);
-  hipLaunchKernelGGL((apply_kernel<R, NV, MODE, TOWER>), dim3(grid), dim3(kBlock), 0, s, a);
-  return hipGetLastError();
-}
+#define RSGPU_DECL(t, w) \
+  hipError_t launch_apply_t##t##_w##w(const ApplyArgs& a, int mode, int rows, hipStream_t s);
+RSGPU_DECL(0, 0) RSGPU_DECL(1, 0) RSGPU_DECL(2, 0) RSGPU_DECL(3, 0) RSGPU_DECL(4, 0)
+RSGPU_DECL(0, 1) RSGPU_DECL(1, 1) RSGPU_DECL(2, 1) RSGPU_DECL(3, 1) RSGPU_DECL(4, 1)
+#undef RSGPU_DECL
 
-template <int NV, int MODE, bool TOWER>
-static hipError_t launch_r(const ApplyArgs& a, hipStream_t s) {
-  switch (a.r) {
-    case 1: return launch_t<1, NV, MODE, TOWER>(a, s);
-    case 2: return launch_t<2, NV, MODE, TOWER>(a, s);
-    case 3: return launch_t<3, NV, MODE, TOWER>(a, s);
-    case 4: return launch_t<4, NV, MODE, TOWER>(a, s);
-    case 5: return launch_t<5, NV, MODE, TOWER>(a, s);
-    case 6: return launch_t<6, NV, MODE, TOWER>(a, s);
-    case 7: return launch_t<7, NV, MODE, TOWER>(a, s);
-    case 8: return launch_t<8, NV, MODE, TOWER>(a, s);
-  }
-  return hipErrorInvalidValue;
-}
+using ApplyFn = hipError_t (*)(const ApplyArgs&, int, int, hipStream_t);
+static const ApplyFn kApplyFns[2][kNumTiles] = {
+    {launch_apply_t0_w0, launch_apply_t1_w0, launch_apply_t2_w0, launch_apply_t3_w0, launch_apply_t4_w0},
+    {launch_apply_t0_w1, launch_apply_t1_w1, launch_apply_t2_w1, launch_apply_t3_w1, launch_apply_t4_w1}};
 
-template <int NV, bool TOWER>
-static hipError_t launch_m(const ApplyArgs& a, int mode, hipStream_t s) {
-  switch (mode) {
-    case kWrite: return launch_r<NV, kWrite, TOWER>(a, s);
-    case kAccumulate: return launch_r<NV, kAccumulate, TOWER>(a, s);
-    case kVerify: return launch_r<NV, kVerify, TOWER>(a, s);
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_apply(const ApplyArgs& a, int mode, int nv, int tower, hipStream_t s) {
+hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s) {
   if (a.n16 == 0) return hipSuccess;
-  if (tower) return nv == 2 ? launch_m<2, true>(a, mode, s) : launch_m<1, true>(a, mode, s);
-  return nv == 2 ? launch_m<2, false>(a, mode, s) : launch_m<1, false>(a, mode, s);
+  if (tile < 0 || tile >= kNumTiles || a.r < 1 || a.r > kMaxOut) return hipErrorInvalidValue;
+  if (rows < 1 || rows > a.r) rows = a.r;
+  return kApplyFns[tower ? 1 : 0][tile](a, mode, rows, s);
 }
 
 hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s) {
@@ -278,8 +147,8 @@ __global__ __launch_bounds__(256) void decode_rows_kernel(const uint16_t* __rest
   __shared__ uint8_t ex[512];
   extern __shared__ uint16_t smem[];  // [e][cols] work matrix, then [n_targets][k] rows
   const int tid = threadIdx.x;
-  for (int i = tid; i < 256; i += blockDim.x) lg[i] = kGf8.log[i];
-  for (int i = tid; i < 512; i += blockDim.x) ex[i] = kGf8.exp[i];
+  for (int i = tid; i < 256; i += blockDim.x) lg[i] = job.field ? job.field[i] : kGf8.log[i];
+  for (int i = tid; i < 512; i += blockDim.x) ex[i] = job.field ? job.field[256 + i] : kGf8.exp[i];
   const int k = job.k;
   int p = 0;  // present data among survivors (a sorted prefix of valid)
   while (p < k && job.valid[p] < k) ++p;
@@ -309,7 +178,7 @@ __global__ __launch_bounds__(256) void decode_rows_kernel(const uint16_t* __rest
     uint16_t v;
     if (c < e) v = prow[missing[c]];
     else if (c < e + p) v = prow[job.valid[c - e]];
-    else v = (c - e - p == r) ? 1 : 0;
+    else v = (c - e - p == r) ? (uint16_t)job.one : 0;
     wm[idx] = v;
   }
   __syncthreads();
@@ -386,7 +255,7 @@ __global__ __launch_bounds__(256) void decode_rows_kernel(const uint16_t* __rest
       for (int q = 0; q < 4; ++q) make_perm_table(co[q], dst + q * kTabWords);
     } else {
       rows_out[idx] = (uint8_t)v;
-      make_perm_table((uint8_t)v, dst);
+      make_perm_table_with([&](uint8_t a, uint8_t b) { return (uint8_t)F.mul(a, b); }, (uint8_t)v, dst);
     }
   }
   if (tid == 0) *status = RSGPU_OK;
@@ -406,7 +275,8 @@ hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, ui
 
 int words_per_coef(int tower) { return tower ? 4 * kTabWords : kTabWords; }
 
-void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out) {
+void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out,
+                       const FieldTables* field) {
   const int W = words_per_coef(tower);
   out.assign((size_t)r * k * W, 0);
   for (int i = 0; i < r * k; ++i) {
@@ -416,7 +286,10 @@ void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vect
       tower_coefficients(rows[i], co);
       for (int q = 0; q < 4; ++q) make_perm_table(co[q], dst + q * kTabWords);
     } else {
-      make_perm_table((uint8_t)rows[i], dst);
+      if (field)
+        make_perm_table_with([field](uint8_t a, uint8_t b) { return field->mul(a, b); }, (uint8_t)rows[i], dst);
+      else
+        make_perm_table((uint8_t)rows[i], dst);
     }
   }
 }

@@ -3,6 +3,7 @@
 // order, reconstruction planning, the device-resident decode-row cache, and the
 // launch plumbing for the matrix-apply kernels.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "batch.h"
@@ -193,7 +194,12 @@ static int upload_codec(Codec& c) {
   HIP_TRY(hipMemcpy(c.d_matrix.p, c.matrix.data(), c.matrix.size() * sizeof(uint16_t),
                     hipMemcpyHostToDevice));
   build_tables_host(c.matrix.data() + c.k * c.k, (int)c.m, (int)c.k, c.field == RSGPU_FIELD_GF16,
-                    c.h_parity_tables);
+                    c.h_parity_tables, c.ftab.get());
+  if (c.ftab) {
+    HIP_TRY(c.d_ftab.alloc(768));
+    HIP_TRY(hipMemcpy(c.d_ftab.p, c.ftab->log, 256, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((uint8_t*)c.d_ftab.p + 256, c.ftab->exp, 512, hipMemcpyHostToDevice));
+  }
   HIP_TRY(c.d_parity_tables.alloc(c.h_parity_tables.size() * 4));
   HIP_TRY(hipMemcpy(c.d_parity_tables.p, c.h_parity_tables.data(), c.h_parity_tables.size() * 4,
                     hipMemcpyHostToDevice));
@@ -213,7 +219,7 @@ static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 // into kMaxOut chunks.  mode kVerify compares with `out` and sets *flag.
 static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
                      const std::vector<const uint8_t*>& in, const std::vector<uint8_t*>& out,
-                     size_t len, int mode, int32_t* flag, hipStream_t s, int nv = 0) {
+                     size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1) {
   const int W = words_per_coef(tower);
   bool vec_ok = true;
   for (auto p : in) vec_ok &= aligned16(p);
@@ -221,8 +227,10 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
   if (mode == kVerify && k_total > kMaxIn) return RSGPU_ERR_UNSUPPORTED;  // handled by caller
   const uint64_t n16 = vec_ok ? len / 16 : 0;
   const size_t tail_begin = (size_t)n16 * 16;
-  // Two vectors per lane only when that still leaves >= 8 workgroups per CU.
-  if (nv == 0) nv = n16 >= (uint64_t)kBlock * 2 * 2048 ? 2 : 1;
+  // Default tiling: 256-lane workgroups with every output, two vectors per lane
+  // only when that still leaves >= 8 workgroups per CU.
+  if (variant < 0 || variant >= kNumApplyVariants) variant = n16 >= (uint64_t)kBlock * 2 * 2048 ? 1 : 0;
+  const int tile = variant % kNumTiles, split = kRowSplit[variant / kNumTiles];
   for (int r0 = 0; r0 < r_total; r0 += kMaxOut) {
     const int rn = std::min(kMaxOut, r_total - r0);
     for (int k0 = 0; k0 < k_total; k0 += kMaxIn) {
@@ -240,7 +248,7 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
         a.tab_row_stride = k_total * W;
         a.tab_col0 = k0;
         a.r = rn;
-        HIP_TRY(launch_apply(a, m, nv, tower ? 1 : 0, s));
+        HIP_TRY(launch_apply(a, m, tile, split ? std::min(split, rn) : rn, tower ? 1 : 0, s));
       }
       if (tail_begin < len) {
         ByteArgs b{};
@@ -292,9 +300,90 @@ static int32_t* thread_flag(int device) {
   return (int32_t*)t_flag.p;
 }
 
+// ------------------------------------------------------------ variant dispatch
+constexpr int kPick = -2;  // impl argument: take the variant from the codec's dispatch state
+
+static bool variant_valid(const Codec& c, int op, size_t erasures, int v) {
+  if (v < 0 || v >= kNumVariants) return false;
+  if (c.family == RSGPU_FAMILY_CLASSIC) return v < kNumApplyVariants;
+  if (v >= kNumApplyVariants) return true;  // Leopard FFT column tiles
+  if (c.family != RSGPU_FAMILY_LEOPARD_GF8) return false;
+  // The matrix path rebuilds the FFT decoder's bytes only when exactly k shards
+  // are present (every input is then a codeword).
+  return op != kOpReconstruct || erasures == c.m;
+}
+
+static int size_bucket(size_t len) {
+  int b = (int)std::lround(2.0 * std::log2((double)std::max<size_t>(len, 1))) - kBucket0;
+  return std::min(std::max(b, 0), kNumBuckets - 1);
+}
+
+// The variant for one call: the forced one, else the dispatch table's entry;
+// -1 (default tiling) when neither applies or the entry is not exact here.
+static int pick_variant(const Codec& c, int op, size_t erasures, size_t len) {
+  int v = c.forced_variant.load(std::memory_order_relaxed);
+  if (v < 0) {
+    std::shared_ptr<const VariantTable> t = std::atomic_load(&c.variant_table);
+    if (!t || t->n_erasures <= 0) return -1;
+    const size_t e = std::min<size_t>(erasures, (size_t)t->n_erasures - 1);
+    v = t->v[((size_t)op * t->n_erasures + e) * kNumBuckets + size_bucket(len)];
+  }
+  return variant_valid(c, op, erasures, v) ? v : -1;
+}
+
+static bool is_matrix_variant(const Codec& c, int v) {
+  return c.family == RSGPU_FAMILY_LEOPARD_GF8 && v >= 0 && v < kNumApplyVariants;
+}
+
+// Classic codec over a Leopard GF(2^8) codec's generator matrix [I; G], in
+// Leopard's byte field.  G is read off one FFT encode of unit columns: data
+// shard i holds the field's identity at byte i (zeros elsewhere), so parity
+// shard j holds G[j][i] at byte i.
+static int leo_matrix_codec(const Codec& c, hipStream_t s, std::shared_ptr<Codec>& out) {
+  std::lock_guard<std::mutex> g(c.mu);
+  if (c.leo_matrix) {
+    out = c.leo_matrix;
+    return RSGPU_OK;
+  }
+  const size_t k = c.k, m = c.m, len = (k + 63) / 64 * 64;
+  DeviceBuffer buf;
+  HIP_TRY(buf.alloc((k + m) * len));
+  std::shared_ptr<const FieldTables> ft = leopard8_field_tables();
+  std::vector<uint8_t> host((k + m) * len, 0);
+  for (size_t i = 0; i < k; ++i) host[i * len + i] = ft->one;
+  HIP_TRY(hipMemcpyAsync(buf.p, host.data(), k * len, hipMemcpyHostToDevice, s));
+  std::vector<rsgpu_cslice> data(k);
+  std::vector<rsgpu_slice> par(m);
+  for (size_t i = 0; i < k; ++i) data[i] = {(const uint8_t*)buf.p + i * len, len};
+  for (size_t j = 0; j < m; ++j) par[j] = {(uint8_t*)buf.p + (k + j) * len, len};
+  int rc = leopard_encode(c, data.data(), par.data(), len, s, -1);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(host.data() + k * len, (uint8_t*)buf.p + k * len, m * len,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  auto mc = std::make_shared<Codec>();
+  mc->k = k;
+  mc->m = m;
+  mc->field = RSGPU_FIELD_GF8;
+  mc->family = RSGPU_FAMILY_CLASSIC;
+  mc->matrix_mode = RSGPU_MATRIX_VANDERMONDE;  // MDS: every k survivors solve
+  mc->use_cache = c.use_cache;
+  mc->cache_capacity = c.cache_capacity;
+  mc->cache_max_bytes = c.cache_max_bytes;
+  mc->ftab = ft;
+  mc->matrix.assign((k + m) * k, 0);
+  for (size_t i = 0; i < k; ++i) mc->matrix[i * k + i] = ft->one;
+  for (size_t j = 0; j < m; ++j)
+    for (size_t i = 0; i < k; ++i) mc->matrix[(k + j) * k + i] = host[(k + j) * len + i];
+  if ((rc = upload_codec(*mc))) return rc;
+  c.leo_matrix = mc;
+  out = mc;
+  return RSGPU_OK;
+}
+
 // ------------------------------------------------------------ encode / verify
 static int encode_sep_impl(const Codec& c, const rsgpu_cslice* data, size_t nd,
-                           const rsgpu_slice* parity, size_t np, hipStream_t s) {
+                           const rsgpu_slice* parity, size_t np, hipStream_t s, int variant = kPick) {
   if (nd < c.k) return RSGPU_TOO_FEW_DATA_SHARDS;
   if (nd > c.k) return RSGPU_TOO_MANY_DATA_SHARDS;
   if (np < c.m) return RSGPU_TOO_FEW_PARITY_SHARDS;
@@ -308,17 +397,25 @@ static int encode_sep_impl(const Codec& c, const rsgpu_cslice* data, size_t nd,
     if (!data[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
   for (size_t i = 0; i < np; ++i)
     if (!parity[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
-  if (c.family != RSGPU_FAMILY_CLASSIC) return leopard_encode(c, data, parity, data[0].len, s);
+  const size_t len = data[0].len;
+  if (variant == kPick) variant = pick_variant(c, kOpEncode, 0, len);
+  if (is_matrix_variant(c, variant)) {
+    if (len % 64) return RSGPU_INCORRECT_SHARD_SIZE;  // leopard.rs:50-56
+    std::shared_ptr<Codec> mc;
+    if ((rc = leo_matrix_codec(c, s, mc))) return rc;
+    return encode_sep_impl(*mc, data, nd, parity, np, s, variant);
+  }
+  if (c.family != RSGPU_FAMILY_CLASSIC) return leopard_encode(c, data, parity, len, s, variant);
   std::vector<const uint8_t*> in(nd);
   std::vector<uint8_t*> out(np);
   for (size_t i = 0; i < nd; ++i) in[i] = data[i].ptr;
   for (size_t i = 0; i < np; ++i) out[i] = parity[i].ptr;
   return run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m,
-                   c.field == RSGPU_FIELD_GF16, in, out, data[0].len, kWrite, nullptr, s);
+                   c.field == RSGPU_FIELD_GF16, in, out, len, kWrite, nullptr, s, variant);
 }
 
 static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* d_flag,
-                       hipStream_t s) {
+                       hipStream_t s, int variant = kPick) {
   const size_t total = c.k + c.m;
   if (n < total) return RSGPU_TOO_FEW_SHARDS;
   if (n > total) return RSGPU_TOO_MANY_SHARDS;
@@ -327,8 +424,16 @@ static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int
   if ((rc = check_elem_len(c, shards[0].len))) return rc;
   for (size_t i = 0; i < n; ++i)
     if (!shards[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (variant == kPick) variant = pick_variant(c, kOpVerify, 0, shards[0].len);
+  if (is_matrix_variant(c, variant)) {
+    if (shards[0].len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
+    std::shared_ptr<Codec> mc;
+    if ((rc = leo_matrix_codec(c, s, mc))) return rc;
+    return verify_impl(*mc, shards, n, d_flag, s, variant);
+  }
   HIP_TRY(launch_fill_i32(d_flag, 0, 1, s));
-  if (c.family != RSGPU_FAMILY_CLASSIC) return leopard_verify(c, shards, shards[0].len, d_flag, s);
+  if (c.family != RSGPU_FAMILY_CLASSIC)
+    return leopard_verify(c, shards, shards[0].len, d_flag, s, variant);
   std::vector<const uint8_t*> in(c.k);
   std::vector<uint8_t*> out(c.m);
   for (size_t i = 0; i < c.k; ++i) in[i] = shards[i].ptr;
@@ -336,7 +441,7 @@ static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int
   const bool tower = c.field == RSGPU_FIELD_GF16;
   if (c.k <= (size_t)kMaxIn)
     return run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m, tower, in, out,
-                     shards[0].len, kVerify, d_flag, s);
+                     shards[0].len, kVerify, d_flag, s, variant);
   // Wide codecs: recompute into scratch, then compare (two passes).
   const size_t len = shards[0].len;
   DeviceBuffer scratch;
@@ -344,7 +449,7 @@ static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int
   std::vector<uint8_t*> tmp(c.m);
   for (size_t j = 0; j < c.m; ++j) tmp[j] = (uint8_t*)scratch.p + j * len;
   rc = run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m, tower, in, tmp, len,
-                 kWrite, nullptr, s);
+                 kWrite, nullptr, s, variant);
   if (rc) return rc;
   // identity rows compare scratch vs stored parity: apply the unit matrix in verify mode
   std::vector<uint16_t> eye(c.m * c.m, 0);
@@ -468,6 +573,8 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
   job.m = (int)c.m;
   job.n_targets = (int)nt;
   job.tower = tower ? 1 : 0;
+  job.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
+  job.one = c.ftab ? c.ftab->one : 1;
   for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)plan.valid[i];
   for (size_t i = 0; i < nt; ++i) job.targets[i] = (uint16_t)plan.targets[i];
   int32_t* st = thread_flag(c.device);
@@ -499,9 +606,18 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
 }
 
 static int reconstruct_impl(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
-                            const uint8_t* required, size_t n_required, hipStream_t s) {
+                            const uint8_t* required, size_t n_required, hipStream_t s,
+                            int variant = kPick) {
+  if (variant == kPick) {
+    size_t erased = 0, len = 0;
+    for (size_t i = 0; i < std::min(n, c.k + c.m); ++i) {
+      if (!shards[i].present) ++erased;
+      else if (!len) len = shards[i].len;
+    }
+    variant = pick_variant(c, kOpReconstruct, erased, len);
+  }
   if (c.family != RSGPU_FAMILY_CLASSIC)
-    return leopard_reconstruct(c, shards, n, goal, required, n_required, s);
+    return leopard_reconstruct(c, shards, n, goal, required, n_required, s, variant);
   Plan plan;
   int rc = make_plan(c, shards, n, goal, required, n_required, plan);
   if (rc || plan.all_present || plan.targets.empty()) return rc;
@@ -512,12 +628,20 @@ static int reconstruct_impl(const Codec& c, rsgpu_slot* shards, size_t n, int go
   for (size_t i = 0; i < c.k; ++i) in[i] = shards[plan.valid[i]].ptr;
   for (size_t t = 0; t < plan.targets.size(); ++t) out[t] = shards[plan.targets[t]].ptr;
   rc = run_apply((const uint32_t*)e->tables.p, (int)c.k, (int)plan.targets.size(),
-                 c.field == RSGPU_FIELD_GF16, in, out, plan.shard_len, kWrite, nullptr, s);
+                 c.field == RSGPU_FIELD_GF16, in, out, plan.shard_len, kWrite, nullptr, s, variant);
   if (rc) return rc;
   // An entry evicted meanwhile is released when `e` goes out of scope; hipFree
   // in its destructor waits for the queued kernels that still read it.
   for (int t : plan.targets) shards[t].present = 1;
   return RSGPU_OK;
+}
+
+int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
+                               const uint8_t* required, size_t n_required, hipStream_t s, int variant) {
+  std::shared_ptr<Codec> mc;
+  int rc = leo_matrix_codec(c, s, mc);
+  if (rc) return rc;
+  return reconstruct_impl(*mc, shards, n, goal, required, n_required, s, variant);
 }
 
 // ------------------------------------------------------------ batch stripes
@@ -855,6 +979,62 @@ int rsgpu_verify_with_buffer(const rsgpu_codec* h, const rsgpu_cslice* shards, s
   HIP_TRY(hipMemcpyAsync(&host, flag, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   *ok = host == 0 ? 1 : 0;
+  return RSGPU_OK;
+}
+
+int rsgpu_variant_count(void) { return kNumVariants; }
+
+int rsgpu_dispatch_dims(int32_t* n_ops, int32_t* n_buckets, int32_t* bucket0) {
+  if (!n_ops || !n_buckets || !bucket0) return RSGPU_ERR_INVALID_ARGUMENT;
+  *n_ops = kNumOps;
+  *n_buckets = kNumBuckets;
+  *bucket0 = kBucket0;
+  return RSGPU_OK;
+}
+
+int rsgpu_variant_info(int32_t variant, int32_t* kind, int32_t* lanes, int32_t* vectors_per_lane,
+                       int32_t* rows_per_group, int32_t* leopard_tile_words) {
+  if (variant < 0 || variant >= kNumVariants || !kind || !lanes || !vectors_per_lane ||
+      !rows_per_group || !leopard_tile_words)
+    return RSGPU_ERR_INVALID_ARGUMENT;
+  *kind = variant < kNumApplyVariants ? 0 : 1;
+  *lanes = variant < kNumApplyVariants ? kTileShapes[variant % kNumTiles].lanes : 256;
+  *vectors_per_lane = variant < kNumApplyVariants ? kTileShapes[variant % kNumTiles].nv : 0;
+  *rows_per_group = variant < kNumApplyVariants ? kRowSplit[variant / kNumTiles] : 0;
+  *leopard_tile_words = variant < kNumApplyVariants ? 0 : kLeoTileWords[variant - kNumApplyVariants];
+  return RSGPU_OK;
+}
+
+int rsgpu_variant_valid(const rsgpu_codec* h, int32_t op, size_t erasures, int32_t variant) {
+  if (!h || op < 0 || op >= kNumOps) return 0;
+  return variant_valid(h->c, op, erasures, variant) ? 1 : 0;
+}
+
+int rsgpu_codec_set_variant(rsgpu_codec* h, int32_t variant) {
+  if (!h || variant < -1 || variant >= kNumVariants) return RSGPU_ERR_INVALID_ARGUMENT;
+  h->c.forced_variant.store(variant, std::memory_order_relaxed);
+  return RSGPU_OK;
+}
+
+int rsgpu_codec_set_variant_table(rsgpu_codec* h, const int16_t* table, size_t n_erasures) {
+  if (!h || (table && n_erasures == 0)) return RSGPU_ERR_INVALID_ARGUMENT;
+  std::shared_ptr<const VariantTable> t;
+  if (table) {
+    auto nt = std::make_shared<VariantTable>();
+    nt->n_erasures = (int)n_erasures;
+    nt->v.assign(table, table + (size_t)kNumOps * n_erasures * kNumBuckets);
+    for (int16_t v : nt->v)
+      if (v < -1 || v >= kNumVariants) return RSGPU_ERR_INVALID_ARGUMENT;
+    t = nt;
+  }
+  std::atomic_store(&h->c.variant_table, t);
+  return RSGPU_OK;
+}
+
+int rsgpu_codec_variant_for(const rsgpu_codec* h, int32_t op, size_t erasures, size_t shard_len,
+                            int32_t* variant) {
+  if (!h || !variant || op < 0 || op >= kNumOps) return RSGPU_ERR_INVALID_ARGUMENT;
+  *variant = pick_variant(h->c, op, erasures, shard_len);
   return RSGPU_OK;
 }
 

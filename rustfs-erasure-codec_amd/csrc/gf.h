@@ -94,18 +94,39 @@ inline uint16_t gf16_exp(uint16_t a, size_t n) {
 //   w[4] = c*{0,64,128,192}   w[5..7] = 0 (pads the entry to 32 bytes)
 constexpr int kTabWords = 8;
 
-__host__ __device__ inline void make_perm_table(uint8_t c, uint32_t* w) {
+// The split needs only that x -> c*x is GF(2)-linear, which holds for
+// multiplication by a constant in any basis, so `mul` may be any field's
+// byte multiplication (the classic 0x11d field, or Leopard's).
+template <class Mul>
+__host__ __device__ inline void make_perm_table_with(Mul mul, uint8_t c, uint32_t* w) {
   uint32_t v[5] = {0, 0, 0, 0, 0};
   for (int i = 0; i < 4; ++i) {
-    v[0] |= (uint32_t)gf8_mul_c(c, (uint8_t)i) << (8 * i);
-    v[1] |= (uint32_t)gf8_mul_c(c, (uint8_t)(i + 4)) << (8 * i);
-    v[2] |= (uint32_t)gf8_mul_c(c, (uint8_t)(8 * i)) << (8 * i);
-    v[3] |= (uint32_t)gf8_mul_c(c, (uint8_t)(8 * (i + 4))) << (8 * i);
-    v[4] |= (uint32_t)gf8_mul_c(c, (uint8_t)(64 * i)) << (8 * i);
+    v[0] |= (uint32_t)mul(c, (uint8_t)i) << (8 * i);
+    v[1] |= (uint32_t)mul(c, (uint8_t)(i + 4)) << (8 * i);
+    v[2] |= (uint32_t)mul(c, (uint8_t)(8 * i)) << (8 * i);
+    v[3] |= (uint32_t)mul(c, (uint8_t)(8 * (i + 4))) << (8 * i);
+    v[4] |= (uint32_t)mul(c, (uint8_t)(64 * i)) << (8 * i);
   }
   for (int i = 0; i < 5; ++i) w[i] = v[i];
   w[5] = w[6] = w[7] = 0;
 }
+
+struct ClassicMul8 {
+  __host__ __device__ uint8_t operator()(uint8_t a, uint8_t b) const { return gf8_mul_c(a, b); }
+};
+
+__host__ __device__ inline void make_perm_table(uint8_t c, uint32_t* w) {
+  make_perm_table_with(ClassicMul8{}, c, w);
+}
+
+// A GF(2^8) given by log/exp tables (exp repeated over 512 entries so that
+// exp[log a + log b] needs no reduction) and its multiplicative identity.
+struct FieldTables {
+  uint8_t log[256];
+  uint8_t exp[512];
+  uint8_t one;
+  uint8_t mul(uint8_t a, uint8_t b) const { return (a == 0 || b == 0) ? 0 : exp[log[a] + log[b]]; }
+};
 
 // Tower multiply by e = [cx, cc] as a 2x2 GF(2^8) map on the byte planes
 // (src/galois_16.rs:100-119, tower_mul_coefficients :224-231):

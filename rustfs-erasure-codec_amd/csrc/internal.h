@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstddef>
 #include <list>
 #include <memory>
@@ -50,10 +51,35 @@ struct ByteArgs {
   int32_t tower;  // 1: GF(2^16) tower elements (byte pairs)
 };
 
-// Launchers (classic_kernels.hip).  `nv` selects the vectors-per-thread
-// variant (1 or 2).  `tower` selects GF((2^8)^2) element pairs; tables are then
-// the 2x2-expanded GF(2^8) coefficients [row][input][4][kTabWords].
-hipError_t launch_apply(const ApplyArgs& a, int mode, int nv, int tower, hipStream_t s);
+// Kernel variants a call can be dispatched to.  Every variant computes the same
+// bytes; they differ only in how the work is tiled.
+//   0 .. kNumApplyVariants-1: matrix apply with tile shape kTileShapes[v % kNumTiles]
+//       and kRowSplit[v / kNumTiles] outputs per workgroup (0: all of them).  On a
+//       Leopard GF(2^8) codec these apply its generator matrix instead of the FFT.
+//   kNumApplyVariants ..: Leopard FFT kernels with a column tile of
+//       kLeoTileWords[v - kNumApplyVariants] words per workgroup (clamped to LDS).
+struct TileShape {
+  int lanes;  // lanes per workgroup
+  int nv;     // 16-byte vectors per lane
+};
+constexpr int kNumTiles = 5;
+constexpr TileShape kTileShapes[kNumTiles] = {{256, 1}, {256, 2}, {64, 1}, {64, 2}, {64, 4}};
+constexpr int kNumRowSplits = 3;
+constexpr int kRowSplit[kNumRowSplits] = {0, 2, 1};
+constexpr int kNumApplyVariants = kNumTiles * kNumRowSplits;
+constexpr int kNumLeoTiles = 5;
+constexpr int kLeoTileWords[kNumLeoTiles] = {4, 8, 16, 32, 64};
+constexpr int kNumVariants = kNumApplyVariants + kNumLeoTiles;
+// Dispatch-table shard-size buckets: round(2 * log2(len)) - kBucket0, clamped.
+constexpr int kBucket0 = 12;
+constexpr int kNumBuckets = 50;
+enum DispatchOp : int { kOpEncode = 0, kOpVerify = 1, kOpReconstruct = 2, kNumOps = 3 };
+
+// Launchers (classic_kernels.hip; the apply kernels are compiled from apply_inst.hip).  `tile` indexes
+// kTileShapes, `rows` is the number of outputs per workgroup.  `tower` selects
+// GF((2^8)^2) element pairs; tables are then the 2x2-expanded GF(2^8)
+// coefficients [row][input][4][kTabWords].
+hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s);
 hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t s);
 
@@ -64,12 +90,15 @@ struct DecodeJob {
   uint16_t valid[256];    // survivor shard indices (first k present), k entries
   uint16_t targets[256];  // requested outputs: missing data first, then parity
   int32_t k, m, n_targets, tower;
+  const uint8_t* field;   // device FieldTables log[256] + exp[512] (nullptr: 0x11d field)
+  int32_t one;            // the field's multiplicative identity
 };
 hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
                               uint32_t* d_tables, int32_t* d_status, hipStream_t s);
 
 // Tables for a host-known row set (encode rows, apply_rows primitive).
-void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out);
+void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out,
+                       const FieldTables* field = nullptr);
 
 struct DeviceBuffer {
   void* p = nullptr;
@@ -88,6 +117,11 @@ struct DecodeEntry {
   std::vector<int> valid;  // survivor indices
   std::vector<int> targets;
   size_t bytes = 0;
+};
+
+struct VariantTable {
+  int n_erasures = 0;      // erasure counts 0 .. n_erasures-1 (larger ones clamp)
+  std::vector<int16_t> v;  // [kNumOps][n_erasures][kNumBuckets]
 };
 
 struct Codec {
@@ -113,6 +147,17 @@ struct Codec {
   mutable uint64_t requests = 0, hits = 0, misses = 0;
   // Leopard device state (leopard.hip)
   std::shared_ptr<void> leopard;
+  // Kernel-variant dispatch (codec.hip): a forced variant (>= 0), else the
+  // installed table [op][erasures][size bucket] (entries < 0: default tiling).
+  std::atomic<int> forced_variant{-1};
+  std::shared_ptr<const VariantTable> variant_table;  // std::atomic_load/store
+  // Leopard GF(2^8): classic codec over the generator matrix [I; G], built on the
+  // first call that dispatches to a matrix-apply variant (guarded by mu).
+  mutable std::shared_ptr<Codec> leo_matrix;
+  // Byte field of the matrix when it is not the 0x11d field (the generator-matrix
+  // codec of a Leopard GF(2^8) codec), with a device copy of log + exp.
+  std::shared_ptr<const FieldTables> ftab;
+  DeviceBuffer d_ftab;
 };
 
 int words_per_coef(int tower);  // kTabWords or 4 * kTabWords

@@ -4,14 +4,26 @@
 
 namespace rsgpu {
 
+// `variant` (see kNumVariants) picks the FFT column tile; -1 or a matrix-apply
+// variant keeps the default tile.  Launches are asynchronous on `s`.
 int init_leopard(Codec& c);
 int leopard_encode(const Codec& c, const rsgpu_cslice* data, const rsgpu_slice* parity, size_t len,
-                   hipStream_t s);
-// Re-encodes into library scratch and compares; sets *d_flag nonzero on mismatch.
+                   hipStream_t s, int variant);
+// Re-encodes in registers and compares; sets *d_flag nonzero on mismatch.
 int leopard_verify(const Codec& c, const rsgpu_cslice* shards, size_t len, int32_t* d_flag,
-                   hipStream_t s);
-// goal: 0 all, 1 data only, 2 required flags
+                   hipStream_t s, int variant);
+// goal: 0 all, 1 data only, 2 required flags.  A matrix-apply variant on a GF(2^8)
+// codec with exactly k shards present goes to leopard_matrix_reconstruct.
 int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
-                        const uint8_t* required, size_t n_required, hipStream_t s);
+                        const uint8_t* required, size_t n_required, hipStream_t s, int variant);
+
+// Leopard GF(2^8) byte field as log/exp tables (its Cantor-basis representation
+// multiplies differently from the 0x11d field of the classic codecs).
+std::shared_ptr<const FieldTables> leopard8_field_tables();
+
+// Generator-matrix path of a Leopard GF(2^8) codec (codec.hip): the operation on
+// the classic matrix-apply kernels over [I; G], where G is the codec's parity map.
+int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
+                               const uint8_t* required, size_t n_required, hipStream_t s, int variant);
 
 }  // namespace rsgpu

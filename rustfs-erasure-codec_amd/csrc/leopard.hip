@@ -617,15 +617,162 @@ __global__ void leo_errloc_kernel(const uint32_t* __restrict__ lg, const uint32_
 }
 
 // ------------------------------------------------------------------- host
+std::shared_ptr<const FieldTables> leopard8_field_tables() {
+  static std::shared_ptr<const FieldTables> t = [] {
+    const LeoField& F = field8();
+    auto ft = std::make_shared<FieldTables>();
+    std::memset(ft.get(), 0, sizeof(FieldTables));
+    for (uint32_t a = 1; a < F.order; ++a) ft->log[a] = (uint8_t)(F.log[a] % F.modulus);
+    for (uint32_t i = 0; i < 512; ++i) ft->exp[i] = (uint8_t)F.exp[i % F.modulus];
+    ft->one = (uint8_t)F.exp[0];
+    return ft;
+  }();
+  return t;
+}
+
 static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopard.get()); }
 
-static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words) {
+// Column tile of a launch: as many words as fit LDS for `rows` lanes, at most 64,
+// or the smaller `want` a dispatched variant asks for.
+static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0) {
   const uint32_t wb = s.gf16 ? 8 : 4;
   uint64_t C = s.lds_limit / ((uint64_t)rows * wb);
   if (C > 64) C = 64;
+  if (want && C > want) C = want;
   if (C < 1) C = 1;
   if (C > words) C = (uint32_t)words;
   return (uint32_t)C;
+}
+
+static uint32_t variant_tile(int variant) {
+  return (variant >= kNumApplyVariants && variant < kNumVariants)
+             ? (uint32_t)kLeoTileWords[variant - kNumApplyVariants]
+             : 0;
+}
+
+// Per-thread staging for the shard-pointer and flag tables of Leopard launches
+// (too large for kernel arguments at up to 65536 shards): pinned host memory
+// copied on the launch stream into a device ring.  A span is reused only after
+// the event recorded behind the kernel that reads it has completed, so calls
+// neither allocate nor synchronise in the steady state.
+struct ArgRing {
+  struct Span {
+    size_t b, e;
+    hipEvent_t ev;
+  };
+  int device = -1;
+  uint8_t* host = nullptr;
+  uint8_t* dev = nullptr;
+  size_t cap = 0, head = 0;
+  std::vector<Span> live;
+  std::vector<hipEvent_t> spare;
+
+  hipError_t drain() {
+    for (auto& sp : live) {
+      hipError_t e = hipEventSynchronize(sp.ev);
+      spare.push_back(sp.ev);
+      if (e != hipSuccess) return e;
+    }
+    live.clear();
+    return hipSuccess;
+  }
+  // Reserves n bytes; *off receives the offset (host + off, dev + off).
+  hipError_t reserve(size_t n, size_t* off) {
+    n = (n + 255) & ~(size_t)255;
+    int d = 0;
+    hipError_t e = hipGetDevice(&d);
+    if (e != hipSuccess) return e;
+    if (d != device || n > cap) {
+      if ((e = drain()) != hipSuccess) return e;
+      if (host) (void)hipHostFree(host);
+      if (dev) (void)hipFree(dev);
+      host = dev = nullptr;
+      for (auto ev : spare) (void)hipEventDestroy(ev);
+      spare.clear();
+      cap = std::max<size_t>(n, (size_t)1 << 20);
+      head = 0;
+      device = d;
+      if ((e = hipHostMalloc((void**)&host, cap, hipHostMallocDefault)) != hipSuccess) return e;
+      if ((e = hipMalloc((void**)&dev, cap)) != hipSuccess) return e;
+    }
+    if (head + n > cap) head = 0;
+    const size_t b = head, en = head + n;
+    size_t keep = 0;
+    for (size_t i = 0; i < live.size(); ++i) {
+      Span sp = live[i];
+      bool overlap = sp.b < en && b < sp.e;
+      if (overlap || hipEventQuery(sp.ev) == hipSuccess) {
+        if (overlap && (e = hipEventSynchronize(sp.ev)) != hipSuccess) return e;
+        spare.push_back(sp.ev);
+      } else {
+        live[keep++] = sp;
+      }
+    }
+    live.resize(keep);
+    head = en;
+    *off = b;
+    return hipSuccess;
+  }
+  // Marks [off, off + n) busy until the work queued so far on s completes.
+  hipError_t commit(size_t off, size_t n, hipStream_t s) {
+    hipEvent_t ev;
+    if (!spare.empty()) {
+      ev = spare.back();
+      spare.pop_back();
+    } else {
+      hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipEventRecord(ev, s);
+    if (e != hipSuccess) {
+      spare.push_back(ev);
+      return e;
+    }
+    live.push_back({off, off + ((n + 255) & ~(size_t)255), ev});
+    return hipSuccess;
+  }
+};
+
+// Owned by its thread for the life of the process (not destroyed at thread exit,
+// when the HIP runtime may already be gone).
+static ArgRing& arg_ring() {
+  static thread_local ArgRing* r = new ArgRing();
+  return *r;
+}
+
+// Copies the pointer table (and optional flag bytes behind it) to the device on
+// stream s; *dptrs / *dflags receive the device addresses.
+static int stage_args(const std::vector<const uint8_t*>& ptrs, const std::vector<uint8_t>* flags,
+                      hipStream_t s, const uint8_t* const** dptrs, const uint8_t** dflags, size_t* off,
+                      size_t* bytes) {
+  ArgRing& ring = arg_ring();
+  const size_t pb = ptrs.size() * sizeof(void*);
+  const size_t n = pb + (flags ? flags->size() : 0);
+  hipError_t e = ring.reserve(n, off);
+  if (e != hipSuccess) {
+    set_hip_error(e, "leopard arg ring");
+    return RSGPU_ERR_OUT_OF_MEMORY;
+  }
+  std::memcpy(ring.host + *off, ptrs.data(), pb);
+  if (flags) std::memcpy(ring.host + *off + pb, flags->data(), flags->size());
+  e = hipMemcpyAsync(ring.dev + *off, ring.host + *off, n, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    set_hip_error(e, "leopard arg copy");
+    return RSGPU_ERR_HIP;
+  }
+  *dptrs = (const uint8_t* const*)(ring.dev + *off);
+  if (dflags) *dflags = ring.dev + *off + pb;
+  *bytes = n;
+  return RSGPU_OK;
+}
+
+static int release_args(size_t off, size_t bytes, hipStream_t s) {
+  hipError_t e = arg_ring().commit(off, bytes, s);
+  if (e != hipSuccess) {
+    set_hip_error(e, "leopard arg ring");
+    return RSGPU_ERR_HIP;
+  }
+  return RSGPU_OK;
 }
 
 int init_leopard(Codec& c) {
@@ -685,19 +832,17 @@ int init_leopard(Codec& c) {
 }
 
 static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs, size_t len, int mode,
-                         int32_t* flag, hipStream_t s, DeviceBuffer& dptrs) {
+                         int32_t* flag, hipStream_t s, int variant) {
   LeoState& st = state(c);
   const uint64_t words = len / (st.gf16 ? 8 : 4);
-  const uint32_t C = tile_words(st, 2 * st.m, words);
-  if (dptrs.alloc(ptrs.size() * sizeof(void*)) != hipSuccess) return RSGPU_ERR_OUT_OF_MEMORY;
-  if (hipMemcpyAsync(dptrs.p, ptrs.data(), ptrs.size() * sizeof(void*), hipMemcpyHostToDevice, s) !=
-      hipSuccess)
-    return RSGPU_ERR_HIP;
+  const uint32_t C = tile_words(st, 2 * st.m, words, variant_tile(variant));
   LeoArgs a{};
+  size_t off = 0, bytes = 0;
+  int rc = stage_args(ptrs, nullptr, s, &a.ptrs, nullptr, &off, &bytes);
+  if (rc) return rc;
   a.layers = (const LeoLayer*)st.enc.layers.p;
   a.blocks = (const LeoBlock*)st.enc.blocks.p;
   a.tables = (const uint32_t*)st.enc.tables.p;
-  a.ptrs = (const uint8_t* const*)dptrs.p;
   a.groups = (const LeoGroup*)st.d_groups.p;
   a.words = words;
   a.C = C;
@@ -720,28 +865,24 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
     set_hip_error(e, "leo_encode_kernel");
     return RSGPU_ERR_HIP;
   }
-  // the pointer table is freed on return: wait for the kernel that reads it
-  if (hipStreamSynchronize(s) != hipSuccess) return RSGPU_ERR_HIP;
-  return RSGPU_OK;
+  return release_args(off, bytes, s);
 }
 
 int leopard_encode(const Codec& c, const rsgpu_cslice* data, const rsgpu_slice* parity, size_t len,
-                   hipStream_t s) {
+                   hipStream_t s, int variant) {
   if (len == 0 || len % 64) return RSGPU_INCORRECT_SHARD_SIZE;  // leopard.rs:50-56
   std::vector<const uint8_t*> ptrs;
   for (size_t i = 0; i < c.k; ++i) ptrs.push_back(data[i].ptr);
   for (size_t i = 0; i < c.m; ++i) ptrs.push_back(parity[i].ptr);
-  DeviceBuffer dptrs;
-  return launch_encode(c, ptrs, len, 0, nullptr, s, dptrs);
+  return launch_encode(c, ptrs, len, 0, nullptr, s, variant);
 }
 
 int leopard_verify(const Codec& c, const rsgpu_cslice* shards, size_t len, int32_t* d_flag,
-                   hipStream_t s) {
+                   hipStream_t s, int variant) {
   if (len == 0 || len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
   std::vector<const uint8_t*> ptrs;
   for (size_t i = 0; i < c.k + c.m; ++i) ptrs.push_back(shards[i].ptr);
-  DeviceBuffer dptrs;
-  return launch_encode(c, ptrs, len, 1, d_flag, s, dptrs);
+  return launch_encode(c, ptrs, len, 1, d_flag, s, variant);
 }
 
 // Decode plan per erasure pattern (DecodePlan::new, leopard_gf8/decode.rs:130-152;
@@ -801,7 +942,7 @@ static int decode_entry(const Codec& c, const std::vector<uint8_t>& present, hip
 
 // validate_leopard_reconstruction + reconstruct_leopard_impl (reconstruct.rs:795-1040)
 int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
-                        const uint8_t* required, size_t n_required, hipStream_t s) {
+                        const uint8_t* required, size_t n_required, hipStream_t s, int variant) {
   const size_t total = c.k + c.m;
   if (goal == 2 && n_required != c.k && n_required != total) return RSGPU_INVALID_SHARD_FLAGS;
   if (n < total) return RSGPU_TOO_FEW_SHARDS;
@@ -829,29 +970,27 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   if (!need) return RSGPU_OK;
   if (total - present > c.m) return RSGPU_TOO_FEW_SHARDS_PRESENT;  // decode.rs:744-747
   LeoState& st = state(c);
+  // With exactly k shards present every input is a codeword, so the generator-
+  // matrix solve over those k shards rebuilds the same bytes as the FFT decoder.
+  if (variant >= 0 && variant < kNumApplyVariants && !st.gf16 && present == c.k)
+    return leopard_matrix_reconstruct(c, shards, n, goal, required, n_required, s, variant);
   std::vector<uint8_t> pres(total);
   for (size_t i = 0; i < total; ++i) pres[i] = shards[i].present ? 1 : 0;
   std::shared_ptr<LeoDecodeEntry> e;
   int rc = decode_entry(c, pres, s, e);
   if (rc) return rc;
   const uint64_t words = len / (st.gf16 ? 8 : 4);
-  const uint32_t C = tile_words(st, st.n, words);
+  const uint32_t C = tile_words(st, st.n, words, variant_tile(variant));
   std::vector<const uint8_t*> ptrs(total);
   for (size_t i = 0; i < total; ++i) ptrs[i] = shards[i].ptr;
-  DeviceBuffer dptrs, dflags;
-  if (dptrs.alloc(total * sizeof(void*)) != hipSuccess || dflags.alloc(total) != hipSuccess)
-    return RSGPU_ERR_OUT_OF_MEMORY;
-  if (hipMemcpyAsync(dptrs.p, ptrs.data(), total * sizeof(void*), hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(dflags.p, flags.data(), total, hipMemcpyHostToDevice, s) != hipSuccess)
-    return RSGPU_ERR_HIP;
   LeoArgs a{};
+  size_t off = 0, bytes = 0;
+  if ((rc = stage_args(ptrs, &flags, s, &a.ptrs, &a.present, &off, &bytes))) return rc;
   a.layers = (const LeoLayer*)e->plan.layers.p;
   a.blocks = (const LeoBlock*)e->plan.blocks.p;
   a.tables = (const uint32_t*)e->plan.tables.p;
-  a.ptrs = (const uint8_t* const*)dptrs.p;
   a.err_tabs = (const uint32_t*)e->err_tabs.p;
   a.inv_tabs = (const uint32_t*)e->inv_tabs.p;
-  a.present = (const uint8_t*)dflags.p;
   a.words = words;
   a.C = C;
   a.k = st.k;
@@ -874,7 +1013,7 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
     set_hip_error(he, "leo_decode_kernel");
     return RSGPU_ERR_HIP;
   }
-  if (hipStreamSynchronize(s) != hipSuccess) return RSGPU_ERR_HIP;  // dptrs/dflags freed on return
+  if ((rc = release_args(off, bytes, s))) return rc;
   for (size_t i = 0; i < total; ++i)
     if (flags[i] & 2) shards[i].present = 1;
   return RSGPU_OK;

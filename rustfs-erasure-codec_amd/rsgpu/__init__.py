@@ -19,6 +19,13 @@ LIB_PATH = os.path.join(_HERE, "librsgpu.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is missing: build it with rustfs-erasure-codec_amd/build.py")
 
+# torch ships its own HIP runtime; loading it first makes librsgpu.so bind to the
+# same runtime, so device pointers and streams are shared with torch tensors.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _lib = C.CDLL(LIB_PATH)
 
 ERROR_NAMES = [
@@ -326,6 +333,135 @@ class ReedSolomon:
         vals = [C.c_uint64() for _ in range(4)]
         _check(_lib.rsgpu_cache_stats(self._h, *[C.byref(v) for v in vals]))
         return dict(zip(["requests", "hits", "misses", "entries"], [v.value for v in vals]))
+
+    # ---- kernel-variant dispatch (every variant produces the same bytes)
+    def set_variant(self, variant):
+        """Force one kernel variant for every call (-1: back to table/default)."""
+        _check(_lib.rsgpu_codec_set_variant(self._h, int(variant)))
+
+    def set_variant_table(self, table):
+        """Install an int16 [n_ops][n_erasures][n_buckets] table (None clears)."""
+        if table is None:
+            _check(_lib.rsgpu_codec_set_variant_table(self._h, None, 0))
+            return
+        t = np.ascontiguousarray(table, np.int16)
+        n_ops, n_buckets, _ = dispatch_dims()
+        if t.ndim != 3 or t.shape[0] != n_ops or t.shape[2] != n_buckets:
+            raise ValueError(f"table must be [{n_ops}][n_erasures][{n_buckets}]")
+        self._vtab = t  # the library copies it; kept only for inspection
+        _check(_lib.rsgpu_codec_set_variant_table(self._h, t.ctypes.data_as(C.POINTER(C.c_int16)),
+                                                  t.shape[1]))
+
+    def variant_valid(self, op, erasures, variant):
+        return bool(_lib.rsgpu_variant_valid(self._h, OPS[op] if isinstance(op, str) else op,
+                                             erasures, int(variant)))
+
+    def variant_for(self, op, erasures, shard_len):
+        v = C.c_int32()
+        _check(_lib.rsgpu_codec_variant_for(self._h, OPS[op] if isinstance(op, str) else op,
+                                            erasures, shard_len, C.byref(v)))
+        return v.value
+
+
+_lib.rsgpu_variant_count.argtypes = []
+_lib.rsgpu_dispatch_dims.argtypes = [_i32p, _i32p, _i32p]
+_lib.rsgpu_variant_info.argtypes = [C.c_int32] + [_i32p] * 5
+_lib.rsgpu_variant_valid.argtypes = [_vp, C.c_int32, _sz, C.c_int32]
+_lib.rsgpu_codec_set_variant.argtypes = [_vp, C.c_int32]
+_lib.rsgpu_codec_set_variant_table.argtypes = [_vp, C.POINTER(C.c_int16), _sz]
+_lib.rsgpu_codec_variant_for.argtypes = [_vp, C.c_int32, _sz, _sz, _i32p]
+
+OPS = {"encode": 0, "verify": 1, "reconstruct": 2}
+
+
+def variant_count():
+    return _lib.rsgpu_variant_count()
+
+
+def dispatch_dims():
+    """(n_ops, n_buckets, bucket0): bucket = clamp(round(2*log2(len)) - bucket0)."""
+    v = [C.c_int32() for _ in range(3)]
+    _check(_lib.rsgpu_dispatch_dims(*[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def variant_info(variant):
+    v = [C.c_int32() for _ in range(5)]
+    _check(_lib.rsgpu_variant_info(int(variant), *[C.byref(x) for x in v]))
+    return dict(zip(["kind", "lanes", "vectors_per_lane", "rows_per_group", "leopard_tile_words"],
+                    [x.value for x in v]))
+
+
+_f32p = C.POINTER(C.c_float)
+_lib.rsgpu_mlp_new.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_vp)]
+_lib.rsgpu_mlp_free.argtypes = [_vp]
+_lib.rsgpu_mlp_dims.argtypes = [_vp, _i32p, _i32p, _i32p, C.POINTER(_sz)]
+_lib.rsgpu_mlp_get_params.argtypes = [_vp, _f32p, _sz]
+_lib.rsgpu_mlp_set_params.argtypes = [_vp, _f32p, _sz]
+_lib.rsgpu_mlp_forward.argtypes = [_vp, _vp, _sz, _vp, _vp]
+_lib.rsgpu_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _sz, _f32p, _f32p, _sz, _vp]
+_lib.rsgpu_mlp_train.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int32, C.c_float, _f32p, _vp]
+
+
+class Mlp:
+    """The dispatcher network (fp32, n_in -> 64 -> 64 -> n_out) on the device.
+    Array arguments are float32 device tensors [batch][n_in] / [batch][n_out]."""
+
+    def __init__(self, n_in, n_out, seed=0):
+        self._h = _vp()
+        _check(_lib.rsgpu_mlp_new(n_in, n_out, seed, C.byref(self._h)))
+        a, hdim, b, n = C.c_int32(), C.c_int32(), C.c_int32(), _sz()
+        _check(_lib.rsgpu_mlp_dims(self._h, C.byref(a), C.byref(hdim), C.byref(b), C.byref(n)))
+        self.n_in, self.n_hidden, self.n_out, self.n_params = a.value, hdim.value, b.value, n.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rsgpu_mlp_free(self._h)
+            self._h = None
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(_lib.rsgpu_mlp_get_params(self._h, p.ctypes.data_as(_f32p), p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters")
+        _check(_lib.rsgpu_mlp_set_params(self._h, p.ctypes.data_as(_f32p), p.size))
+
+    @staticmethod
+    def _dev(t, cols):
+        if t.dtype != _torch().float32 or not t.is_cuda or not t.is_contiguous() or t.shape[-1] != cols:
+            raise ValueError("expected a contiguous float32 cuda tensor with %d columns" % cols)
+        return t.data_ptr()
+
+    def forward(self, x, stream=None):
+        torch = _torch()
+        y = torch.empty((x.shape[0], self.n_out), dtype=torch.float32, device=x.device)
+        _check(_lib.rsgpu_mlp_forward(self._h, self._dev(x, self.n_in), x.shape[0], y.data_ptr(),
+                                      _stream(stream)))
+        return y
+
+    def loss_grad(self, x, t, w, stream=None):
+        loss = C.c_float()
+        g = np.zeros(self.n_params, np.float32)
+        _check(_lib.rsgpu_mlp_loss_grad(self._h, self._dev(x, self.n_in), self._dev(t, self.n_out),
+                                        self._dev(w, self.n_out), x.shape[0], C.byref(loss),
+                                        g.ctypes.data_as(_f32p), g.size, _stream(stream)))
+        return loss.value, g
+
+    def train(self, x, t, w, steps, lr, stream=None):
+        loss = C.c_float()
+        _check(_lib.rsgpu_mlp_train(self._h, self._dev(x, self.n_in), self._dev(t, self.n_out),
+                                    self._dev(w, self.n_out), x.shape[0], int(steps), float(lr),
+                                    C.byref(loss), _stream(stream)))
+        return loss.value
+
+
+def _torch():
+    import torch
+    return torch
 
 
 _lib.rsgpu_batch_new.argtypes = [C.POINTER(_vp)]

@@ -1,0 +1,413 @@
+"""Learned kernel-variant dispatcher.
+
+Every call of the data path can run one of ``variant_count()`` kernel variants
+(tile shapes of the matrix-apply kernel, the generator-matrix path of Leopard
+GF(2^8) codecs, Leopard FFT column tiles).  All of them produce the same bytes,
+so choosing among them only changes time.  The reference chooses its CPU kernel
+by a fixed cascade (``SelectedEncodeKernel::select``,
+src/galois_8/codegen.rs:10-44; chunking by ``ParallelPolicy``,
+src/core/parallel.rs:65-127).  Here the choice is learned:
+
+* ``collect`` times every valid variant (and the default tiling) on seeded
+  synthetic calls -- field/family, k, m, shard size, operation, erasure count --
+  with HIP events around each call, so host gaps do not count;
+* ``Dispatcher.fit`` trains the fp32 network (``rsgpu.Mlp``: features -> 64 ->
+  64 -> one predicted log-time per variant) on the device with the library's
+  own forward/backward/Adam kernels;
+* ``Dispatcher.install`` evaluates the network for every (op, erasures, size
+  bucket) of a codec and installs the argmin as the codec's variant table;
+  shapes outside the trained envelope fall back to ``autotune`` (exhaustive
+  timing), which is also the judge of the network's regret.
+
+    python -m rsgpu.dispatch collect --n 3000 --out timings.npz
+    python -m rsgpu.dispatch train --data timings.npz --out dispatch_mi355x.npz
+"""
+import argparse
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import (CSlice, Mlp, OPS, ReedSolomon, Slice, Slot, dispatch_dims, lib, variant_count,
+               _check, _stream)
+
+KINDS = ["gf8", "gf16", "leopard_gf8", "leopard_gf16"]
+N_FEATURES = 14
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_MODEL = os.path.join(DATA_DIR, "dispatch_mi355x.npz")
+DEFAULT_TIMINGS = os.path.join(DATA_DIR, "timings_mi355x.npz")
+
+
+def features(kind, k, m, shard_len, op, erasures):
+    """Network inputs of one call (all O(1) magnitudes)."""
+    kid = KINDS.index(kind) if isinstance(kind, str) else int(kind)
+    op = OPS[op] if isinstance(op, str) else int(op)
+    l2 = (math.log2(max(shard_len, 1)) - 16.0) / 6.0
+    return np.array([
+        kid == 1, kid == 2, kid == 3,
+        math.log2(k) / 12.0, math.log2(m) / 10.0 if m > 0 else 0.0, math.log2(k + m) / 12.0,
+        l2, l2 * l2,
+        op == 0, op == 1, op == 2,
+        erasures / max(m, 1), math.log2(1 + erasures) / 10.0,
+        (math.log2(max(shard_len, 1)) + math.log2(k + m) - 24.0) / 6.0,
+    ], np.float32)
+
+
+def make_codec(kind, k, m):
+    if kind == "gf8":
+        return ReedSolomon(k, m)
+    if kind == "gf16":
+        return ReedSolomon(k, m, field="gf16")
+    return ReedSolomon(k, m, codec_family=kind)
+
+
+class OpCase:
+    """One synthetic call (shards of uniform random bytes in one device buffer)
+    that can be re-run under any forced variant."""
+
+    def __init__(self, torch, rs, kind, k, m, shard_len, op, erasures, seed=0):
+        self.torch, self.rs = torch, rs
+        self.kind, self.k, self.m, self.len = kind, k, m, shard_len
+        self.op = OPS[op] if isinstance(op, str) else op
+        n = k + m
+        pitch = (shard_len + 255) // 256 * 256 + 256  # avoid power-of-two shard strides
+        self.buf = torch.empty(n * pitch, dtype=torch.uint8, device="cuda")
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        self.buf.random_(0, 256, generator=g)
+        base = self.buf.data_ptr()
+        ptrs = [base + i * pitch for i in range(n)]
+        rng = np.random.default_rng(seed)
+        self.erased = sorted(rng.choice(n, erasures, replace=False).tolist()) if erasures else []
+        self.flag = torch.zeros(1, dtype=torch.int32, device="cuda")
+        if self.op == 0:
+            self.args = (Slice * n)(*[Slice(p, shard_len) for p in ptrs])
+        elif self.op == 1:
+            self.args = (CSlice * n)(*[CSlice(p, shard_len) for p in ptrs])
+        else:
+            self.args = (Slot * n)(*[Slot(p, shard_len, 1) for p in ptrs])
+        self.n = n
+        self.stream = _stream(None)
+
+    def run(self):
+        L = lib()
+        h = self.rs._h
+        if self.op == 0:
+            rc = L.rsgpu_encode(h, self.args, self.n, self.stream)
+        elif self.op == 1:
+            rc = L.rsgpu_verify_async(h, self.args, self.n, self.flag.data_ptr(), self.stream)
+        else:
+            for i in self.erased:
+                self.args[i].present = 0
+            rc = L.rsgpu_reconstruct(h, self.args, self.n, self.stream)
+        _check(rc)
+
+    def valid_variants(self):
+        e = len(self.erased)
+        return [v for v in range(variant_count()) if self.rs.variant_valid(self.op, e, v)]
+
+    def time(self, variants, reps=None, rounds=2):
+        """Device microseconds per call for each variant (-1 = default tiling):
+        the mean over `reps` event-bracketed calls, minimum over interleaved rounds."""
+        torch = self.torch
+        if reps is None:
+            self.rs.set_variant(-1)
+            self.run()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            self.run()
+            b.record()
+            b.synchronize()
+            est = max(a.elapsed_time(b) * 1000.0, 1.0)
+            reps = int(min(40, max(3, 300.0 / est)))
+        best = {v: float("inf") for v in variants}
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for _ in range(rounds):
+            for v in variants:
+                self.rs.set_variant(v)
+                self.run()  # warm-up (decode-row cache, matrix-path codec)
+                for a, b in ev:
+                    a.record()
+                    self.run()
+                    b.record()
+                ev[-1][1].synchronize()
+                t = sum(a.elapsed_time(b) for a, b in ev) * 1000.0 / reps
+                best[v] = min(best[v], t)
+        self.rs.set_variant(-1)
+        return best
+
+
+def sample_specs(n, seed=0x5EED_D15C):
+    """Seeded call shapes: (kind, k, m, shard_len, op, erasures)."""
+    rng = np.random.default_rng(seed)
+    leo8 = [(10, 4), (16, 8), (32, 8), (64, 32), (100, 28), (128, 64), (192, 64)]
+    leo16 = [(10, 4), (64, 16), (300, 45), (1024, 256), (4096, 1024)]
+
+    def loguni(lo, hi, align):
+        s = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        return max(align, int(round(s / align)) * align)
+
+    out = []
+    for _ in range(n):
+        u = rng.random()
+        if u < 0.62:
+            kind = "gf8"
+            k = int(round(math.exp(rng.uniform(math.log(2), math.log(32)))))
+            m = int(rng.integers(1, 9))
+            size = loguni(1 << 10, 16 << 20, 64)
+        elif u < 0.72:
+            kind = "gf16"
+            k = int(rng.integers(2, 21))
+            m = int(rng.integers(1, 7))
+            size = loguni(1 << 10, 4 << 20, 64)
+        elif u < 0.90:
+            kind = "leopard_gf8"
+            k, m = leo8[int(rng.integers(len(leo8)))]
+            size = loguni(4 << 10, 1 << 20, 64)
+        else:
+            kind = "leopard_gf16"
+            k, m = leo16[int(rng.integers(len(leo16)))]
+            size = loguni(4 << 10, 256 << 10, 64)
+            if k + m > 2000:
+                size = min(size, 128 << 10)
+        w = rng.random()
+        if w < 0.4:
+            op, e = "encode", 0
+        elif w < 0.65:
+            op, e = "verify", 0
+        else:
+            op = "reconstruct"
+            e = m if (kind.startswith("leopard") and rng.random() < 0.5) else int(rng.integers(1, m + 1))
+        out.append((kind, k, m, size, op, e))
+    return out
+
+
+def collect(specs, progress=True, seed=1):
+    """Times every valid variant and the default tiling for each spec."""
+    import torch
+
+    V = variant_count()
+    N = len(specs)
+    X = np.zeros((N, N_FEATURES), np.float32)
+    T = np.full((N, V), np.nan, np.float32)
+    D = np.zeros(N, np.float32)
+    meta = np.zeros((N, 6), np.int64)
+    codecs = {}
+    t0 = time.time()
+    for i, (kind, k, m, size, op, e) in enumerate(specs):
+        key = (kind, k, m)
+        if key not in codecs:
+            codecs[key] = make_codec(kind, k, m)
+        rs = codecs[key]
+        case = OpCase(torch, rs, kind, k, m, size, op, e, seed=seed + i)
+        vs = case.valid_variants()
+        t = case.time(vs + [-1])
+        for v in vs:
+            T[i, v] = t[v]
+        D[i] = t[-1]
+        X[i] = features(kind, k, m, size, op, e)
+        meta[i] = [KINDS.index(kind), k, m, size, OPS[op], e]
+        del case
+        if progress and (i + 1) % 100 == 0:
+            print(f"[collect] {i + 1}/{N} cases, {time.time() - t0:.0f} s", flush=True)
+    return {"X": X, "T": T, "D": D, "meta": meta}
+
+
+def save(path, data):
+    np.savez_compressed(path, **data)
+
+
+def load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def targets(T):
+    """Per-row centred log-times and their weights (0 for invalid variants)."""
+    valid = np.isfinite(T)
+    lt = np.where(valid, np.log(np.where(valid, T, 1.0)), 0.0)
+    mean = lt.sum(1, keepdims=True) / np.maximum(valid.sum(1, keepdims=True), 1)
+    return np.where(valid, lt - mean, 0.0).astype(np.float32), valid.astype(np.float32)
+
+
+def split(n, frac=0.2, seed=7):
+    """Deterministic train / held-out index split."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    n_test = int(round(n * frac))
+    return np.sort(perm[n_test:]), np.sort(perm[:n_test])
+
+
+def regret(T, D, picks):
+    """Per-row t(pick)/t(best) - 1 and summary statistics; D = default-tiling times."""
+    best = np.nanmin(T, axis=1)
+    rows = np.arange(T.shape[0])
+    tp = T[rows, picks]
+    r = tp / best - 1.0
+    rd = D / best - 1.0
+    return {
+        "mean": float(np.mean(r)), "median": float(np.median(r)), "p90": float(np.quantile(r, 0.9)),
+        "total": float(tp.sum() / best.sum() - 1.0),
+        "default_mean": float(np.mean(rd)), "default_total": float(D.sum() / best.sum() - 1.0),
+        "gain_vs_default_total": float(D.sum() / tp.sum()),
+        "gain_vs_default_geomean": float(np.exp(np.mean(np.log(D / tp)))),
+        "n": int(T.shape[0]),
+    }
+
+
+class Dispatcher:
+    """The trained network plus the envelope of shapes it was trained on."""
+
+    def __init__(self, seed=0):
+        self.mlp = Mlp(N_FEATURES, variant_count(), seed=seed)
+        self.envelope = {}  # kind -> (k_min, k_max, m_min, m_max, len_min, len_max)
+
+    # -- training
+    def fit(self, X, T, meta=None, steps=3000, lr=3e-3, stream=None):
+        import torch
+
+        t, w = targets(T)
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
+        x_d, t_d, w_d = dev(X), dev(t), dev(w)
+        loss = self.mlp.train(x_d, t_d, w_d, steps, lr, stream)
+        torch.cuda.synchronize()
+        if meta is not None:
+            self.set_envelope(meta)
+        return loss
+
+    def set_envelope(self, meta):
+        self.envelope = {}
+        for kid, kind in enumerate(KINDS):
+            sel = meta[:, 0] == kid
+            if sel.any():
+                mm = meta[sel]
+                self.envelope[kind] = (int(mm[:, 1].min()), int(mm[:, 1].max()), int(mm[:, 2].min()),
+                                       int(mm[:, 2].max()), int(mm[:, 3].min()), int(mm[:, 3].max()))
+
+    def in_envelope(self, kind, k, m, shard_len):
+        env = self.envelope.get(kind)
+        if env is None:
+            return False
+        k0, k1, m0, m1, l0, l1 = env
+        return k0 <= k <= k1 and m0 <= m <= m1 and l0 // 2 <= shard_len <= l1 * 2
+
+    def predict(self, X):
+        import torch
+
+        x = torch.from_numpy(np.ascontiguousarray(X, np.float32)).cuda()
+        y = self.mlp.forward(x)
+        return y.cpu().numpy()
+
+    def pick(self, X, valid):
+        y = self.predict(X)
+        y = np.where(valid, y, np.inf)
+        return np.argmin(y, axis=1)
+
+    # -- persistence
+    def save(self, path):
+        env = np.array([[KINDS.index(k), *v] for k, v in self.envelope.items()], np.int64)
+        np.savez_compressed(path, params=self.mlp.get_params(), envelope=env)
+
+    @classmethod
+    def load(cls, path=DEFAULT_MODEL):
+        d = cls()
+        with np.load(path, allow_pickle=False) as z:
+            d.mlp.set_params(z["params"])
+            d.envelope = {KINDS[int(r[0])]: tuple(int(x) for x in r[1:]) for r in z["envelope"]}
+        return d
+
+    # -- installing decisions on a codec
+    def table_for(self, rs, kind):
+        n_ops, n_buckets, b0 = dispatch_dims()
+        k, m = rs.data_shard_count(), rs.parity_shard_count()
+        V = variant_count()
+        rows, where = [], []
+        for op in range(n_ops):
+            for e in range(m + 1):
+                if (op < 2) != (e == 0):
+                    continue
+                for b in range(n_buckets):
+                    ln = int(round(2.0 ** ((b + b0) / 2.0)))
+                    rows.append(features(kind, k, m, ln, op, e))
+                    where.append((op, e, b, ln))
+        table = np.full((n_ops, m + 1, n_buckets), -1, np.int16)
+        X = np.stack(rows)
+        valid = np.zeros((len(rows), V), bool)
+        cache = {}
+        for i, (op, e, b, ln) in enumerate(where):
+            key = (op, e)
+            if key not in cache:
+                cache[key] = np.array([rs.variant_valid(op, e, v) for v in range(V)])
+            valid[i] = cache[key]
+        picks = self.pick(X, valid)
+        for i, ((op, e, b, ln), p) in enumerate(zip(where, picks)):
+            if valid[i].any() and self.in_envelope(kind, k, m, ln):
+                table[op, e, b] = p
+        # encode/verify rows only use e = 0; copy them so any erasure index resolves
+        table[0, 1:, :] = table[0, 0, :]
+        table[1, 1:, :] = table[1, 0, :]
+        return table
+
+    def install(self, rs, kind, autotune_calls=()):
+        """Installs the network's table on `rs`.  Calls in `autotune_calls`
+        ((op, erasures, shard_len) triples) whose shape lies outside the trained
+        envelope are decided by exhaustive timing instead."""
+        table = self.table_for(rs, kind)
+        k, m = rs.data_shard_count(), rs.parity_shard_count()
+        if autotune_calls and not self.in_envelope(kind, k, m, 0):
+            import torch
+
+            _, n_buckets, b0 = dispatch_dims()
+            for op, e, ln in autotune_calls:
+                op = OPS[op] if isinstance(op, str) else op
+                b = min(max(int(round(2 * math.log2(ln))) - b0, 0), n_buckets - 1)
+                best, _ = autotune(torch, rs, kind, ln, op, e)
+                table[op, e, b] = best
+        rs.set_variant_table(table)
+        return table
+
+
+def autotune(torch, rs, kind, shard_len, op, erasures, seed=0):
+    """Exhaustive search: times every valid variant of this call and returns
+    (best variant, {variant: us}) -- the fallback outside the trained envelope and
+    the judge of the network's regret."""
+    k, m = rs.data_shard_count(), rs.parity_shard_count()
+    case = OpCase(torch, rs, kind, k, m, shard_len, op, erasures, seed=seed)
+    t = case.time(case.valid_variants())
+    return min(t, key=t.get), t
+
+
+def _main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m rsgpu.dispatch")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("--n", type=int, default=3000)
+    c.add_argument("--seed", type=int, default=0x5EEDD15C)
+    c.add_argument("--out", default=DEFAULT_TIMINGS)
+    t = sub.add_parser("train")
+    t.add_argument("--data", default=DEFAULT_TIMINGS)
+    t.add_argument("--out", default=DEFAULT_MODEL)
+    t.add_argument("--steps", type=int, default=3000)
+    t.add_argument("--lr", type=float, default=3e-3)
+    a = ap.parse_args(argv)
+    if a.cmd == "collect":
+        specs = sample_specs(a.n, a.seed)
+        data = collect(specs)
+        save(a.out, data)
+        print(f"[collect] wrote {a.out}: {int(np.isfinite(data['T']).sum())} timing samples", flush=True)
+    else:
+        data = load(a.data)
+        tr, te = split(len(data["X"]))
+        d = Dispatcher()
+        loss = d.fit(data["X"][tr], data["T"][tr], data["meta"][tr], steps=a.steps, lr=a.lr)
+        valid = np.isfinite(data["T"][te])
+        picks = d.pick(data["X"][te], valid)
+        r = regret(data["T"][te], data["D"][te], picks)
+        print(f"[train] loss {loss:.4f}; held-out regret {r}", flush=True)
+        d.save(a.out)
+
+
+if __name__ == "__main__":
+    sys.exit(_main())

@@ -1,0 +1,98 @@
+"""The learned dispatcher: its HIP network against the CPU oracle (forward
+outputs and gradients, fp32 vs float64 within the stated tolerance), training
+on the device, and the trained dispatcher judged by exhaustive timing (regret)
+and against the default tiling (gain)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from gpu_util import announce, rand_shards, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+rsgpu = pytest.importorskip("rsgpu")
+from rsgpu import dispatch as D  # noqa: E402
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import mlp as M  # noqa: E402
+
+# fp32 kernels vs the float64 oracle: sums of at most a few thousand fp32 products
+RTOL = 2e-4
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _rand_problem(seed, n_in, n_out, B):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.25, M.n_params(n_in, n_out)).astype(np.float32)
+    x = rng.normal(0, 1, (B, n_in)).astype(np.float32)
+    t = rng.normal(0, 1, (B, n_out)).astype(np.float32)
+    w = (rng.random((B, n_out)) < 0.8).astype(np.float32)
+    w[0, 0] = 1.0  # at least one target
+    return p, x, t, w
+
+
+@pytest.mark.parametrize("n_in,n_out,B", [(14, 20, 700), (3, 1, 1), (16, 32, 257), (5, 7, 2600)])
+def test_forward_and_gradient_match_oracle(n_in, n_out, B):
+    announce(f"mlp grad check n_in={n_in} n_out={n_out} B={B}")
+    p, x, t, w = _rand_problem(n_in * 100 + B, n_in, n_out, B)
+    net = rsgpu.Mlp(n_in, n_out, seed=1)
+    net.set_params(p)
+    assert np.array_equal(net.get_params(), p)
+    y = net.forward(_dev(x)).cpu().numpy()
+    y_ref, _ = M.forward(p.astype(np.float64), x, n_out)
+    assert np.allclose(y, y_ref, rtol=RTOL, atol=RTOL * np.abs(y_ref).max())
+    loss, g = net.loss_grad(_dev(x), _dev(t), _dev(w))
+    L_ref, g_ref = M.loss_grad(p.astype(np.float64), x, t, w)
+    assert abs(loss - L_ref) <= RTOL * abs(L_ref)
+    scale = np.abs(g_ref).max()
+    assert np.allclose(g, g_ref, rtol=1e-3, atol=RTOL * scale), np.abs(g - g_ref).max() / scale
+    # the step is deterministic: the same call gives the same bits
+    loss2, g2 = net.loss_grad(_dev(x), _dev(t), _dev(w))
+    assert loss2 == loss and np.array_equal(g, g2)
+
+
+def test_adam_step_matches_oracle():
+    """One device Adam step equals the textbook update applied to the oracle gradient."""
+    announce("mlp adam step")
+    n_in, n_out, B = 6, 4, 300
+    p, x, t, w = _rand_problem(5, n_in, n_out, B)
+    net = rsgpu.Mlp(n_in, n_out)
+    net.set_params(p)
+    lr = 1e-2
+    net.train(_dev(x), _dev(t), _dev(w), 1, lr)
+    _, g = M.loss_grad(p.astype(np.float64), x, t, w)
+    m = 0.1 * g
+    v = 0.001 * g * g
+    expect = p - lr * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    got = net.get_params()
+    # The first Adam step moves each parameter by lr * g / (|g| + eps): its size
+    # is fixed, its sign is the gradient's.  Where |g| is at fp32 noise level the
+    # sign is not determined, so those entries are only checked for the step size.
+    clear = np.abs(g) > 1e-3 * np.abs(g).max()
+    assert clear.sum() > 0.5 * g.size
+    assert np.allclose(got[clear], expect[clear], rtol=1e-4, atol=1e-5)
+    assert np.all(np.abs(got - p) <= lr * (1 + 1e-4))
+
+
+def test_training_on_device_fits_a_smooth_function():
+    announce("mlp training")
+    rng = np.random.default_rng(11)
+    B, n_in, n_out = 2048, 4, 3
+    x = rng.uniform(-1, 1, (B, n_in)).astype(np.float32)
+    t = np.stack([np.sin(2 * x[:, 0]) + x[:, 1] ** 2, x[:, 2] * x[:, 3], np.abs(x[:, 0] - x[:, 3])],
+                 1).astype(np.float32)
+    w = np.ones_like(t)
+    net = rsgpu.Mlp(n_in, n_out, seed=3)
+    l0, _ = net.loss_grad(_dev(x), _dev(t), _dev(w))
+    net.train(_dev(x), _dev(t), _dev(w), 1500, 3e-3)
+    l1, _ = net.loss_grad(_dev(x), _dev(t), _dev(w))
+    assert l1 < 0.05 * l0, (l0, l1)
+    # the final loss agrees with the oracle at the trained parameters
+    L_ref = M.loss(net.get_params().astype(np.float64), x, t, w)
+    assert abs(l1 - L_ref) <= 1e-3 * L_ref + 1e-6
