@@ -96,3 +96,89 @@ def test_training_on_device_fits_a_smooth_function():
     # the final loss agrees with the oracle at the trained parameters
     L_ref = M.loss(net.get_params().astype(np.float64), x, t, w)
     assert abs(l1 - L_ref) <= 1e-3 * L_ref + 1e-6
+
+
+def test_dispatcher_trained_on_device_regret_and_gain():
+    """Train on the committed timing set (80%) with the device kernels; on the
+    held-out 20% the network's picks must be within 10% of the exhaustively
+    measured best on average and beat the default tiling."""
+    announce("dispatcher held-out regret")
+    data = D.load(D.DEFAULT_TIMINGS)
+    assert int(np.isfinite(data["T"]).sum()) > 40000
+    tr, te = D.split(len(data["X"]))
+    disp = D.Dispatcher(seed=0)
+    loss = disp.fit(data["X"][tr], data["T"][tr], data["meta"][tr], steps=3000, lr=3e-3)
+    assert loss < 0.05
+    valid = np.isfinite(data["T"][te])
+    picks = disp.pick(data["X"][te], valid)
+    assert valid[np.arange(len(picks)), picks].all()  # only exact variants are picked
+    r = D.regret(data["T"][te], data["D"][te], picks)
+    print(f"held-out regret {r}", flush=True)
+    assert r["mean"] <= 0.10, r
+    assert r["total"] <= 0.05, r
+    assert r["gain_vs_default_total"] >= 1.2, r
+    assert r["mean"] < r["default_mean"] / 4, r
+
+
+def test_dispatcher_live_regret_against_exhaustive_autotune():
+    """The shipped network, installed as variant tables on fresh codecs, judged
+    live on call shapes it never saw: exhaustive timing of every valid variant is
+    the judge; the default tiling is the method without the network."""
+    announce("dispatcher live regret")
+    disp = D.Dispatcher.load()
+    specs = D.sample_specs(48, seed=20261203)
+    rows = []
+    for i, (kind, k, m, size, op, e) in enumerate(specs):
+        rs = D.make_codec(kind, k, m)
+        disp.install(rs, kind)
+        v = rs.variant_for(op, e, size)
+        # time with the table removed: variant -1 is then the default tiling
+        rs.set_variant_table(None)
+        case = D.OpCase(torch, rs, kind, k, m, size, op, e, seed=900 + i)
+        vs = case.valid_variants()
+        assert v == -1 or v in vs
+        t = case.time(vs + [-1])
+        best = min(t[x] for x in vs)
+        rows.append((kind, t[v] if v >= 0 else t[-1], t[-1], best))
+    tp = np.array([r[1] for r in rows])
+    td = np.array([r[2] for r in rows])
+    tb = np.array([r[3] for r in rows])
+    regret = tp / tb - 1.0
+    geo_gain = float(np.exp(np.mean(np.log(td / tp))))
+    for kind in D.KINDS:
+        sel = np.array([r[0] == kind for r in rows])
+        if sel.any():
+            print(f"  {kind}: n={sel.sum()} regret mean {regret[sel].mean():.3f} "
+                  f"gain vs default geomean {np.exp(np.mean(np.log(td[sel] / tp[sel]))):.3f}", flush=True)
+    print(f"live regret mean {regret.mean():.3f} total {tp.sum() / tb.sum() - 1:.3f}; gain vs default "
+          f"geomean {geo_gain:.3f} total {td.sum() / tp.sum():.3f}", flush=True)
+    assert regret.mean() <= 0.10
+    assert geo_gain >= 1.1           # per call, the network beats the default tiling
+    assert td.sum() / tp.sum() >= 0.98  # and never costs total time
+
+
+def test_installed_tables_keep_outputs_exact():
+    announce("dispatcher install exact")
+    disp = D.Dispatcher.load()
+    import oracle_lib as O
+    for kind, k, m, n in [("gf8", 10, 4, 1 << 16), ("leopard_gf8", 12, 4, 8192)]:
+        rs = D.make_codec(kind, k, m)
+        disp.install(rs, kind)
+        rng = np.random.default_rng(k + n)
+        data = rand_shards(rng, k, n)
+        if kind == "gf8":
+            _, mat = O.build_matrix(0, 0, k, m)
+            par = O.encode(0, mat, k, m, data)
+        else:
+            _, par = O.leo_encode(8, k, m, data)
+        sh = [to_dev(d) for d in data] + [torch.zeros(n, dtype=torch.uint8, device="cuda")
+                                         for _ in range(m)]
+        rs.encode(sh)
+        full = [to_host(x) for x in sh]
+        for j in range(m):
+            assert (full[k + j] == par[j]).all(), (kind, j)
+        for erased in ([0, 3, k + 1, k + 2], [1]):
+            work = [None if i in erased else to_dev(full[i]) for i in range(k + m)]
+            rs.reconstruct(work)
+            for i in erased:
+                assert (to_host(work[i]) == full[i]).all(), (kind, erased, i)
