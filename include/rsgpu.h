@@ -221,6 +221,60 @@ int rsgpu_codec_set_variant_table(rsgpu_codec* codec, const int16_t* table, size
 int rsgpu_codec_variant_for(const rsgpu_codec* codec, int32_t op, size_t erasures, size_t shard_len,
                             int32_t* variant);
 
+/* ---- request server: HBM admission and continuous batching.
+ * No counterpart in the reference (each call there runs to completion on the
+ * caller's thread; stream.rs:677-960 pipelines one object).  Requests name a
+ * classic codec, an op and k+m device shard pointers that are read in place; the
+ * server allocates the outputs (encode: m parity shards; reconstruct: one per
+ * erased shard, in shard order) from its arena.  A request is admitted, in
+ * arrival order, once its footprint (k+m)*shard_len fits under hbm_budget and its
+ * outputs fit in the arena; admitted requests are packed into the next idle
+ * batch slot (max_inflight_batches streams, at most max_batch_ops per batch),
+ * so batches overlap on the device.  Outputs stay valid until released. ---- */
+typedef struct rsgpu_server rsgpu_server;
+typedef struct {
+  uint64_t hbm_budget;          /* admission limit on the summed footprint of admitted requests */
+  uint64_t arena_bytes;         /* device memory for outputs */
+  int32_t max_inflight_batches; /* batch slots (streams), 1..16 */
+  int32_t max_batch_ops;        /* requests per batch */
+} rsgpu_server_config;
+typedef struct {
+  int32_t op;                   /* 0 encode, 1 verify, 2 reconstruct */
+  int32_t reserved;
+  const rsgpu_codec* codec;
+  size_t shard_len;
+  const uint8_t* const* shards; /* k+m device pointers (erased ones are not read) */
+  const uint8_t* erased;        /* reconstruct: k+m flags, nonzero = missing */
+  uint64_t arrival_ns;          /* replay: arrival offset from the start of the replay */
+  uint64_t tag;                 /* caller's unique id */
+} rsgpu_request;
+typedef struct {
+  uint64_t tag;
+  int32_t status;               /* 0, or the validation error of the op */
+  int32_t verify_ok;            /* verify: 1 if every parity shard matched */
+  uint64_t arrival_ns, admit_ns, launch_ns, done_ns;
+  uint64_t footprint;
+} rsgpu_completion;
+typedef struct {
+  uint64_t completed, batches, launched_ops, max_inflight_batches, peak_footprint, footprint;
+} rsgpu_server_counters;
+int rsgpu_server_new(const rsgpu_server_config* config, rsgpu_server** out);
+/* Waits for in-flight batches, then frees the server and its arena. */
+void rsgpu_server_free(rsgpu_server* server);
+/* Queues a request arriving now. */
+int rsgpu_server_submit(rsgpu_server* server, const rsgpu_request* request);
+/* One loop turn: retire finished batches, admit, launch. */
+int rsgpu_server_pump(rsgpu_server* server, uint32_t* batches_in_flight);
+int rsgpu_server_poll(rsgpu_server* server, rsgpu_completion* out, size_t max, size_t* n);
+int rsgpu_server_outputs(const rsgpu_server* server, uint64_t tag, uint8_t** outputs, size_t n);
+int rsgpu_server_release(rsgpu_server* server, uint64_t tag);
+/* Open-loop replay: submits each request at its arrival offset (requests sorted
+ * by arrival), runs the loop until all complete; out[i] is request i's record
+ * (times relative to the replay start).  keep_outputs = 0 releases on completion. */
+int rsgpu_server_replay(rsgpu_server* server, const rsgpu_request* requests, size_t n,
+                        int32_t keep_outputs, rsgpu_completion* out);
+int rsgpu_server_stats(const rsgpu_server* server, rsgpu_server_counters* stats);
+
 /* ---- the dispatcher's network: fp32 x[n_in <= 16] -> 64 ReLU -> 64 ReLU ->
  * y[n_out <= 32], trained on the device.  It predicts per-variant log-times from
  * call features so a dispatcher can fill variant tables without timing every

@@ -316,8 +316,11 @@ static int upload(rsgpu_batch* b, hipStream_t s) {
   auto& p = b->plan;
   size_t sb = std::max<size_t>(p.stripes.size(), 1) * sizeof(BatchStripe);
   size_t pb = std::max<size_t>(p.ptrs.size(), 1) * sizeof(void*);
-  if (b->d_stripes.bytes < sb && b->d_stripes.alloc(sb) != hipSuccess) return RSGPU_ERR_OUT_OF_MEMORY;
-  if (b->d_ptrs.bytes < pb && b->d_ptrs.alloc(pb) != hipSuccess) return RSGPU_ERR_OUT_OF_MEMORY;
+  // grow geometrically: a reallocation frees the old buffer, which waits for the device
+  if (b->d_stripes.bytes < sb && b->d_stripes.alloc(std::max(sb, 2 * b->d_stripes.bytes)) != hipSuccess)
+    return RSGPU_ERR_OUT_OF_MEMORY;
+  if (b->d_ptrs.bytes < pb && b->d_ptrs.alloc(std::max(pb, 2 * b->d_ptrs.bytes)) != hipSuccess)
+    return RSGPU_ERR_OUT_OF_MEMORY;
   if (!p.stripes.empty() &&
       hipMemcpyAsync(b->d_stripes.p, p.stripes.data(), p.stripes.size() * sizeof(BatchStripe),
                      hipMemcpyHostToDevice, s) != hipSuccess)

@@ -1,0 +1,183 @@
+"""Request server (continuous batching under an HBM budget) and the serving
+workload: an open-loop Poisson stream of encode / verify / reconstruct requests
+on objects of log-uniform size, each object one stripe of a 10+4 / 12+4 / 16+4
+classic codec with shard length ceil(size / k) rounded up to 64 bytes.
+
+Request inputs are read in place from a device-resident store of encoded
+stripes (many distinct stripes per shape, so repeated requests do not hit in
+cache); the server allocates each request's outputs from its arena.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import OPS, ReedSolomon, RsError, _check, _vp, _sz, lib
+
+_u8p = C.POINTER(C.c_uint8)
+
+
+class Request(C.Structure):
+    _fields_ = [("op", C.c_int32), ("reserved", C.c_int32), ("codec", _vp), ("shard_len", _sz),
+                ("shards", C.POINTER(_vp)), ("erased", _u8p), ("arrival_ns", C.c_uint64),
+                ("tag", C.c_uint64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("verify_ok", C.c_int32),
+                ("arrival_ns", C.c_uint64), ("admit_ns", C.c_uint64), ("launch_ns", C.c_uint64),
+                ("done_ns", C.c_uint64), ("footprint", C.c_uint64)]
+
+
+class Config(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("arena_bytes", C.c_uint64),
+                ("max_inflight_batches", C.c_int32), ("max_batch_ops", C.c_int32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("completed", "batches", "launched_ops",
+                                          "max_inflight_batches", "peak_footprint", "footprint")]
+
+
+_L = lib()
+_L.rsgpu_server_new.argtypes = [C.POINTER(Config), C.POINTER(_vp)]
+_L.rsgpu_server_free.argtypes = [_vp]
+_L.rsgpu_server_submit.argtypes = [_vp, C.POINTER(Request)]
+_L.rsgpu_server_pump.argtypes = [_vp, C.POINTER(C.c_uint32)]
+_L.rsgpu_server_poll.argtypes = [_vp, C.POINTER(Completion), _sz, C.POINTER(_sz)]
+_L.rsgpu_server_outputs.argtypes = [_vp, C.c_uint64, C.POINTER(_vp), _sz]
+_L.rsgpu_server_release.argtypes = [_vp, C.c_uint64]
+_L.rsgpu_server_replay.argtypes = [_vp, C.POINTER(Request), _sz, C.c_int32, C.POINTER(Completion)]
+_L.rsgpu_server_stats.argtypes = [_vp, C.POINTER(Counters)]
+
+
+class Server:
+    def __init__(self, hbm_budget, arena_bytes, max_inflight_batches=4, max_batch_ops=4096):
+        self._h = _vp()
+        cfg = Config(hbm_budget, arena_bytes, max_inflight_batches, max_batch_ops)
+        _check(_L.rsgpu_server_new(C.byref(cfg), C.byref(self._h)))
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _L.rsgpu_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, req):
+        _check(_L.rsgpu_server_submit(self._h, C.byref(req)))
+
+    def pump(self):
+        n = C.c_uint32()
+        _check(_L.rsgpu_server_pump(self._h, C.byref(n)))
+        return n.value
+
+    def poll(self, max_n=1024):
+        buf = (Completion * max_n)()
+        n = _sz()
+        _check(_L.rsgpu_server_poll(self._h, buf, max_n, C.byref(n)))
+        return list(buf[:n.value])
+
+    def outputs(self, tag, n):
+        arr = (_vp * n)()
+        _check(_L.rsgpu_server_outputs(self._h, tag, arr, n))
+        return [arr[i] for i in range(n)]
+
+    def release(self, tag):
+        _check(_L.rsgpu_server_release(self._h, tag))
+
+    def replay(self, reqs, keep_outputs=False):
+        out = (Completion * len(reqs))()
+        _check(_L.rsgpu_server_replay(self._h, reqs, len(reqs), int(keep_outputs), out))
+        return out
+
+    def stats(self):
+        c = Counters()
+        _check(_L.rsgpu_server_stats(self._h, C.byref(c)))
+        return {n: getattr(c, n) for n, _ in Counters._fields_}
+
+
+SHAPES = [(10, 4), (12, 4), (16, 4)]
+
+
+class Store:
+    """Encoded stripes resident on the device: `per_shape` stripes of shard length
+    `max_shard` for each shape, laid out with a 256-byte shard pitch pad."""
+
+    def __init__(self, torch, shapes=SHAPES, per_shape=8, max_shard=(64 << 20) // 10, seed=5):
+        self.torch = torch
+        self.max_shard = (max_shard + 63) // 64 * 64
+        self.pitch = self.max_shard + 256
+        self.codecs = {s: ReedSolomon(*s) for s in shapes}
+        self.per_shape = per_shape
+        total = sum((k + m) * per_shape for k, m in shapes) * self.pitch
+        self.buf = torch.empty(total, dtype=torch.uint8, device="cuda")
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        self.buf.random_(0, 256, generator=g)
+        self.base = {}
+        off = 0
+        from . import Batch, Slice
+        b = Batch()
+        for k, m in shapes:
+            self.base[(k, m)] = off
+            for j in range(per_shape):
+                ptrs = [self.buf.data_ptr() + off + (j * (k + m) + i) * self.pitch for i in range(k + m)]
+                arr = (Slice * (k + m))(*[Slice(p, self.max_shard) for p in ptrs])
+                _check(lib().rsgpu_batch_add_encode(b._h, self.codecs[(k, m)]._h, arr, k + m))
+            off += (k + m) * per_shape * self.pitch
+        b.run()
+        torch.cuda.synchronize()
+
+    def shard_ptrs(self, shape, stripe):
+        k, m = shape
+        off = self.base[shape] + stripe * (k + m) * self.pitch
+        return [self.buf.data_ptr() + off + i * self.pitch for i in range(k + m)]
+
+
+def poisson_requests(store, n, rate, seed=0, lo=16 << 10, hi=64 << 20, mix=(0.6, 0.2, 0.2)):
+    """n requests with exponential inter-arrival times (mean 1/rate s).  Returns the
+    ctypes request array plus per-request metadata (kept alive with the array)."""
+    rng = np.random.default_rng(seed)
+    reqs = (Request * n)()
+    keep, meta = [], []
+    t = 0.0
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        shape = SHAPES[int(rng.integers(len(SHAPES)))]
+        k, m = shape
+        obj = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        S = min(store.max_shard, max(64, int(math.ceil(obj / k / 64.0)) * 64))
+        u = rng.random()
+        op = 0 if u < mix[0] else (1 if u < mix[0] + mix[1] else 2)
+        stripe = int(rng.integers(store.per_shape))
+        ptrs = (_vp * (k + m))(*store.shard_ptrs(shape, stripe))
+        erased = None
+        er = []
+        if op == 2:
+            e = int(rng.integers(1, m + 1))
+            er = sorted(rng.choice(k + m, e, replace=False).tolist())
+            erased = (C.c_uint8 * (k + m))(*[1 if j in er else 0 for j in range(k + m)])
+        keep.append((ptrs, erased))
+        reqs[i] = Request(op, 0, store.codecs[shape]._h, S, C.cast(ptrs, C.POINTER(_vp)),
+                          C.cast(erased, _u8p) if erased is not None else None,
+                          int(t * 1e9), i)
+        meta.append((shape, S, op, er, stripe))
+    return reqs, keep, meta
+
+
+def latency_summary(comps, meta):
+    done = np.array([c.done_ns for c in comps], np.float64)
+    arr = np.array([c.arrival_ns for c in comps], np.float64)
+    lat = (done - arr) / 1e6
+    data = sum(s[0] * S for (s, S, op, er, st) in meta)
+    moved = sum((s[0] + (s[1] if op != 2 else len(er))) * S for (s, S, op, er, st) in meta)
+    span = (done.max() - arr.min()) / 1e9
+    return {
+        "requests": len(comps), "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+        "mean_ms": float(lat.mean()), "seconds": span,
+        "data_gbs": data / span / 1e9, "moved_gbs": moved / span / 1e9,
+        "offered_req_s": len(comps) / ((arr.max() - arr.min()) / 1e9 if arr.max() > arr.min() else 1),
+    }
