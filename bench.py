@@ -10,6 +10,12 @@ bytes, device-resident.  One step encodes all 512 stripes.
                         per stripe (data and parity mixed), plain and batched
   encode_8x3_*        : the 8+3 x 1 MiB shape
   stripe_latency_ms   : p50/p99 device time of one plain stripe encode
+  *_dispatched_*      : per-stripe calls with the learned dispatcher's variant tables
+  dispatch_live       : the dispatcher judged live on unseen call shapes (regret vs exhaustive
+                        timing, gain vs the default tiling)
+  leopard             : Leopard GF8 128+64 / GF16 1024+256 / 4096+1024, 4-256 KiB shards
+  serving             : open-loop Poisson request streams through the request server
+                        (continuous batching, 200 GB admission budget): p50/p99 latency, GB/s
   roofline            : the batch kernel (dominant kernel) vs the 8 TB/s HBM peak
   cpu_baseline        : the oracle (scalar C restatement of the reference algorithm) on a
                         bounded sample, one host thread
@@ -41,6 +47,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ragged", type=int, default=8192,
                     help="stripes in the ragged mix (0 skips it)")
+    ap.add_argument("--no-leopard", dest="leopard", action="store_false",
+                    help="skip the Leopard GF8 128+64 / GF16 1024+256 / 4096+1024 section")
+    ap.add_argument("--dispatch-live", type=int, default=32,
+                    help="call shapes on which the dispatcher is judged live (0 skips)")
+    ap.add_argument("--rates", default="2000,10000,50000",
+                    help="request rates (req/s) for the serving section ('' skips)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the tuned batch encode launches plus a 1 GiB device copy "
                          "(the reference kernel for scaling rocprofv3 byte counters)")
@@ -205,6 +217,67 @@ def cpu_baseline(k, m, shard, stripes):
                       "1 thread"}
 
 
+def leopard_section(torch, rsgpu, DSP, disp):
+    """Leopard codecs at 4-256 KiB shards: device time per call with the default
+    kernel choice and with the dispatcher's variant tables (data GB/s = k*S/t)."""
+    out = {}
+    for name, fam, k, m in [("gf8_128+64", "leopard_gf8", 128, 64),
+                            ("gf16_1024+256", "leopard_gf16", 1024, 256),
+                            ("gf16_4096+1024", "leopard_gf16", 4096, 1024)]:
+        rs = rsgpu.ReedSolomon(k, m, codec_family=fam)
+        rows = {}
+        for S in (4096, 65536, 262144):
+            for op, e in (("encode", 0), ("reconstruct", m)):
+                case = DSP.OpCase(torch, rs, fam, k, m, S, op, e, seed=S + e)
+                rs.set_variant_table(None)
+                t_def = case.time([-1])[-1]
+                disp.install(rs, fam)
+                v = rs.variant_for(op, e, S)
+                t_dsp = case.time([-1])[-1]
+                rs.set_variant_table(None)
+                rows[f"{op}_{S // 1024}KiB"] = {
+                    "default_gbs": round(k * S / t_def / 1e3, 2),
+                    "dispatched_gbs": round(k * S / t_dsp / 1e3, 2),
+                    "dispatched_variant": v, "dispatched_us": round(t_dsp, 2)}
+                del case
+        out[name] = rows
+        del rs
+    return out
+
+
+def serving_section(torch, rates, budget=200e9):
+    """Open-loop Poisson streams through the request server (continuous batching,
+    HBM admission); objects log-uniform 16 KiB-64 MiB on 10+4 / 12+4 / 16+4."""
+    from rsgpu import serve as SV
+
+    store = SV.Store(torch, per_shape=8)
+    srv = SV.Server(hbm_budget=int(budget), arena_bytes=16 << 30, max_inflight_batches=4,
+                    max_batch_ops=4096)
+    reqs, keep, meta = SV.poisson_requests(store, 400, 20000.0, seed=1)
+    srv.replay(reqs)  # warm-up: decode rows of the erasure patterns, batch buffers
+    out = {"budget_bytes": int(budget), "store_bytes": int(store.buf.numel()),
+           "batch_slots": 4, "rates": {}}
+    for rate in rates:
+        n = int(min(rate * 1.0, 25000))
+        reqs, keep, meta = SV.poisson_requests(store, n, float(rate), seed=rate)
+        comps = srv.replay(reqs)
+        summ = SV.latency_summary(comps, meta)
+        summ = {k: (round(float(v), 4) if isinstance(v, float) or hasattr(v, "dtype") else v)
+                for k, v in summ.items()}
+        out["rates"][str(rate)] = summ
+    # capacity: every request arrives at once
+    reqs, keep, meta = SV.poisson_requests(store, 4000, 1e12, seed=77)
+    comps = srv.replay(reqs)
+    cap = SV.latency_summary(comps, meta)
+    out["saturated_4000_requests"] = {"data_gbs": round(float(cap["data_gbs"]), 1),
+                                      "moved_gbs": round(float(cap["moved_gbs"]), 1),
+                                      "seconds": round(float(cap["seconds"]), 4)}
+    out["stats"] = srv.stats()
+    srv.close()
+    del store
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -231,6 +304,14 @@ def main():
     t_enc_plain = timed(torch, lambda: [st.encode(s, sp) for s in range(N)], args.steps, args.warmup)
     lat = event_durations(torch, lambda i: st.encode(i % N, sp), min(N, 256))
     ref_parity = st.buf[:, k:].clone()
+    from rsgpu import dispatch as DSP
+
+    disp = DSP.Dispatcher.load()
+    disp.install(st.rs, "gf8")
+    enc_variant = st.rs.variant_for("encode", 0, S)
+    t_enc_plain_d = timed(torch, lambda: [st.encode(s, sp) for s in range(N)], args.steps, args.warmup)
+    st.rs.set_variant_table(None)
+    assert torch.equal(st.buf[:, k:], ref_parity), "dispatched encode differs"
 
     # ---- batched encode: one persistent launch, replayed from a hipGraph
     b_enc = st.batch("encode", sp)
@@ -309,6 +390,11 @@ def main():
         rg.make_consistent(sp)
         torch.cuda.synchronize()
         t_rg_plain = timed(torch, lambda: rg.run_plain(sp), 1, 1)
+        for (rk, rm), rcodec in rg.codecs.items():
+            disp.install(rcodec, "gf8")
+        t_rg_plain_d = timed(torch, lambda: rg.run_plain(sp), 1, 1)
+        for rcodec in rg.codecs.values():
+            rcodec.set_variant_table(None)
         b_rg = rg.make_batch(sp)
         rg_tune = b_rg.autotune(reps=2)
 
@@ -322,10 +408,19 @@ def main():
                   "plain_gbs": gbs(rg.data_bytes, t_rg_plain),
                   "batched_gbs": gbs(rg.data_bytes, t_rg_batch),
                   "batched_bytes_moved_gbs": gbs(rg.moved_bytes, t_rg_batch),
+                  "plain_dispatched_gbs": gbs(rg.data_bytes, t_rg_plain_d),
                   "batched_vs_plain": round(t_rg_plain / t_rg_batch, 2),
+                  "batched_vs_plain_dispatched": round(t_rg_plain_d / t_rg_batch, 2),
                   "batched_ms": round(t_rg_batch * 1e3, 3), "plain_ms": round(t_rg_plain * 1e3, 3),
                   "tune": rg_tune}
         del b_rg, rg
+
+    leopard = leopard_section(torch, rsgpu, DSP, disp) if args.leopard else None
+    dispatch_live = None
+    if args.dispatch_live:
+        dispatch_live = DSP.evaluate_live(torch, disp, DSP.sample_specs(args.dispatch_live,
+                                                                        seed=20261204))
+    serving = serving_section(torch, [int(r) for r in args.rates.split(",")]) if args.rates else None
 
     cpu = None if args.no_cpu else cpu_baseline(k, m, S, args.cpu_stripes)
     rec_moved = N * k * S + n_erased * S
@@ -349,6 +444,8 @@ def main():
                    "shard_pitch_bytes": pitch},
         "encode_batched_gbs": gbs(data_bytes, t_enc_batch),
         "encode_plain_gbs": gbs(data_bytes, t_enc_plain),
+        "encode_plain_dispatched_gbs": gbs(data_bytes, t_enc_plain_d),
+        "encode_plain_dispatched_variant": enc_variant,
         "encode_batched_bytes_moved_gbs": gbs(N * (k + m) * S, t_enc_batch),
         "encode_batched_gbs_pow2_pitch": gbs(data_bytes, t_enc_p2),
         "encode_batched_vs_plain": round(t_enc_plain / t_enc_batch, 2),
@@ -360,6 +457,9 @@ def main():
         "encode_8x3_batched_gbs": gbs(N * 8 * S, t83_batch),
         "encode_8x3_plain_gbs": gbs(N * 8 * S, t83_plain),
         "ragged": ragged,
+        "dispatch_live": dispatch_live,
+        "leopard": leopard,
+        "serving": serving,
         "stripe_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                               "p99": round(float(np.percentile(lat, 99)), 4)},
         "batch_tune": tune,

@@ -379,6 +379,41 @@ def autotune(torch, rs, kind, shard_len, op, erasures, seed=0):
     return min(t, key=t.get), t
 
 
+def evaluate_live(torch, disp, specs, seed=900):
+    """Judges the network live: for each call shape, the variant its installed
+    table picks vs exhaustive timing of every valid variant (regret) and vs the
+    default tiling (gain).  Returns per-kind and overall summaries."""
+    rows = []
+    for i, (kind, k, m, size, op, e) in enumerate(specs):
+        rs = make_codec(kind, k, m)
+        disp.install(rs, kind)
+        v = rs.variant_for(op, e, size)
+        rs.set_variant_table(None)  # variant -1 is then the default tiling
+        case = OpCase(torch, rs, kind, k, m, size, op, e, seed=seed + i)
+        vs = case.valid_variants()
+        if v != -1 and v not in vs:
+            raise AssertionError(f"table picked invalid variant {v} for {(kind, k, m, size, op, e)}")
+        t = case.time(vs + [-1])
+        best = min(t[x] for x in vs)
+        rows.append((kind, t[v] if v >= 0 else t[-1], t[-1], best))
+        del case
+    tp = np.array([r[1] for r in rows])
+    td = np.array([r[2] for r in rows])
+    tb = np.array([r[3] for r in rows])
+    reg = tp / tb - 1.0
+
+    def summ(sel):
+        return {"n": int(sel.sum()), "regret_mean": float(reg[sel].mean()),
+                "regret_total": float(tp[sel].sum() / tb[sel].sum() - 1.0),
+                "gain_vs_default_geomean": float(np.exp(np.mean(np.log(td[sel] / tp[sel])))),
+                "gain_vs_default_total": float(td[sel].sum() / tp[sel].sum())}
+
+    out = summ(np.ones(len(rows), bool))
+    out["by_kind"] = {kind: summ(np.array([r[0] == kind for r in rows])) for kind in KINDS
+                      if any(r[0] == kind for r in rows)}
+    return out
+
+
 def _main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m rsgpu.dispatch")
     sub = ap.add_subparsers(dest="cmd", required=True)

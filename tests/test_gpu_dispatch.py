@@ -127,34 +127,13 @@ def test_dispatcher_live_regret_against_exhaustive_autotune():
     announce("dispatcher live regret")
     disp = D.Dispatcher.load()
     specs = D.sample_specs(48, seed=20261203)
-    rows = []
-    for i, (kind, k, m, size, op, e) in enumerate(specs):
-        rs = D.make_codec(kind, k, m)
-        disp.install(rs, kind)
-        v = rs.variant_for(op, e, size)
-        # time with the table removed: variant -1 is then the default tiling
-        rs.set_variant_table(None)
-        case = D.OpCase(torch, rs, kind, k, m, size, op, e, seed=900 + i)
-        vs = case.valid_variants()
-        assert v == -1 or v in vs
-        t = case.time(vs + [-1])
-        best = min(t[x] for x in vs)
-        rows.append((kind, t[v] if v >= 0 else t[-1], t[-1], best))
-    tp = np.array([r[1] for r in rows])
-    td = np.array([r[2] for r in rows])
-    tb = np.array([r[3] for r in rows])
-    regret = tp / tb - 1.0
-    geo_gain = float(np.exp(np.mean(np.log(td / tp))))
-    for kind in D.KINDS:
-        sel = np.array([r[0] == kind for r in rows])
-        if sel.any():
-            print(f"  {kind}: n={sel.sum()} regret mean {regret[sel].mean():.3f} "
-                  f"gain vs default geomean {np.exp(np.mean(np.log(td[sel] / tp[sel]))):.3f}", flush=True)
-    print(f"live regret mean {regret.mean():.3f} total {tp.sum() / tb.sum() - 1:.3f}; gain vs default "
-          f"geomean {geo_gain:.3f} total {td.sum() / tp.sum():.3f}", flush=True)
-    assert regret.mean() <= 0.10
-    assert geo_gain >= 1.1           # per call, the network beats the default tiling
-    assert td.sum() / tp.sum() >= 0.98  # and never costs total time
+    r = D.evaluate_live(torch, disp, specs)
+    for kind, v in r["by_kind"].items():
+        print(f"  {kind}: {v}", flush=True)
+    print(f"live: {r}", flush=True)
+    assert r["regret_mean"] <= 0.10
+    assert r["gain_vs_default_geomean"] >= 1.1   # per call, the network beats the default tiling
+    assert r["gain_vs_default_total"] >= 0.98    # and never costs total time
 
 
 def test_installed_tables_keep_outputs_exact():
