@@ -54,6 +54,7 @@ enum {
   RSGPU_ERR_NO_DEVICE = 102,
   RSGPU_ERR_UNSUPPORTED = 103,      /* shape beyond this build's device limits */
   RSGPU_ERR_OUT_OF_MEMORY = 104,
+  RSGPU_ERR_IO = 105,               /* a block-stream read or write callback failed */
 };
 
 /* ---- codec construction: src/core/options.rs:1-147, src/core/codec.rs:264-615 ---- */
@@ -140,6 +141,38 @@ int rsgpu_reconstruct_data(const rsgpu_codec* codec, rsgpu_slot* shards, size_t 
 /* ReedSolomon::reconstruct_some (required flags of length k or k+m). */
 int rsgpu_reconstruct_some(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
                            const uint8_t* required, size_t n_required, void* stream);
+
+/* ---- block streams: src/core/stream.rs:677-1031 ----
+ * Shard streams are read and written through callbacks, one block at a time
+ * (block_size clamped to [1 KiB, 16 MiB]; the reference's default is 4 MiB).
+ * read(user, shard, buf, len) returns the bytes read (0 = end of stream, < 0 =
+ * error); write(user, shard, buf, len) returns the bytes written (<= 0 = error).
+ * Failures return RSGPU_ERR_IO (callback failure) or a codec status, with the
+ * StreamError of the reference (stream.rs:107-171) in *err: kind, shard index
+ * and the callback's return value.  Leopard codecs return
+ * RSGPU_UNSUPPORTED_CODEC_FAMILY as in the reference.  Blocks are staged through
+ * two pinned host buffers so host I/O overlaps the device copies and kernels. */
+enum { RSGPU_STREAM_OK = 0, RSGPU_STREAM_READ = 1, RSGPU_STREAM_WRITE = 2, RSGPU_STREAM_CODEC = 3 };
+typedef struct rsgpu_stream_error {
+  int32_t kind;        /* RSGPU_STREAM_* */
+  int32_t shard_index;
+  int64_t io_error;    /* READ/WRITE: the callback's return value; CODEC: the status */
+} rsgpu_stream_error;
+typedef int64_t (*rsgpu_read_cb)(void* user, size_t shard, uint8_t* buf, size_t len);
+typedef int64_t (*rsgpu_write_cb)(void* user, size_t shard, const uint8_t* buf, size_t len);
+/* ReedSolomon::encode_stream (stream.rs:677-752): readers 0..k-1 are the data
+ * streams, writers 0..m-1 the parity streams (error index k + j). */
+int rsgpu_encode_stream(const rsgpu_codec* codec, size_t n_data, size_t n_parity, rsgpu_read_cb read,
+                        rsgpu_write_cb write, void* user, size_t block_size, rsgpu_stream_error* err);
+/* ReedSolomon::verify_stream (stream.rs:760-812). */
+int rsgpu_verify_stream(const rsgpu_codec* codec, size_t n_shards, rsgpu_read_cb read, void* user,
+                        size_t block_size, int32_t* ok, rsgpu_stream_error* err);
+/* ReedSolomon::reconstruct_stream_from (stream.rs:948-1031): has_reader[i] /
+ * has_writer[i] give the Some/None layout of the reference's two slices. */
+int rsgpu_reconstruct_stream_from(const rsgpu_codec* codec, size_t n_readers, size_t n_writers,
+                                  const uint8_t* has_reader, const uint8_t* has_writer,
+                                  rsgpu_read_cb read, rsgpu_write_cb write, void* user,
+                                  size_t block_size, rsgpu_stream_error* err);
 
 /* ---- host-pointer variants (copy through library-owned device staging) ---- */
 int rsgpu_encode_host(const rsgpu_codec* codec, uint8_t* const* shards, size_t n_shards,

@@ -130,6 +130,13 @@ uint64_t ora_derived_seed(int op_tag, size_t k, size_t m, size_t shard_size);
 /* FNV-style checksum fold, tests/golden_vectors.rs:5-18 */
 uint64_t ora_checksum_shards(const uint8_t* const* shards, const size_t* lens, size_t n);
 
+/* cpu_simd.c: the benchmark's CPU baseline (SIMD restatement of the reference's
+ * CPU encode, multithreaded).  isa: 0 auto, 1 scalar, 2 AVX2 nibble tables,
+ * 3 AVX-512 GFNI.  Returns the instruction set used, < 0 on bad input. */
+int ora_cpu_isa(void);
+int ora_cpu_encode(const uint16_t* matrix, int k, int m, const uint8_t* const* data,
+                   uint8_t* const* parity, size_t len, int threads, int isa);
+
 #ifdef __cplusplus
 }
 #endif

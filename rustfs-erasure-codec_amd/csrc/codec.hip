@@ -1185,6 +1185,7 @@ const char* rsgpu_status_name(int s) {
     case RSGPU_ERR_NO_DEVICE: return "NoDevice";
     case RSGPU_ERR_UNSUPPORTED: return "Unsupported";
     case RSGPU_ERR_OUT_OF_MEMORY: return "OutOfMemory";
+    case RSGPU_ERR_IO: return "IoError";
   }
   return "Unknown";
 }

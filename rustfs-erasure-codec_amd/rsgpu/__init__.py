@@ -51,6 +51,16 @@ class RsError(Exception):
         super().__init__(f"{self.name}" + (f" ({detail})" if detail else ""))
 
 
+class StreamError(Exception):
+    """StreamError of the block-stream calls (src/core/stream.rs:107-171): `kind` is
+    "read", "write" or "codec", `shard_index` the shard, `error` the exception the
+    reader/writer raised (or its negative return value), or the codec RsError."""
+
+    def __init__(self, kind, shard_index, error):
+        self.kind, self.shard_index, self.error = kind, shard_index, error
+        super().__init__(f"{kind} error on shard {shard_index}: {error}")
+
+
 class Options(C.Structure):
     _fields_ = [
         ("field", C.c_int32), ("codec_family", C.c_int32), ("matrix_mode", C.c_int32),
@@ -334,6 +344,59 @@ class ReedSolomon:
         _check(_lib.rsgpu_cache_stats(self._h, *[C.byref(v) for v in vals]))
         return dict(zip(["requests", "hits", "misses", "entries"], [v.value for v in vals]))
 
+    # ---- block streams (src/core/stream.rs:677-1031)
+    def encode_stream(self, data, parity, block_size=4 << 20):
+        """Reads the k data streams block by block and writes the m parity streams."""
+        io = _StreamIO(list(data), list(parity), writer_offset=0)
+        err = _StreamErr()
+        rc = _lib.rsgpu_encode_stream(self._h, len(io.readers), len(io.writers), io.rd, io.wr, None,
+                                      block_size, C.byref(err))
+        io.raise_for(rc, err, writer_base=self._k)
+
+    def verify_stream(self, shards, block_size=4 << 20):
+        io = _StreamIO(list(shards), [], writer_offset=0)
+        err = _StreamErr()
+        ok = C.c_int32()
+        rc = _lib.rsgpu_verify_stream(self._h, len(io.readers), io.rd, None, block_size, C.byref(ok),
+                                      C.byref(err))
+        io.raise_for(rc, err)
+        return bool(ok.value)
+
+    def reconstruct_stream_from(self, readers, writers, block_size=4 << 20):
+        """readers[i] / writers[i]: a stream or None (the reference's Option layout)."""
+        readers, writers = list(readers), list(writers)
+        io = _StreamIO(readers, writers, writer_offset=0)
+        hr = (C.c_uint8 * max(1, len(readers)))(*[r is not None for r in readers])
+        hw = (C.c_uint8 * max(1, len(writers)))(*[w is not None for w in writers])
+        err = _StreamErr()
+        rc = _lib.rsgpu_reconstruct_stream_from(self._h, len(readers), len(writers), hr, hw, io.rd,
+                                                io.wr, None, block_size, C.byref(err))
+        io.raise_for(rc, err)
+
+    def reconstruct_stream(self, shards, block_size=4 << 20):
+        """In-memory form (stream.rs:839-930): `shards` are io.BytesIO; non-empty ones are
+        present (read from position 0), empty ones receive the rebuilt bytes."""
+        total = self.total_shard_count()
+        if len(shards) != total:
+            raise StreamError("codec", 0, RsError(1 if len(shards) < total else 2))
+        if self.codec_family() != "classic":
+            raise StreamError("codec", 0, RsError(15))
+        present = [len(s.getbuffer()) != 0 for s in shards]
+        if not any(present):
+            return
+        if sum(present) < self.data_shard_count():
+            raise StreamError("codec", 0, RsError(10))
+        readers, writers = [], []
+        for s, p in zip(shards, present):
+            if p:
+                s.seek(0)
+                readers.append(s)
+                writers.append(None)
+            else:
+                readers.append(None)
+                writers.append(_AppendWriter(s))
+        self.reconstruct_stream_from(readers, writers, block_size)
+
     # ---- kernel-variant dispatch (every variant produces the same bytes)
     def set_variant(self, variant):
         """Force one kernel variant for every call (-1: back to table/default)."""
@@ -361,6 +424,80 @@ class ReedSolomon:
         _check(_lib.rsgpu_codec_variant_for(self._h, OPS[op] if isinstance(op, str) else op,
                                             erasures, shard_len, C.byref(v)))
         return v.value
+
+
+class _StreamErr(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("shard_index", C.c_int32), ("io_error", C.c_int64)]
+
+
+_READ_CB = C.CFUNCTYPE(C.c_int64, _vp, _sz, C.POINTER(C.c_uint8), _sz)
+_WRITE_CB = C.CFUNCTYPE(C.c_int64, _vp, _sz, C.POINTER(C.c_uint8), _sz)
+_lib.rsgpu_encode_stream.argtypes = [_vp, _sz, _sz, _READ_CB, _WRITE_CB, _vp, _sz, C.POINTER(_StreamErr)]
+_lib.rsgpu_verify_stream.argtypes = [_vp, _sz, _READ_CB, _vp, _sz, _i32p, C.POINTER(_StreamErr)]
+_lib.rsgpu_reconstruct_stream_from.argtypes = [_vp, _sz, _sz, _u8p, _u8p, _READ_CB, _WRITE_CB, _vp, _sz,
+                                               C.POINTER(_StreamErr)]
+
+
+class _AppendWriter:
+    """Appends to a BytesIO without moving its position (the reference writes
+    through the cursor's Vec, stream.rs:423-436)."""
+
+    def __init__(self, bio):
+        self.bio = bio
+
+    def write(self, b):
+        pos = self.bio.tell()
+        self.bio.seek(0, 2)
+        n = self.bio.write(b)
+        self.bio.seek(pos)
+        return n
+
+
+class _StreamIO:
+    """ctypes callbacks over Python readers (readinto/read) and writers (write)."""
+
+    def __init__(self, readers, writers, writer_offset=0):
+        self.readers, self.writers = readers, writers
+        self.exc = {}
+
+        def rd(_user, shard, buf, n):
+            try:
+                r = self.readers[shard]
+                view = (C.c_uint8 * n).from_address(C.addressof(buf.contents))
+                if hasattr(r, "readinto"):
+                    got = r.readinto(memoryview(view).cast("B"))
+                    return int(got or 0)
+                data = r.read(n)
+                C.memmove(view, data, len(data))
+                return len(data)
+            except Exception as e:  # reported as StreamError("read", shard, e)
+                self.exc[("read", shard)] = e
+                return -1
+
+        def wr(_user, shard, buf, n):
+            try:
+                got = self.writers[shard].write(C.string_at(buf, n))
+                return n if got is None else int(got)
+            except Exception as e:
+                self.exc[("write", shard)] = e
+                return -1
+
+        self.rd, self.wr = _READ_CB(rd), _WRITE_CB(wr)
+
+    def raise_for(self, rc, err, writer_base=0):
+        if rc == 0:
+            return
+        if rc == 105:  # RSGPU_ERR_IO
+            kind = "read" if err.kind == 1 else "write"
+            idx = err.shard_index
+            local = idx - writer_base if kind == "write" else idx
+            e = self.exc.get((kind, local))
+            if e is None:
+                e = OSError(f"{kind} returned {err.io_error}")
+            raise StreamError(kind, idx, e)
+        if 1 <= rc <= 16:
+            raise StreamError("codec", err.shard_index, RsError(rc))
+        raise RsError(rc)
 
 
 _lib.rsgpu_variant_count.argtypes = []

@@ -81,6 +81,9 @@ def _declare(L):
     L.ora_smallrng_fill.argtypes = [C.c_uint64, u8p, C.c_size_t]
     L.ora_derived_seed.restype = C.c_uint64
     L.ora_derived_seed.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t]
+    L.ora_cpu_isa.restype = C.c_int
+    L.ora_cpu_encode.argtypes = [u16p, C.c_int, C.c_int, ptrs, ptrs, C.c_size_t, C.c_int, C.c_int]
+    L.ora_cpu_encode.restype = C.c_int
     L.ora_checksum_shards.restype = C.c_uint64
     L.ora_checksum_shards.argtypes = [ptrs, C.POINTER(C.c_size_t), C.c_size_t]
 
@@ -145,7 +148,16 @@ def matrix_invert(field, a):
     return rc, out
 
 
+def _field(f):
+    """The oracle's field ids are 8 (GF(2^8)) and 16 (the GF(2^16) tower); the C
+    side would read any other value as the tower field."""
+    if f not in (GF8, GF16):
+        raise ValueError(f"oracle field must be GF8 (8) or GF16 (16), not {f!r}")
+    return f
+
+
 def build_matrix(field, mode, k, m, custom=None):
+    _field(field)
     out = np.zeros((k + m, k), np.uint16)
     cp = None
     if custom is not None:
@@ -156,6 +168,7 @@ def build_matrix(field, mode, k, m, custom=None):
 
 
 def apply_rows(field, rows, inputs):
+    _field(field)
     rows = np.ascontiguousarray(rows, np.uint16)
     r, k = rows.shape
     inputs = [np.ascontiguousarray(x, np.uint8) for x in inputs]
@@ -165,6 +178,7 @@ def apply_rows(field, rows, inputs):
 
 
 def encode(field, matrix, k, m, data):
+    _field(field)
     data = [np.ascontiguousarray(x, np.uint8) for x in data]
     parity = [np.zeros_like(data[0]) for _ in range(m)]
     matrix = np.ascontiguousarray(matrix, np.uint16)
@@ -173,6 +187,7 @@ def encode(field, matrix, k, m, data):
 
 
 def verify(field, matrix, k, m, shards):
+    _field(field)
     shards = [np.ascontiguousarray(x, np.uint8) for x in shards]
     ok = C.c_int(0)
     matrix = np.ascontiguousarray(matrix, np.uint16)
@@ -181,6 +196,7 @@ def verify(field, matrix, k, m, shards):
 
 
 def reconstruct(field, matrix, k, m, shards, present, data_only=False, required=None):
+    _field(field)
     """shards: list of uint8 arrays (missing ones are overwritten in place)."""
     shards = [np.ascontiguousarray(x, np.uint8) for x in shards]
     pres = np.ascontiguousarray(present, np.uint8)
@@ -192,6 +208,7 @@ def reconstruct(field, matrix, k, m, shards, present, data_only=False, required=
 
 
 def decode_matrix(field, matrix, k, m, present):
+    _field(field)
     pres = np.ascontiguousarray(present, np.uint8)
     valid = (C.c_int * k)()
     dec = np.zeros((k, k), np.uint16)
@@ -277,3 +294,17 @@ def checksum_shards(shards):
     shards = [np.ascontiguousarray(x, np.uint8) for x in shards]
     lens = (C.c_size_t * len(shards))(*[s.size for s in shards])
     return lib().ora_checksum_shards(_ptrs(shards), lens, len(shards))
+
+
+CPU_ISA = {0: "auto", 1: "scalar", 2: "avx2", 3: "gfni-avx512"}
+
+
+def cpu_encode(matrix, k, m, data, threads=1, isa=0, parity=None):
+    """The benchmark's CPU baseline (oracle/cpu_simd.c); returns (isa used, parity)."""
+    data = [np.ascontiguousarray(x, np.uint8) for x in data]
+    if parity is None:
+        parity = [np.zeros_like(data[0]) for _ in range(m)]
+    matrix = np.ascontiguousarray(matrix, np.uint16)
+    used = lib().ora_cpu_encode(_u16(matrix), k, m, _ptrs(data), _ptrs(parity), data[0].size,
+                                threads, isa)
+    return used, parity

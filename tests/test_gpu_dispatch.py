@@ -146,8 +146,8 @@ def test_installed_tables_keep_outputs_exact():
         rng = np.random.default_rng(k + n)
         data = rand_shards(rng, k, n)
         if kind == "gf8":
-            _, mat = O.build_matrix(0, 0, k, m)
-            par = O.encode(0, mat, k, m, data)
+            _, mat = O.build_matrix(O.GF8, O.VANDERMONDE, k, m)
+            par = O.encode(O.GF8, mat, k, m, data)
         else:
             _, par = O.leo_encode(8, k, m, data)
         sh = [to_dev(d) for d in data] + [torch.zeros(n, dtype=torch.uint8, device="cuda")

@@ -1,0 +1,174 @@
+"""Block-stream encode / verify / reconstruct (src/core/stream.rs:677-1031) on the
+GPU path: bytes against the oracle, the zero-padding and length rules, the
+error kinds and indices of StreamError, and the reference's own stream tests
+(stream.rs tests: basic, multi-block, partial last block, verify, reconstruct)."""
+import io
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import announce
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+rsgpu = pytest.importorskip("rsgpu")
+
+
+def ref_data(n):
+    # random_data of the reference's stream tests (stream.rs tests)
+    return bytes(((i * 73 + 17) & 0xFF) for i in range(n))
+
+
+def oracle_parity(k, m, data):
+    rc, mat = O.build_matrix(O.GF8, O.VANDERMONDE, k, m)
+    arrs = [np.frombuffer(d, np.uint8) for d in data]
+    return [p.tobytes() for p in O.encode(O.GF8, mat, k, m, arrs)]
+
+
+@pytest.mark.parametrize("k,m,total,block", [(3, 2, 4096, 4 << 20), (3, 2, 10 * 1024, 4096),
+                                             (10, 4, (1 << 20) + 333, 65536), (6, 3, 5000, 0),
+                                             (20, 4, 3 << 20, 1 << 20)])
+def test_encode_stream_matches_oracle(k, m, total, block):
+    announce(f"stream encode {k}+{m} total={total} block={block}")
+    rng = np.random.default_rng(total + k)
+    data = [rng.integers(0, 256, total, dtype=np.uint8).tobytes() for _ in range(k)]
+    rs = rsgpu.ReedSolomon(k, m)
+    writers = [io.BytesIO() for _ in range(m)]
+    rs.encode_stream([io.BytesIO(d) for d in data], writers, block_size=block)
+    exp = oracle_parity(k, m, data)
+    for j in range(m):
+        assert writers[j].getvalue() == exp[j], j
+    assert rs.verify_stream([io.BytesIO(d) for d in data] + [io.BytesIO(w.getvalue()) for w in writers],
+                            block_size=block)
+
+
+def test_encode_stream_zero_pads_short_streams():
+    """Each block is as long as the longest read; shorter data streams are padded
+    with zeros (stream.rs:259-283)."""
+    announce("stream encode zero padding")
+    k, m = 4, 2
+    lens = [3000, 2500, 3000, 100]
+    rng = np.random.default_rng(1)
+    data = [rng.integers(0, 256, n, dtype=np.uint8).tobytes() for n in lens]
+    rs = rsgpu.ReedSolomon(k, m)
+    writers = [io.BytesIO() for _ in range(m)]
+    rs.encode_stream([io.BytesIO(d) for d in data], writers, block_size=1024)
+    padded = [d + bytes(3000 - len(d)) for d in data]
+    exp = oracle_parity(k, m, padded)
+    assert [w.getvalue() for w in writers] == exp
+
+
+def test_verify_stream_detects_corruption_and_length_mismatch():
+    announce("stream verify")
+    k, m, n = 3, 2, 9000
+    data = [ref_data(n) for _ in range(k)]
+    par = oracle_parity(k, m, data)
+    rs = rsgpu.ReedSolomon(k, m)
+    good = data + par
+    assert rs.verify_stream([io.BytesIO(x) for x in good], block_size=2048)
+    bad = list(good)
+    b = bytearray(bad[4])
+    b[8500] ^= 0x40  # in the last (partial) block
+    bad[4] = bytes(b)
+    assert not rs.verify_stream([io.BytesIO(x) for x in bad], block_size=2048)
+    short = list(good)
+    short[1] = short[1][:5000]
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.verify_stream([io.BytesIO(x) for x in short], block_size=2048)
+    assert ei.value.kind == "codec" and ei.value.shard_index == 1
+    assert ei.value.error.name == "IncorrectShardSize"
+    # empty streams verify as true (no blocks)
+    assert rs.verify_stream([io.BytesIO(b"") for _ in range(k + m)])
+
+
+@pytest.mark.parametrize("missing", [(0,), (1, 4), (2, 3), (0, 1)])
+def test_reconstruct_stream_from_rebuilds_missing(missing):
+    announce(f"stream reconstruct {missing}")
+    k, m, n = 3, 2, 10 * 1024 + 7
+    rng = np.random.default_rng(sum(missing) + 3)
+    data = [rng.integers(0, 256, n, dtype=np.uint8).tobytes() for _ in range(k)]
+    full = data + oracle_parity(k, m, data)
+    rs = rsgpu.ReedSolomon(k, m)
+    readers = [None if i in missing else io.BytesIO(full[i]) for i in range(k + m)]
+    writers = [io.BytesIO() if i in missing else None for i in range(k + m)]
+    rs.reconstruct_stream_from(readers, writers, block_size=4096)
+    for i in missing:
+        assert writers[i].getvalue() == full[i], i
+
+
+def test_reconstruct_stream_cursor_form():
+    announce("stream reconstruct cursors")
+    k, m, n = 4, 2, 6000
+    data = [ref_data(n) for _ in range(k)]
+    full = data + oracle_parity(k, m, data)
+    rs = rsgpu.ReedSolomon(k, m)
+    shards = [io.BytesIO(x) for x in full]
+    for s in shards:
+        s.seek(0, 2)  # present cursors are read from position 0 regardless
+    shards[1] = io.BytesIO()
+    shards[5] = io.BytesIO()
+    rs.reconstruct_stream(shards, block_size=1024)
+    assert shards[1].getvalue() == full[1] and shards[5].getvalue() == full[5]
+    # nothing present: nothing to do
+    rs.reconstruct_stream([io.BytesIO() for _ in range(k + m)])
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.reconstruct_stream([io.BytesIO(full[0])] + [io.BytesIO() for _ in range(k + m - 1)])
+    assert ei.value.error.name == "TooFewShardsPresent"
+
+
+def test_stream_errors_match_reference():
+    announce("stream errors")
+    k, m = 3, 2
+    rs = rsgpu.ReedSolomon(k, m)
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.encode_stream([io.BytesIO(b"x")] * 2, [io.BytesIO(), io.BytesIO()])
+    assert ei.value.error.name == "TooFewShards" and ei.value.shard_index == 0
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.verify_stream([io.BytesIO(b"x")] * 6)
+    assert ei.value.error.name == "TooManyShards"
+    leo = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf8")
+    with pytest.raises(rsgpu.StreamError) as ei:
+        leo.encode_stream([io.BytesIO(b"x")] * 3, [io.BytesIO(), io.BytesIO()])
+    assert ei.value.error.name == "UnsupportedCodecFamily"
+    # layout: a shard with both a reader and a writer
+    readers = [io.BytesIO(b"abc")] * 5
+    writers = [None, io.BytesIO(), None, None, None]
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.reconstruct_stream_from(readers, writers)
+    assert ei.value.error.name == "InvalidShardFlags" and ei.value.shard_index == 1
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.reconstruct_stream_from([None, None, None, io.BytesIO(b"a"), io.BytesIO(b"b")],
+                                   [io.BytesIO(), io.BytesIO(), io.BytesIO(), None, None])
+    assert ei.value.error.name == "TooFewShardsPresent"
+
+    class Boom:
+        def __init__(self, after):
+            self.left = after
+
+        def readinto(self, buf):
+            if self.left <= 0:
+                raise IOError("disk gone")
+            n = min(len(buf), self.left, 1024)
+            buf[:n] = b"\x01" * n
+            self.left -= n
+            return n
+
+    # a read error on the second block: the first block's parity is already written
+    writers = [io.BytesIO(), io.BytesIO()]
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.encode_stream([io.BytesIO(ref_data(8192)), Boom(4096), io.BytesIO(ref_data(8192))], writers,
+                         block_size=4096)
+    assert ei.value.kind == "read" and ei.value.shard_index == 1
+    assert isinstance(ei.value.error, IOError)
+    first = oracle_parity(k, m, [ref_data(4096), b"\x01" * 4096, ref_data(4096)])
+    assert [w.getvalue() for w in writers] == first
+
+    class Full:
+        def write(self, b):
+            raise IOError("no space")
+
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.encode_stream([io.BytesIO(ref_data(100))] * 3, [io.BytesIO(), Full()])
+    assert ei.value.kind == "write" and ei.value.shard_index == k + 1

@@ -24,14 +24,15 @@ def n_variants():
 
 
 def _classic_expected(field, k, m, data):
-    f = 0 if field == "gf8" else 1
-    rc, mat = O.build_matrix(f, 0, k, m)
+    f = O.GF8 if field == "gf8" else O.GF16
+    rc, mat = O.build_matrix(f, O.VANDERMONDE, k, m)
     assert rc == 0
     return mat, O.encode(f, mat, k, m, data)
 
 
 @pytest.mark.parametrize("field,k,m,length", [("gf8", 10, 4, 65536 + 48), ("gf8", 3, 9, 4096),
-                                              ("gf8", 70, 3, 1000), ("gf16", 6, 3, 8192 + 2),
+                                              ("gf8", 70, 3, 1000), ("gf8", 7, 3, 4099),
+                                              ("gf16", 6, 3, 8192 + 2),
                                               ("gf16", 5, 5, 333 * 2)])
 def test_classic_every_variant_matches_oracle(field, k, m, length):
     announce(f"variants classic {field} {k}+{m} len={length}")
