@@ -55,6 +55,8 @@ def parse():
                     help="call shapes on which the dispatcher is judged live (0 skips)")
     ap.add_argument("--rates", default="2000,10000,50000",
                     help="request rates (req/s) for the serving section ('' skips)")
+    ap.add_argument("--leopard-only", action="store_true",
+                    help="only the Leopard section (for profiling its kernels)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the tuned batch encode launches plus a 1 GiB device copy "
                          "(the reference kernel for scaling rocprofv3 byte counters)")
@@ -297,6 +299,15 @@ def main():
         sys.exit(2)
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.leopard_only:
+        import torch
+
+        import rsgpu
+        from rsgpu import dispatch as DSP
+
+        torch.cuda.init()
+        print(json.dumps({"leopard": leopard_section(torch, rsgpu, DSP, DSP.Dispatcher.load())}))
         return
     import numpy as np
     import torch

@@ -310,6 +310,7 @@ struct LeoArgs {
   uint32_t n_groups, fft_layer0, fft_layers, ifft_layer0, ifft_layers;
   int32_t* flag;               // verify
   uint32_t mode;               // 0 encode, 1 verify
+  uint32_t vec;                // every shard pointer 16-byte aligned
 };
 
 template <bool G16>
@@ -323,41 +324,94 @@ struct Word<true> {
   uint32_t lo, hi;
 };
 
-template <bool G16>
-__device__ __forceinline__ Word<G16> lds_get(const uint32_t* lds, uint32_t idx) {
-  Word<G16> w;
-  if constexpr (G16) {
-    uint2 t = reinterpret_cast<const uint2*>(lds)[idx];
-    w.lo = t.x;
-    w.hi = t.y;
+// G consecutive words of one LDS row.  G = 4 moves them as 16-byte LDS accesses
+// (consecutive lanes take consecutive 16-byte slots: conflict free); the row
+// stride C must then be a multiple of 4.
+template <bool G16, int G>
+__device__ __forceinline__ void lds_getv(const uint32_t* lds, uint32_t idx, Word<G16> (&w)[G]) {
+  if constexpr (G == 4 && !G16) {
+    uint4 t = *reinterpret_cast<const uint4*>(lds + idx);
+    w[0].v = t.x, w[1].v = t.y, w[2].v = t.z, w[3].v = t.w;
+  } else if constexpr (G == 4 && G16) {
+    uint4 t0 = *reinterpret_cast<const uint4*>(lds + 2 * idx);
+    uint4 t1 = *reinterpret_cast<const uint4*>(lds + 2 * idx + 4);
+    w[0].lo = t0.x, w[0].hi = t0.y, w[1].lo = t0.z, w[1].hi = t0.w;
+    w[2].lo = t1.x, w[2].hi = t1.y, w[3].lo = t1.z, w[3].hi = t1.w;
   } else {
-    w.v = lds[idx];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if constexpr (G16) {
+        uint2 t = reinterpret_cast<const uint2*>(lds)[idx + g];
+        w[g].lo = t.x, w[g].hi = t.y;
+      } else {
+        w[g].v = lds[idx + g];
+      }
+    }
   }
-  return w;
 }
-template <bool G16>
-__device__ __forceinline__ void lds_put(uint32_t* lds, uint32_t idx, Word<G16> w) {
-  if constexpr (G16) reinterpret_cast<uint2*>(lds)[idx] = make_uint2(w.lo, w.hi);
-  else lds[idx] = w.v;
+template <bool G16, int G>
+__device__ __forceinline__ void lds_putv(uint32_t* lds, uint32_t idx, const Word<G16> (&w)[G]) {
+  if constexpr (G == 4 && !G16) {
+    *reinterpret_cast<uint4*>(lds + idx) = make_uint4(w[0].v, w[1].v, w[2].v, w[3].v);
+  } else if constexpr (G == 4 && G16) {
+    *reinterpret_cast<uint4*>(lds + 2 * idx) = make_uint4(w[0].lo, w[0].hi, w[1].lo, w[1].hi);
+    *reinterpret_cast<uint4*>(lds + 2 * idx + 4) = make_uint4(w[2].lo, w[2].hi, w[3].lo, w[3].hi);
+  } else {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if constexpr (G16) reinterpret_cast<uint2*>(lds)[idx + g] = make_uint2(w[g].lo, w[g].hi);
+      else lds[idx + g] = w[g].v;
+    }
+  }
 }
 
-__device__ __forceinline__ uint32_t tmul(const uint32_t* t, const Sel& s) {
+// A twiddle's multiply tables held in registers for the words of one item:
+// GF8 one 5-dword table, GF16 the four byte maps (low->low, low->high,
+// high->low, high->high) at dword offsets 0, 8, 16, 24.
+template <bool G16>
+struct Tab;
+template <>
+struct Tab<false> {
+  uint32_t a[5];
+};
+template <>
+struct Tab<true> {
+  uint32_t a[4][5];
+};
+
+__device__ __forceinline__ void load5(const uint32_t* p, uint32_t (&a)[5]) {
+  uint4 q = *reinterpret_cast<const uint4*>(p);  // tables are 32-byte aligned
+  a[0] = q.x, a[1] = q.y, a[2] = q.z, a[3] = q.w, a[4] = p[4];
+}
+template <bool G16>
+__device__ __forceinline__ Tab<G16> load_tab(const uint32_t* p) {
+  Tab<G16> t;
+  if constexpr (G16) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) load5(p + j * kTabWords, t.a[j]);
+  } else {
+    load5(p, t.a);
+  }
+  return t;
+}
+
+__device__ __forceinline__ uint32_t tmul5(const uint32_t (&t)[5], const Sel& s) {
   uint32_t a = __builtin_amdgcn_perm(t[1], t[0], s.s0);
   uint32_t b = __builtin_amdgcn_perm(t[3], t[2], s.s1);
   uint32_t c = __builtin_amdgcn_perm(t[4], t[4], s.s2);
   return xor3(a, b, c);
 }
 
-// y * g^log for the table t
+// y * g^log for the twiddle's tables
 template <bool G16>
-__device__ __forceinline__ Word<G16> wmul(const uint32_t* t, Word<G16> y) {
+__device__ __forceinline__ Word<G16> wmul(const Tab<G16>& t, Word<G16> y) {
   Word<G16> r;
   if constexpr (G16) {
     Sel sl = selectors(y.lo), sh = selectors(y.hi);
-    r.lo = tmul(t, sl) ^ tmul(t + 2 * kTabWords, sh);
-    r.hi = tmul(t + kTabWords, sl) ^ tmul(t + 3 * kTabWords, sh);
+    r.lo = tmul5(t.a[0], sl) ^ tmul5(t.a[2], sh);
+    r.hi = tmul5(t.a[1], sl) ^ tmul5(t.a[3], sh);
   } else {
-    r.v = tmul(t, selectors(y.v));
+    r.v = tmul5(t.a, selectors(y.v));
   }
   return r;
 }
@@ -372,69 +426,83 @@ __device__ __forceinline__ void wxor(Word<G16>& a, Word<G16> b) {
 }
 // fft_dit2: x ^= y * t; y ^= x      ifft_dit2: y ^= x; x ^= y * t
 template <bool G16>
-__device__ __forceinline__ void fwd2(Word<G16>& x, Word<G16>& y, const uint32_t* t) {
+__device__ __forceinline__ void fwd2(Word<G16>& x, Word<G16>& y, const Tab<G16>& t) {
   wxor(x, wmul<G16>(t, y));
   wxor(y, x);
 }
 template <bool G16>
-__device__ __forceinline__ void inv2(Word<G16>& x, Word<G16>& y, const uint32_t* t) {
+__device__ __forceinline__ void inv2(Word<G16>& x, Word<G16>& y, const Tab<G16>& t) {
   wxor(y, x);
   wxor(x, wmul<G16>(t, y));
 }
 
-// Runs layers [l0, l0 + nl) over LDS lanes [base, base + rows).
-template <bool G16>
-__device__ void run_layers(uint32_t* lds, uint32_t lane_base, uint32_t l0, uint32_t nl,
-                           const LeoArgs& a) {
+// Runs layers [l0, l0 + nl) over the LDS lanes.  An item is one butterfly on G
+// consecutive words, so its twiddle tables are loaded once for G words.
+template <bool G16, int G>
+__device__ __forceinline__ void run_layers(uint32_t* lds, uint32_t l0, uint32_t nl, const LeoArgs& a) {
   constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
-  const uint32_t C = a.C;
+  const uint32_t C = a.C, ng = a.C / G;
   for (uint32_t l = l0; l < l0 + nl; ++l) {
     const LeoLayer L = a.layers[l];
-    const uint32_t items = L.n_blocks * L.cnt * C;
+    const uint32_t items = L.n_blocks * L.cnt * ng;
     for (uint32_t it = threadIdx.x; it < items; it += blockDim.x) {
-      const uint32_t w = it % C, bf = it / C;
-      const LeoBlock B = a.blocks[L.block0 + bf / L.cnt];
-      const uint32_t i = B.r + bf % L.cnt;
+      const uint32_t bf = it / ng, wq = (it - bf * ng) * G;
+      const uint32_t blk = bf / L.cnt;
+      const LeoBlock B = a.blocks[L.block0 + blk];
+      const uint32_t i = B.r + (bf - blk * L.cnt);
       const uint32_t d = L.dist;
-      const uint32_t* t01 = a.tables + (size_t)B.t01 * TW;
+      const Tab<G16> t01 = load_tab<G16>(a.tables + (size_t)B.t01 * TW);
       if (L.radix == 2) {
-        Word<G16> x = lds_get<G16>(lds, (lane_base + i) * C + w);
-        Word<G16> y = lds_get<G16>(lds, (lane_base + i + d) * C + w);
-        if (L.inverse) inv2<G16>(x, y, t01);
-        else fwd2<G16>(x, y, t01);
-        lds_put<G16>(lds, (lane_base + i) * C + w, x);
-        lds_put<G16>(lds, (lane_base + i + d) * C + w, y);
-      } else {
-        const uint32_t* t23 = a.tables + (size_t)B.t23 * TW;
-        const uint32_t* t02 = a.tables + (size_t)B.t02 * TW;
-        Word<G16> x0 = lds_get<G16>(lds, (lane_base + i) * C + w);
-        Word<G16> x1 = lds_get<G16>(lds, (lane_base + i + d) * C + w);
-        Word<G16> x2 = lds_get<G16>(lds, (lane_base + i + 2 * d) * C + w);
-        Word<G16> x3 = lds_get<G16>(lds, (lane_base + i + 3 * d) * C + w);
-        if (L.inverse) {  // ifft_dit4 (ops.rs:556-586)
-          inv2<G16>(x0, x1, t01);
-          inv2<G16>(x2, x3, t23);
-          inv2<G16>(x1, x3, t02);
-          inv2<G16>(x0, x2, t02);
-        } else {  // fft_dit4 (ops.rs:516-548)
-          fwd2<G16>(x0, x2, t02);
-          fwd2<G16>(x1, x3, t02);
-          fwd2<G16>(x0, x1, t01);
-          fwd2<G16>(x2, x3, t23);
+        Word<G16> x[G], y[G];
+        lds_getv<G16, G>(lds, i * C + wq, x);
+        lds_getv<G16, G>(lds, (i + d) * C + wq, y);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (L.inverse) inv2<G16>(x[g], y[g], t01);
+          else fwd2<G16>(x[g], y[g], t01);
         }
-        lds_put<G16>(lds, (lane_base + i) * C + w, x0);
-        lds_put<G16>(lds, (lane_base + i + d) * C + w, x1);
-        lds_put<G16>(lds, (lane_base + i + 2 * d) * C + w, x2);
-        lds_put<G16>(lds, (lane_base + i + 3 * d) * C + w, x3);
+        lds_putv<G16, G>(lds, i * C + wq, x);
+        lds_putv<G16, G>(lds, (i + d) * C + wq, y);
+      } else {
+        const Tab<G16> t23 = load_tab<G16>(a.tables + (size_t)B.t23 * TW);
+        const Tab<G16> t02 = load_tab<G16>(a.tables + (size_t)B.t02 * TW);
+        Word<G16> x0[G], x1[G], x2[G], x3[G];
+        lds_getv<G16, G>(lds, i * C + wq, x0);
+        lds_getv<G16, G>(lds, (i + d) * C + wq, x1);
+        lds_getv<G16, G>(lds, (i + 2 * d) * C + wq, x2);
+        lds_getv<G16, G>(lds, (i + 3 * d) * C + wq, x3);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (L.inverse) {  // ifft_dit4 (ops.rs:556-586)
+            inv2<G16>(x0[g], x1[g], t01);
+            inv2<G16>(x2[g], x3[g], t23);
+            inv2<G16>(x1[g], x3[g], t02);
+            inv2<G16>(x0[g], x2[g], t02);
+          } else {  // fft_dit4 (ops.rs:516-548)
+            fwd2<G16>(x0[g], x2[g], t02);
+            fwd2<G16>(x1[g], x3[g], t02);
+            fwd2<G16>(x0[g], x1[g], t01);
+            fwd2<G16>(x2[g], x3[g], t23);
+          }
+        }
+        lds_putv<G16, G>(lds, i * C + wq, x0);
+        lds_putv<G16, G>(lds, (i + d) * C + wq, x1);
+        lds_putv<G16, G>(lds, (i + 2 * d) * C + wq, x2);
+        lds_putv<G16, G>(lds, (i + 3 * d) * C + wq, x3);
       }
     }
     __syncthreads();
   }
 }
 
-// Global addresses of word w of tile `tile` of a shard.
+// Global access to words [word, word + G) of a shard; words past `words` read as
+// zero and are not written.  G = 4 with `vec` (16-byte aligned shards, word a
+// multiple of 4) uses 16-byte loads/stores: GF8 words are 4 bytes, GF16 words are
+// the (low-byte, high-byte) dword pairs of the split layout (ops.rs:520-568:
+// element i of a 64-byte block = byte[i] | byte[i + 32] << 8), and 4 aligned
+// words sit in one block's low and high halves.
 template <bool G16>
-__device__ __forceinline__ Word<G16> gload(const uint8_t* shard, uint64_t word) {
+__device__ __forceinline__ Word<G16> gload1(const uint8_t* shard, uint64_t word) {
   Word<G16> r;
   if constexpr (G16) {
     const uint64_t b = word >> 3, j = word & 7;
@@ -446,13 +514,60 @@ __device__ __forceinline__ Word<G16> gload(const uint8_t* shard, uint64_t word) 
   return r;
 }
 template <bool G16>
-__device__ __forceinline__ void gstore(uint8_t* shard, uint64_t word, Word<G16> v) {
+__device__ __forceinline__ void gstore1(uint8_t* shard, uint64_t word, Word<G16> v) {
   if constexpr (G16) {
     const uint64_t b = word >> 3, j = word & 7;
     *reinterpret_cast<uint32_t*>(shard + b * 64 + j * 4) = v.lo;
     *reinterpret_cast<uint32_t*>(shard + b * 64 + 32 + j * 4) = v.hi;
   } else {
     *reinterpret_cast<uint32_t*>(shard + word * 4) = v.v;
+  }
+}
+template <bool G16, int G>
+__device__ __forceinline__ void gloadv(const uint8_t* shard, uint64_t word, uint64_t words, bool vec,
+                                       Word<G16> (&w)[G]) {
+  if constexpr (G == 4) {
+    if (vec && word + 4 <= words) {
+      if constexpr (G16) {
+        const uint8_t* p = shard + (word >> 3) * 64 + (word & 7) * 4;
+        uint4 lo = load16(p), hi = load16(p + 32);
+        w[0] = {lo.x, hi.x}, w[1] = {lo.y, hi.y}, w[2] = {lo.z, hi.z}, w[3] = {lo.w, hi.w};
+      } else {
+        uint4 q = load16(shard + word * 4);
+        w[0].v = q.x, w[1].v = q.y, w[2].v = q.z, w[3].v = q.w;
+      }
+      return;
+    }
+  }
+  w[0] = word < words ? gload1<G16>(shard, word) : Word<G16>{};
+  if constexpr (G > 1) {
+    w[1] = word + 1 < words ? gload1<G16>(shard, word + 1) : Word<G16>{};
+    w[2] = word + 2 < words ? gload1<G16>(shard, word + 2) : Word<G16>{};
+    w[3] = word + 3 < words ? gload1<G16>(shard, word + 3) : Word<G16>{};
+  }
+}
+template <bool G16, int G>
+__device__ __forceinline__ void gstorev(uint8_t* shard, uint64_t word, uint64_t words, bool vec,
+                                        const Word<G16> (&w)[G]) {
+  if constexpr (G == 4) {
+    if (vec && word + 4 <= words) {
+      if constexpr (G16) {
+        uint8_t* p = shard + (word >> 3) * 64 + (word & 7) * 4;
+        store16(p, make_uint4(w[0].lo, w[1].lo, w[2].lo, w[3].lo));
+        store16(p + 32, make_uint4(w[0].hi, w[1].hi, w[2].hi, w[3].hi));
+      } else {
+        store16(shard + word * 4, make_uint4(w[0].v, w[1].v, w[2].v, w[3].v));
+      }
+      return;
+    }
+  }
+  // partial last vector or unaligned shards: word by word (no indexed loop, so
+  // the words stay in registers)
+  if (word < words) gstore1<G16>(shard, word, w[0]);
+  if constexpr (G > 1) {
+    if (word + 1 < words) gstore1<G16>(shard, word + 1, w[1]);
+    if (word + 2 < words) gstore1<G16>(shard, word + 2, w[2]);
+    if (word + 3 < words) gstore1<G16>(shard, word + 3, w[3]);
   }
 }
 template <bool G16>
@@ -464,92 +579,136 @@ __device__ __forceinline__ bool wequal(Word<G16> a, Word<G16> b) {
 // encode_with_tables (leopard_gf8/encode.rs:194-372; leopard_gf16/encode.rs:109-253):
 // groups of m data lanes are IFFT'd (skew offset m-1+group*m) and XORed into the
 // first m lanes, which are then FFT'd; lanes 0..parity are the parity.
-template <bool G16>
-__global__ __launch_bounds__(256) void leo_encode_kernel(const LeoArgs a) {
-  extern __shared__ uint32_t lds[];
-  const uint32_t C = a.C, m = a.m;
+template <bool G16, int G>
+__device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds) {
+  const uint32_t C = a.C, m = a.m, ng = C / G;
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   uint32_t* A = lds;
   uint32_t* Bv = lds + (size_t)m * C * (G16 ? 2 : 1);
   for (uint32_t g = 0; g < a.n_groups; ++g) {
-    const LeoGroup G = a.groups[g];
+    const LeoGroup Gp = a.groups[g];
     uint32_t* dst = g == 0 ? A : Bv;
-    for (uint32_t it = threadIdx.x; it < m * C; it += blockDim.x) {
-      const uint32_t lane = it / C, w = it % C;
-      Word<G16> v{};
-      if (lane < G.mtrunc && w0 + w < a.words) v = gload<G16>(a.ptrs[G.first + lane], w0 + w);
-      lds_put<G16>(dst, lane * C + w, v);
+    for (uint32_t it = threadIdx.x; it < m * ng; it += blockDim.x) {
+      const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+      Word<G16> v[G];
+      if (lane < Gp.mtrunc) {
+        gloadv<G16, G>(a.ptrs[Gp.first + lane], w0 + wq, a.words, a.vec != 0, v);
+      } else {
+#pragma unroll
+        for (int q = 0; q < G; ++q) v[q] = Word<G16>{};
+      }
+      lds_putv<G16, G>(dst, lane * C + wq, v);
     }
     __syncthreads();
-    run_layers<G16>(dst, 0, G.layer0, G.n_layers, a);
+    run_layers<G16, G>(dst, Gp.layer0, Gp.n_layers, a);
     if (g > 0) {
-      for (uint32_t it = threadIdx.x; it < m * C; it += blockDim.x) {
-        Word<G16> x = lds_get<G16>(A, it), y = lds_get<G16>(Bv, it);
-        wxor(x, y);
-        lds_put<G16>(A, it, x);
+      for (uint32_t it = threadIdx.x; it < m * ng; it += blockDim.x) {
+        Word<G16> x[G], y[G];
+        lds_getv<G16, G>(A, it * G, x);
+        lds_getv<G16, G>(Bv, it * G, y);
+#pragma unroll
+        for (int q = 0; q < G; ++q) wxor(x[q], y[q]);
+        lds_putv<G16, G>(A, it * G, x);
       }
       __syncthreads();
     }
   }
-  run_layers<G16>(A, 0, a.fft_layer0, a.fft_layers, a);
+  run_layers<G16, G>(A, a.fft_layer0, a.fft_layers, a);
   bool mismatch = false;
-  for (uint32_t it = threadIdx.x; it < a.parity * C; it += blockDim.x) {
-    const uint32_t lane = it / C, w = it % C;
-    if (w0 + w >= a.words) continue;
+  for (uint32_t it = threadIdx.x; it < a.parity * ng; it += blockDim.x) {
+    const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+    if (w0 + wq >= a.words) continue;
     uint8_t* out = const_cast<uint8_t*>(a.ptrs[a.k + lane]);
-    Word<G16> v = lds_get<G16>(A, lane * C + w);
-    if (a.mode == 0) gstore<G16>(out, w0 + w, v);
-    else mismatch |= !wequal<G16>(v, gload<G16>(out, w0 + w));
+    Word<G16> v[G];
+    lds_getv<G16, G>(A, lane * C + wq, v);
+    if (a.mode == 0) {
+      gstorev<G16, G>(out, w0 + wq, a.words, a.vec != 0, v);
+    } else {
+      Word<G16> s[G];
+      gloadv<G16, G>(out, w0 + wq, a.words, a.vec != 0, s);
+#pragma unroll
+      for (int q = 0; q < G; ++q)
+        if (w0 + wq + q < a.words) mismatch |= !wequal<G16>(v[q], s[q]);
+    }
   }
   if (mismatch) atomicOr(a.flag, 1);
 }
 
-// reconstruct_with_resources (leopard_gf8/decode.rs:319-429; leopard_gf16/decode.rs:816-905).
 template <bool G16>
-__global__ __launch_bounds__(256) void leo_decode_kernel(const LeoArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void leo_encode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
+  if (a.C % 4 == 0) leo_encode_body<G16, 4>(a, lds);
+  else leo_encode_body<G16, 1>(a, lds);
+}
+
+// reconstruct_with_resources (leopard_gf8/decode.rs:319-429; leopard_gf16/decode.rs:816-905).
+template <bool G16, int G>
+__device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds) {
   constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
-  const uint32_t C = a.C, n = a.n, m = a.m;
+  const uint32_t C = a.C, n = a.n, m = a.m, ng = C / G;
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   // lanes: 0..parity parity shards, m..m+k data shards, scaled by the error locator
-  for (uint32_t it = threadIdx.x; it < n * C; it += blockDim.x) {
-    const uint32_t lane = it / C, w = it % C;
-    Word<G16> v{};
+  for (uint32_t it = threadIdx.x; it < n * ng; it += blockDim.x) {
+    const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+    Word<G16> v[G];
     int shard = -1;
     if (lane < a.parity) shard = (int)(a.k + lane);
     else if (lane >= m && lane < m + a.k) shard = (int)(lane - m);
-    if (shard >= 0 && (a.present[shard] & 1) && w0 + w < a.words) {
-      v = gload<G16>(a.ptrs[shard], w0 + w);
-      v = wmul<G16>(a.err_tabs + (size_t)lane * TW, v);
+    if (shard >= 0 && (a.present[shard] & 1)) {
+      gloadv<G16, G>(a.ptrs[shard], w0 + wq, a.words, a.vec != 0, v);
+      const Tab<G16> te = load_tab<G16>(a.err_tabs + (size_t)lane * TW);
+#pragma unroll
+      for (int q = 0; q < G; ++q) v[q] = wmul<G16>(te, v[q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < G; ++q) v[q] = Word<G16>{};
     }
-    lds_put<G16>(lds, it, v);
+    lds_putv<G16, G>(lds, lane * C + wq, v);
   }
   __syncthreads();
-  run_layers<G16>(lds, 0, a.ifft_layer0, a.ifft_layers, a);
+  run_layers<G16, G>(lds, a.ifft_layer0, a.ifft_layers, a);
   // formal derivative (decode.rs:537-551): in closed form every lane j takes
   // lane j ^ XOR_{b : bit b of j clear, j + 2^b < n} lane (j + 2^b), all of
   // them before the update.
-  for (uint32_t it0 = 0; it0 < n * C; it0 += blockDim.x) {
+  for (uint32_t it0 = 0; it0 < n * ng; it0 += blockDim.x) {
     const uint32_t it = it0 + threadIdx.x;
-    Word<G16> acc{};
-    if (it < n * C) {
-      const uint32_t j = it / C, w = it % C;
-      acc = lds_get<G16>(lds, it);
-      for (uint32_t bit = 1; bit < n; bit <<= 1)
-        if (!(j & bit) && j + bit < n) wxor(acc, lds_get<G16>(lds, (j + bit) * C + w));
+    Word<G16> acc[G];
+    const bool live = it < n * ng;
+    uint32_t j = 0, wq = 0;
+    if (live) {
+      j = it / ng, wq = (it - j * ng) * G;
+      lds_getv<G16, G>(lds, j * C + wq, acc);
+      for (uint32_t bit = 1; bit < n; bit <<= 1) {
+        if ((j & bit) || j + bit >= n) continue;
+        Word<G16> y[G];
+        lds_getv<G16, G>(lds, (j + bit) * C + wq, y);
+#pragma unroll
+        for (int q = 0; q < G; ++q) wxor(acc[q], y[q]);
+      }
     }
     __syncthreads();
-    if (it < n * C) lds_put<G16>(lds, it, acc);
+    if (live) lds_putv<G16, G>(lds, j * C + wq, acc);
     __syncthreads();
   }
-  run_layers<G16>(lds, 0, a.fft_layer0, a.fft_layers, a);
-  for (uint32_t it = threadIdx.x; it < (a.k + a.parity) * C; it += blockDim.x) {
-    const uint32_t s = it / C, w = it % C;
-    if ((a.present[s] & 3) != 2 || w0 + w >= a.words) continue;  // missing and required
+  run_layers<G16, G>(lds, a.fft_layer0, a.fft_layers, a);
+  for (uint32_t it = threadIdx.x; it < (a.k + a.parity) * ng; it += blockDim.x) {
+    const uint32_t s = it / ng, wq = (it - s * ng) * G;
+    if ((a.present[s] & 3) != 2 || w0 + wq >= a.words) continue;  // missing and required
     const uint32_t lane = s >= a.k ? s - a.k : m + s;
-    Word<G16> v = wmul<G16>(a.inv_tabs + (size_t)lane * TW, lds_get<G16>(lds, lane * C + w));
-    gstore<G16>(const_cast<uint8_t*>(a.ptrs[s]), w0 + w, v);
+    const Tab<G16> ti = load_tab<G16>(a.inv_tabs + (size_t)lane * TW);
+    Word<G16> v[G];
+    lds_getv<G16, G>(lds, lane * C + wq, v);
+#pragma unroll
+    for (int q = 0; q < G; ++q) v[q] = wmul<G16>(ti, v[q]);
+    gstorev<G16, G>(const_cast<uint8_t*>(a.ptrs[s]), w0 + wq, a.words, a.vec != 0, v);
   }
+}
+
+template <bool G16>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void leo_decode_kernel(const LeoArgs a) {
+  extern __shared__ uint32_t lds[];
+  if (a.C % 4 == 0) leo_decode_body<G16, 4>(a, lds);
+  else leo_decode_body<G16, 1>(a, lds);
 }
 
 // Error locator in closed form: err[i] = sum_{j erased} log[i ^ j] mod (2^n - 1)
@@ -641,7 +800,14 @@ static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uin
   if (want && C > want) C = want;
   if (C < 1) C = 1;
   if (C > words) C = (uint32_t)words;
+  if (C >= 4) C &= ~(uint64_t)3;  // whole 4-word vectors (16-byte LDS / global accesses)
   return (uint32_t)C;
+}
+
+static bool all_aligned16(const std::vector<const uint8_t*>& ptrs) {
+  for (auto p : ptrs)
+    if ((uintptr_t)p & 15) return false;
+  return true;
 }
 
 static uint32_t variant_tile(int variant) {
@@ -854,6 +1020,7 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   a.fft_layers = st.fft_layers;
   a.flag = flag;
   a.mode = (uint32_t)mode;
+  a.vec = all_aligned16(ptrs) ? 1 : 0;
   const size_t lds = (size_t)2 * st.m * C * (st.gf16 ? 8 : 4);
   const unsigned grid = (unsigned)((words + C - 1) / C);
   if (st.gf16)
@@ -1002,6 +1169,7 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   a.ifft_layers = e->n_ifft;
   a.fft_layer0 = e->n_ifft;
   a.fft_layers = e->n_fft;
+  a.vec = all_aligned16(ptrs) ? 1 : 0;
   const size_t lds = (size_t)st.n * C * (st.gf16 ? 8 : 4);
   const unsigned grid = (unsigned)((words + C - 1) / C);
   if (st.gf16)
