@@ -16,6 +16,7 @@ bytes, device-resident.  One step encodes all 512 stripes.
   leopard             : Leopard GF8 128+64 / GF16 1024+256 / 4096+1024, 4-256 KiB shards
   serving             : open-loop Poisson request streams through the request server
                         (continuous batching, 200 GB admission budget): p50/p99 latency, GB/s
+  host_boundary       : PCIe-inclusive rates of the host-pointer and block-stream entry points
   roofline            : the batch kernel (dominant kernel) vs the 8 TB/s HBM peak
   cpu_baseline        : C restatement of the reference's SIMD CPU encode (GFNI / AVX2 row
                         kernels, chunked over a thread pool) on distinct host-resident stripes
@@ -258,6 +259,38 @@ def leopard_section(torch, rsgpu, DSP, disp):
     return out
 
 
+def host_section(rsgpu):
+    """PCIe-inclusive rates (never `value`): the host-pointer entry point (copy in,
+    encode, copy out per call) and the block-stream encode (4 MiB blocks through
+    pinned double-buffered staging), both on 10+4 with host-resident shards."""
+    import io
+
+    import numpy as np
+
+    k, m = 10, 4
+    rs = rsgpu.ReedSolomon(k, m)
+    rng = np.random.default_rng(3)
+    S = 4 << 20
+    shards = [np.frombuffer(rng.bytes(S), np.uint8).copy() for _ in range(k)] + \
+             [np.zeros(S, np.uint8) for _ in range(m)]
+    rs.encode_host(shards)
+    t0 = time.perf_counter()
+    reps = 8
+    for _ in range(reps):
+        rs.encode_host(shards)
+    t_host = (time.perf_counter() - t0) / reps
+    S2 = 64 << 20
+    data = [rng.bytes(S2) for _ in range(k)]
+    t0 = time.perf_counter()
+    writers = [io.BytesIO() for _ in range(m)]
+    rs.encode_stream([io.BytesIO(d) for d in data], writers, block_size=4 << 20)
+    t_stream = time.perf_counter() - t0
+    return {"encode_host_gbs": round(k * S / t_host / 1e9, 2),
+            "encode_host_shard_bytes": S,
+            "encode_stream_gbs": round(k * S2 / t_stream / 1e9, 2),
+            "encode_stream_bytes_per_shard": S2, "stream_block_bytes": 4 << 20}
+
+
 def serving_section(torch, rates, budget=200e9):
     """Open-loop Poisson streams through the request server (continuous batching,
     HBM admission); objects log-uniform 16 KiB-64 MiB on 10+4 / 12+4 / 16+4."""
@@ -443,6 +476,7 @@ def main():
         dispatch_live = DSP.evaluate_live(torch, disp, DSP.sample_specs(args.dispatch_live,
                                                                         seed=20261204))
     serving = serving_section(torch, [int(r) for r in args.rates.split(",")]) if args.rates else None
+    host = host_section(rsgpu)
 
     cpu = None if args.no_cpu else cpu_baseline(k, m, S, args.cpu_stripes, args.cpu_threads)
     rec_moved = N * k * S + n_erased * S
@@ -482,6 +516,7 @@ def main():
         "dispatch_live": dispatch_live,
         "leopard": leopard,
         "serving": serving,
+        "host_boundary": host,
         "stripe_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                               "p99": round(float(np.percentile(lat, 99)), 4)},
         "batch_tune": tune,

