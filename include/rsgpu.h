@@ -115,6 +115,21 @@ int rsgpu_encode(const rsgpu_codec* codec, const rsgpu_slice* shards, size_t n_s
 int rsgpu_encode_sep(const rsgpu_codec* codec, const rsgpu_cslice* data, size_t n_data,
                      const rsgpu_slice* parity, size_t n_parity, void* stream);
 
+/* ---- incremental encode: src/core/encode.rs:892-1132 (classic codecs only;
+ * Leopard codecs return RSGPU_UNSUPPORTED_CODEC_FAMILY) ---- */
+/* ReedSolomon::encode_single: parity = c * shards[i_data] when i_data == 0, parity
+ * ^= c * shards[i_data] otherwise (ShardByShard feeds shards 0..k-1 in order). */
+int rsgpu_encode_single(const rsgpu_codec* codec, size_t i_data, const rsgpu_slice* shards,
+                        size_t n_shards, void* stream);
+/* ReedSolomon::encode_single_sep */
+int rsgpu_encode_single_sep(const rsgpu_codec* codec, size_t i_data, rsgpu_cslice single,
+                            const rsgpu_slice* parity, size_t n_parity, void* stream);
+/* ReedSolomon::update: new_data[i].ptr == NULL marks an unchanged shard (None);
+ * parity ^= c * (old ^ new) for every changed shard. */
+int rsgpu_update(const rsgpu_codec* codec, const rsgpu_cslice* old_data, size_t n_old,
+                 const rsgpu_cslice* new_data, size_t n_new, const rsgpu_slice* parity,
+                 size_t n_parity, void* stream);
+
 /* ---- verify: src/core/verify.rs:14-237 ---- */
 /* ReedSolomon::verify / verify_with_workspace: *ok = 1 when every parity shard
  * equals the recomputed parity.  Synchronous (waits for the stream).  Classic

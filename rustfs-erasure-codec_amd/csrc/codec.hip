@@ -217,9 +217,14 @@ static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 // Apply rows (tables [r][k_total][W], row stride k_total*W) to inputs -> outputs.
 // len is in bytes.  Splits k into kMaxIn chunks (later chunks accumulate) and r
 // into kMaxOut chunks.  mode kVerify compares with `out` and sets *flag.
+// `table_cols` (default k_total) is the column count of the table rows and
+// `col0` the table column of in[0], so a subset of a wider table can be applied.
+// mode kAccumulate XORs into the outputs from the first input on.
 static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
                      const std::vector<const uint8_t*>& in, const std::vector<uint8_t*>& out,
-                     size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1) {
+                     size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1,
+                     int table_cols = -1, int col0 = 0) {
+  if (table_cols < 0) table_cols = k_total;
   const int W = words_per_coef(tower);
   bool vec_ok = true;
   for (auto p : in) vec_ok &= aligned16(p);
@@ -235,18 +240,18 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
     const int rn = std::min(kMaxOut, r_total - r0);
     for (int k0 = 0; k0 < k_total; k0 += kMaxIn) {
       const int kn = std::min(kMaxIn, k_total - k0);
-      const int m = (mode == kVerify) ? kVerify : (k0 == 0 ? kWrite : kAccumulate);
+      const int m = (mode == kVerify) ? kVerify : (k0 == 0 && mode == kWrite ? kWrite : kAccumulate);
       if (n16) {
         ApplyArgs a{};
         for (int i = 0; i < kn; ++i) a.in[i] = in[k0 + i];
         for (int j = 0; j < rn; ++j) a.out[j] = out[r0 + j];
-        a.tables = d_tables + (size_t)r0 * k_total * W;
+        a.tables = d_tables + (size_t)r0 * table_cols * W;
         a.flag = flag;
         a.n16 = n16;
         a.v_begin = 0;
         a.k = kn;
-        a.tab_row_stride = k_total * W;
-        a.tab_col0 = k0;
+        a.tab_row_stride = table_cols * W;
+        a.tab_col0 = col0 + k0;
         a.r = rn;
         HIP_TRY(launch_apply(a, m, tile, split ? std::min(split, rn) : rn, tower ? 1 : 0, s));
       }
@@ -254,13 +259,13 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
         ByteArgs b{};
         for (int i = 0; i < kn; ++i) b.in[i] = in[k0 + i];
         for (int j = 0; j < rn; ++j) b.out[j] = out[r0 + j];
-        b.tables = d_tables + (size_t)r0 * k_total * W;
+        b.tables = d_tables + (size_t)r0 * table_cols * W;
         b.flag = flag;
         b.begin = tail_begin;
         b.end = len;
         b.k = kn;
-        b.tab_row_stride = k_total * W;
-        b.tab_col0 = k0;
+        b.tab_row_stride = table_cols * W;
+        b.tab_col0 = col0 + k0;
         b.r = rn;
         b.mode = m;
         b.tower = tower ? 1 : 0;
@@ -980,6 +985,104 @@ int rsgpu_verify_with_buffer(const rsgpu_codec* h, const rsgpu_cslice* shards, s
   HIP_TRY(hipStreamSynchronize(s));
   *ok = host == 0 ? 1 : 0;
   return RSGPU_OK;
+}
+
+// encode_single_sep (src/core/encode.rs:913-931): parity = c * data for i_data
+// == 0, parity ^= c * data otherwise (code_single_slice, encode.rs:676-708).
+int rsgpu_encode_single_sep(const rsgpu_codec* h, size_t i_data, rsgpu_cslice single,
+                            const rsgpu_slice* parity, size_t np, void* stream) {
+  if (!h || (!parity && np)) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  if (c.family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  if (i_data >= c.k) return RSGPU_INVALID_INDEX;
+  if (np < c.m) return RSGPU_TOO_FEW_PARITY_SHARDS;
+  if (np > c.m) return RSGPU_TOO_MANY_PARITY_SHARDS;
+  int rc = check_slices(parity, np);
+  if (rc) return rc;
+  if (single.len != parity[0].len) return RSGPU_INCORRECT_SHARD_SIZE;
+  if ((rc = check_elem_len(c, single.len))) return rc;
+  if (!single.ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  std::vector<uint8_t*> out(np);
+  for (size_t j = 0; j < np; ++j) {
+    if (!parity[j].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+    out[j] = parity[j].ptr;
+  }
+  const size_t len = single.len;
+  return run_apply((const uint32_t*)c.d_parity_tables.p, 1, (int)c.m, c.field == RSGPU_FIELD_GF16,
+                   {single.ptr}, out, len, i_data == 0 ? kWrite : kAccumulate, nullptr,
+                   (hipStream_t)stream, pick_variant(c, kOpEncode, 0, len), (int)c.k, (int)i_data);
+}
+
+// encode_single (src/core/encode.rs:892-911): shards[i_data] into shards[k..].
+int rsgpu_encode_single(const rsgpu_codec* h, size_t i_data, const rsgpu_slice* shards, size_t n,
+                        void* stream) {
+  if (!h || (!shards && n)) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  if (c.family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  if (i_data >= c.k) return RSGPU_INVALID_INDEX;
+  if (n < c.k + c.m) return RSGPU_TOO_FEW_SHARDS;
+  if (n > c.k + c.m) return RSGPU_TOO_MANY_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  return rsgpu_encode_single_sep(h, i_data, {shards[i_data].ptr, shards[i_data].len}, shards + c.k, c.m,
+                                 stream);
+}
+
+// update (src/core/encode.rs:1048-1132): parity ^= c * (old ^ new) for every
+// changed data shard, as one accumulate pass over the inputs [old_i, new_i].
+int rsgpu_update(const rsgpu_codec* h, const rsgpu_cslice* old_data, size_t n_old,
+                 const rsgpu_cslice* new_data, size_t n_new, const rsgpu_slice* parity, size_t np,
+                 void* stream) {
+  if (!h || (!old_data && n_old) || (!new_data && n_new) || (!parity && np))
+    return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  if (c.family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  if (n_old < c.k) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (n_old > c.k) return RSGPU_TOO_MANY_DATA_SHARDS;
+  if (np < c.m) return RSGPU_TOO_FEW_PARITY_SHARDS;
+  if (np > c.m) return RSGPU_TOO_MANY_PARITY_SHARDS;
+  if (n_new < c.k) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (n_new > c.k) return RSGPU_TOO_MANY_DATA_SHARDS;
+  int rc = check_slices(old_data, n_old);
+  if (rc || (rc = check_slices(parity, np))) return rc;
+  const size_t len = old_data[0].len;
+  if (parity[0].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+  std::vector<size_t> changed;
+  for (size_t i = 0; i < n_new; ++i) {
+    if (!new_data[i].ptr) continue;  // None: unchanged
+    if (new_data[i].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+    changed.push_back(i);
+  }
+  if (changed.empty()) return RSGPU_OK;
+  if ((rc = check_elem_len(c, len))) return rc;
+  for (size_t i = 0; i < n_old; ++i)
+    if (!old_data[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+  std::vector<uint8_t*> out(np);
+  for (size_t j = 0; j < np; ++j) {
+    if (!parity[j].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+    out[j] = parity[j].ptr;
+  }
+  // gather table columns [i, i] for every changed i from the host copy
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  const size_t W = (size_t)words_per_coef(tower), nc = 2 * changed.size();
+  std::vector<uint32_t> tab(c.m * nc * W);
+  std::vector<const uint8_t*> in;
+  for (size_t q = 0; q < changed.size(); ++q) {
+    const size_t i = changed[q];
+    in.push_back(old_data[i].ptr);
+    in.push_back(new_data[i].ptr);
+    for (size_t j = 0; j < c.m; ++j)
+      for (int t = 0; t < 2; ++t)
+        std::memcpy(&tab[(j * nc + 2 * q + t) * W], &c.h_parity_tables[(j * c.k + i) * W], W * 4);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  DeviceBuffer dt;
+  HIP_TRY(dt.alloc(tab.size() * 4));
+  HIP_TRY(hipMemcpyAsync(dt.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
+  rc = run_apply((const uint32_t*)dt.p, (int)nc, (int)c.m, tower, in, out, len, kAccumulate, nullptr, s,
+                 pick_variant(c, kOpEncode, 0, len));
+  HIP_TRY(hipStreamSynchronize(s));  // the gathered tables are freed on return
+  return rc;
 }
 
 int rsgpu_variant_count(void) { return kNumVariants; }

@@ -61,6 +61,15 @@ class StreamError(Exception):
         super().__init__(f"{kind} error on shard {shard_index}: {error}")
 
 
+class SBSError(Exception):
+    """SBSError (src/errors.rs): "TooManyCalls", "LeftoverShards", or "RSError" with
+    the codec's RsError in `error`."""
+
+    def __init__(self, kind, error=None):
+        self.kind, self.error = kind, error
+        super().__init__(kind if error is None else f"{kind}({error})")
+
+
 class Options(C.Structure):
     _fields_ = [
         ("field", C.c_int32), ("codec_family", C.c_int32), ("matrix_mode", C.c_int32),
@@ -104,6 +113,9 @@ _lib.rsgpu_verify_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz, _i32p]
 _lib.rsgpu_reconstruct_host.argtypes = [_vp, C.POINTER(_vp), _u8p, _sz, _sz, C.c_int32]
 _lib.rsgpu_apply_rows.argtypes = [C.c_int32, _u8p, _sz, _sz, C.POINTER(CSlice), C.POINTER(Slice), _vp]
 _lib.rsgpu_cache_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
+_lib.rsgpu_encode_single.argtypes = [_vp, _sz, C.POINTER(Slice), _sz, _vp]
+_lib.rsgpu_encode_single_sep.argtypes = [_vp, _sz, CSlice, C.POINTER(Slice), _sz, _vp]
+_lib.rsgpu_update.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _vp]
 _lib.rsgpu_status_name.restype = C.c_char_p
 _lib.rsgpu_status_name.argtypes = [C.c_int]
 _lib.rsgpu_last_hip_error.restype = C.c_char_p
@@ -343,6 +355,24 @@ class ReedSolomon:
         vals = [C.c_uint64() for _ in range(4)]
         _check(_lib.rsgpu_cache_stats(self._h, *[C.byref(v) for v in vals]))
         return dict(zip(["requests", "hits", "misses", "entries"], [v.value for v in vals]))
+
+    # ---- incremental encode (src/core/encode.rs:892-1132)
+    def encode_single(self, i_data, shards, stream=None):
+        arr = (Slice * len(shards))(*[Slice(*_ptr_len(t)) for t in shards])
+        _check(_lib.rsgpu_encode_single(self._h, i_data, arr, len(shards), _stream(stream)))
+
+    def encode_single_sep(self, i_data, single, parity, stream=None):
+        par = (Slice * len(parity))(*[Slice(*_ptr_len(t)) for t in parity])
+        _check(_lib.rsgpu_encode_single_sep(self._h, i_data, CSlice(*_ptr_len(single)), par,
+                                            len(parity), _stream(stream)))
+
+    def update(self, old_data, new_data, parity, stream=None):
+        """new_data[i] is None for an unchanged shard."""
+        old = (CSlice * len(old_data))(*[CSlice(*_ptr_len(t)) for t in old_data])
+        new = (CSlice * len(new_data))(*[CSlice(*_ptr_len(t)) for t in new_data])
+        par = (Slice * len(parity))(*[Slice(*_ptr_len(t)) for t in parity])
+        _check(_lib.rsgpu_update(self._h, old, len(old_data), new, len(new_data), par, len(parity),
+                                 _stream(stream)))
 
     # ---- block streams (src/core/stream.rs:677-1031)
     def encode_stream(self, data, parity, block_size=4 << 20):
@@ -688,3 +718,79 @@ def apply_rows(rows, inputs, outputs, field="gf8", stream=None):
     ins = (CSlice * k)(*[CSlice(*_ptr_len(t)) for t in inputs])
     outs = (Slice * r)(*[Slice(*_ptr_len(t)) for t in outputs])
     _check(_lib.rsgpu_apply_rows(f, rows_b.ctypes.data_as(_u8p), r, k, ins, outs, _stream(stream)))
+
+
+class ShardByShard:
+    """Encodes the data shards of one stripe one call at a time
+    (src/core/shard_by_shard.rs:59-180) through ReedSolomon.encode_single."""
+
+    def __init__(self, codec):
+        self.codec = codec
+        self.cur_input = 0
+
+    def parity_ready(self):
+        return self.cur_input == self.codec.data_shard_count()
+
+    def reset(self):
+        if self.cur_input > 0 and not self.parity_ready():
+            raise SBSError("LeftoverShards")
+        self.cur_input = 0
+
+    def reset_force(self):
+        self.cur_input = 0
+
+    def cur_input_index(self):
+        return self.cur_input
+
+    def _checks(self, fn):
+        if self.parity_ready():
+            raise SBSError("TooManyCalls")
+        try:
+            fn()
+        except RsError as e:
+            raise SBSError("RSError", e) from None
+
+    def encode(self, shards, stream=None):
+        shards = list(shards)
+        c = self.codec
+
+        def checks():
+            n = c.total_shard_count()
+            if len(shards) != n:
+                raise RsError(1 if len(shards) < n else 2)
+            lens = [0 if t is None else t.numel() for t in shards]
+            if lens[0] == 0:
+                raise RsError(11)
+            if any(x != lens[0] for x in lens):
+                raise RsError(9)
+
+        self._checks(checks)
+        try:
+            c.encode_single(self.cur_input, shards, stream)
+        except RsError as e:
+            raise SBSError("RSError", e) from None
+        self.cur_input += 1
+
+    def encode_sep(self, data, parity, stream=None):
+        data, parity = list(data), list(parity)
+        c = self.codec
+
+        def checks():
+            if len(data) != c.data_shard_count():
+                raise RsError(3 if len(data) < c.data_shard_count() else 4)
+            if len(parity) != c.parity_shard_count():
+                raise RsError(5 if len(parity) < c.parity_shard_count() else 6)
+            for group in (data, parity):
+                if group[0].numel() == 0:
+                    raise RsError(11)
+                if any(t.numel() != group[0].numel() for t in group):
+                    raise RsError(9)
+            if data[0].numel() != parity[0].numel():
+                raise RsError(9)
+
+        self._checks(checks)
+        try:
+            c.encode_single_sep(self.cur_input, data[self.cur_input], parity, stream)
+        except RsError as e:
+            raise SBSError("RSError", e) from None
+        self.cur_input += 1
