@@ -107,6 +107,22 @@ typedef struct rsgpu_slice { uint8_t* ptr; size_t len; } rsgpu_slice;
  * length; their bytes are overwritten and `present` is set to 1. */
 typedef struct rsgpu_slot { uint8_t* ptr; size_t len; int32_t present; } rsgpu_slot;
 
+/* EncodingFormat::stable_id_bytes (src/core/format.rs:133-219, version 1): the
+ * persistent identifier of the effective format (family, field, counts, matrix
+ * coefficients).  *len receives the descriptor length; with out == NULL only the
+ * length is returned. */
+int rsgpu_codec_format_id(const rsgpu_codec* codec, uint8_t* out, size_t cap, size_t* len);
+/* Shard length in bytes that ReedSolomon::split gives data_len bytes
+ * (codec.rs:476-505; Leopard: leopard_aligned_shard_len, leopard.rs:75-98). */
+int rsgpu_split_len(const rsgpu_codec* codec, size_t data_len, size_t* shard_len);
+/* ReedSolomon::split into k device shards of rsgpu_split_len bytes (zero-padded). */
+int rsgpu_split(const rsgpu_codec* codec, const uint8_t* data, size_t data_len,
+                const rsgpu_slice* shards, size_t n_shards, void* stream);
+/* ReedSolomon::join (codec.rs:507-538): the first min(out_len, k * len) bytes of the
+ * k data shards into `out`; *written receives that count. */
+int rsgpu_join(const rsgpu_codec* codec, const rsgpu_cslice* shards, size_t n_shards, size_t out_len,
+               uint8_t* out, size_t* written, void* stream);
+
 /* ---- encode: src/core/encode.rs:935-998 ---- */
 /* ReedSolomon::encode: shards[0..k] data (read), shards[k..k+m] parity (written). */
 int rsgpu_encode(const rsgpu_codec* codec, const rsgpu_slice* shards, size_t n_shards,

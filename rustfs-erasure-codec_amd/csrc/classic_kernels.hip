@@ -88,6 +88,18 @@ hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ void fill_u8_kernel(uint8_t* p, uint8_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(fill_u8_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t s) {
   hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, v, n);
   return hipGetLastError();

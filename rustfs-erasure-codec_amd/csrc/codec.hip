@@ -1085,6 +1085,123 @@ int rsgpu_update(const rsgpu_codec* h, const rsgpu_cslice* old_data, size_t n_ol
   return rc;
 }
 
+// ------------------------------------------------- format identity, split, join
+// EncodingFormat::stable_id_bytes (src/core/format.rs:133-219), version 1:
+// magic "RSEFMT\0\0", u16 version, u8 family, u32 + field id, u32 element width,
+// u64 field order, u64 data, u64 parity, u64 rows, u64 cols, then the matrix
+// coefficients row-major (Leopard formats carry no matrix: 0 x 0).  Integers are
+// big-endian.
+int rsgpu_codec_format_id(const rsgpu_codec* h, uint8_t* out, size_t cap, size_t* len) {
+  if (!h || !len || (!out && cap)) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  const bool tower = c.field == RSGPU_FIELD_GF16 && c.family == RSGPU_FAMILY_CLASSIC;
+  // galois_8::Field / galois_16::Field stable_format (galois_8/mod.rs:73-83, galois_16.rs:27-37);
+  // Leopard codecs are ReedSolomon<galois_8::Field> over byte input.
+  const char* field_id = tower ? "rustfs.gf16.tower-11d-x2-02x-80.ax-ac.v1" : "rustfs.gf8.poly-11d.u8.v1";
+  const uint32_t width = tower ? 2 : 1;
+  const uint64_t order = tower ? 65536 : 256;
+  const bool leo = c.family != RSGPU_FAMILY_CLASSIC;
+  const uint64_t rows = leo ? 0 : c.k + c.m, cols = leo ? 0 : c.k;
+  std::vector<uint8_t> b;
+  auto put = [&](uint64_t v, int n) {
+    for (int i = n - 1; i >= 0; --i) b.push_back((uint8_t)(v >> (8 * i)));
+  };
+  const char magic[8] = {'R', 'S', 'E', 'F', 'M', 'T', 0, 0};
+  b.insert(b.end(), magic, magic + 8);
+  put(1, 2);
+  b.push_back((uint8_t)(c.family == RSGPU_FAMILY_CLASSIC ? 0 : c.family == RSGPU_FAMILY_LEOPARD_GF8 ? 1 : 2));
+  const size_t fl = std::strlen(field_id);
+  put(fl, 4);
+  b.insert(b.end(), field_id, field_id + fl);
+  put(width, 4);
+  for (uint64_t v : {order, (uint64_t)c.k, (uint64_t)c.m, rows, cols}) put(v, 8);
+  for (uint64_t i = 0; i < rows * cols; ++i) {
+    const uint16_t e = c.matrix[i];
+    if (tower) {  // [x coefficient, constant] (galois_16.rs:31-37)
+      b.push_back((uint8_t)(e >> 8));
+      b.push_back((uint8_t)e);
+    } else {
+      b.push_back((uint8_t)e);
+    }
+  }
+  *len = b.size();
+  if (cap < b.size()) return out ? RSGPU_ERR_INVALID_ARGUMENT : RSGPU_OK;
+  std::memcpy(out, b.data(), b.size());
+  return RSGPU_OK;
+}
+
+// leopard_aligned_shard_len (src/core/leopard.rs:75-98)
+static size_t leopard_aligned_len(size_t data_len, size_t shards) {
+  if (shards == 0) shards = 1;
+  const size_t per = data_len / shards + (data_len % shards ? 1 : 0);
+  size_t r = per % 64 ? per + (64 - per % 64) : per;
+  if (r < per) r = SIZE_MAX;  // saturate
+  r = std::max<size_t>(r, 64);
+  return r - r % 64;
+}
+
+// Shard length in bytes that split() gives `data_len` bytes (codec.rs:476-505):
+// classic ceil(elements / k) elements, Leopard a nonzero multiple of 64 bytes.
+int rsgpu_split_len(const rsgpu_codec* h, size_t data_len, size_t* shard_len) {
+  if (!h || !shard_len) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  const size_t eb = c.field == RSGPU_FIELD_GF16 && c.family == RSGPU_FAMILY_CLASSIC ? 2 : 1;
+  if (data_len % eb) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (c.family != RSGPU_FAMILY_CLASSIC) {
+    *shard_len = leopard_aligned_len(data_len, c.k);
+  } else {
+    const size_t el = data_len / eb;
+    *shard_len = (el / c.k + (el % c.k ? 1 : 0)) * eb;
+  }
+  return RSGPU_OK;
+}
+
+// ReedSolomon::split (codec.rs:476-505) into caller shards of split_len bytes:
+// shard i takes bytes [i * len, (i + 1) * len) of data, zero-padded.
+int rsgpu_split(const rsgpu_codec* h, const uint8_t* data, size_t data_len, const rsgpu_slice* shards,
+                size_t n, void* stream) {
+  if (!h || (!data && data_len) || (!shards && n)) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  size_t len = 0;
+  int rc = rsgpu_split_len(h, data_len, &len);
+  if (rc) return rc;
+  if (n != c.k) return RSGPU_ERR_INVALID_ARGUMENT;
+  for (size_t i = 0; i < n; ++i)
+    if (shards[i].len != len || (!shards[i].ptr && len)) return RSGPU_ERR_INVALID_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  for (size_t i = 0; i < n; ++i) {
+    const size_t start = i * len, end = std::min(start + len, data_len);
+    const size_t have = start < data_len ? end - start : 0;
+    if (have) HIP_TRY(hipMemcpyAsync(shards[i].ptr, data + start, have, hipMemcpyDeviceToDevice, s));
+    if (have < len) HIP_TRY(launch_fill_u8(shards[i].ptr + have, 0, len - have, s));
+  }
+  return RSGPU_OK;
+}
+
+// ReedSolomon::join (codec.rs:507-538): the first min(out_len, k * len) bytes of
+// the data shards, in order; *written receives that count.
+int rsgpu_join(const rsgpu_codec* h, const rsgpu_cslice* shards, size_t n, size_t out_len, uint8_t* out,
+               size_t* written, void* stream) {
+  if (!h || (!shards && n) || !written) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  if (n < c.k) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (n > c.k) return RSGPU_TOO_MANY_DATA_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  const size_t len = shards[0].len;
+  const size_t target = std::min(out_len, len * n);
+  if (!out && target) return RSGPU_ERR_INVALID_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  size_t done = 0;
+  for (size_t i = 0; i < n && done < target; ++i) {
+    const size_t take = std::min(len, target - done);
+    HIP_TRY(hipMemcpyAsync(out + done, shards[i].ptr, take, hipMemcpyDeviceToDevice, s));
+    done += take;
+  }
+  *written = target;
+  return RSGPU_OK;
+}
+
 int rsgpu_variant_count(void) { return kNumVariants; }
 
 int rsgpu_dispatch_dims(int32_t* n_ops, int32_t* n_buckets, int32_t* bucket0) {

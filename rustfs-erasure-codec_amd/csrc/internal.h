@@ -82,6 +82,7 @@ enum DispatchOp : int { kOpEncode = 0, kOpVerify = 1, kOpReconstruct = 2, kNumOp
 hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s);
 hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint64_t n, hipStream_t s);
 
 // Decode rows on the device (classic_kernels.hip).  One workgroup solves the
 // missing-data equations of the selected survivors and composes the requested

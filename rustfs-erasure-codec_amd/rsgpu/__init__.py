@@ -113,6 +113,10 @@ _lib.rsgpu_verify_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz, _i32p]
 _lib.rsgpu_reconstruct_host.argtypes = [_vp, C.POINTER(_vp), _u8p, _sz, _sz, C.c_int32]
 _lib.rsgpu_apply_rows.argtypes = [C.c_int32, _u8p, _sz, _sz, C.POINTER(CSlice), C.POINTER(Slice), _vp]
 _lib.rsgpu_cache_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
+_lib.rsgpu_codec_format_id.argtypes = [_vp, _u8p, _sz, C.POINTER(_sz)]
+_lib.rsgpu_split_len.argtypes = [_vp, _sz, C.POINTER(_sz)]
+_lib.rsgpu_split.argtypes = [_vp, _vp, _sz, C.POINTER(Slice), _sz, _vp]
+_lib.rsgpu_join.argtypes = [_vp, C.POINTER(CSlice), _sz, _sz, _vp, C.POINTER(_sz), _vp]
 _lib.rsgpu_encode_single.argtypes = [_vp, _sz, C.POINTER(Slice), _sz, _vp]
 _lib.rsgpu_encode_single_sep.argtypes = [_vp, _sz, CSlice, C.POINTER(Slice), _sz, _vp]
 _lib.rsgpu_update.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _vp]
@@ -355,6 +359,41 @@ class ReedSolomon:
         vals = [C.c_uint64() for _ in range(4)]
         _check(_lib.rsgpu_cache_stats(self._h, *[C.byref(v) for v in vals]))
         return dict(zip(["requests", "hits", "misses", "entries"], [v.value for v in vals]))
+
+    # ---- format identity, split, join (src/core/format.rs, src/core/codec.rs:476-538)
+    def stable_id_bytes(self):
+        n = _sz()
+        _check(_lib.rsgpu_codec_format_id(self._h, None, 0, C.byref(n)))
+        buf = (C.c_uint8 * n.value)()
+        _check(_lib.rsgpu_codec_format_id(self._h, buf, n.value, C.byref(n)))
+        return bytes(buf)
+
+    def split_len(self, data_len):
+        n = _sz()
+        _check(_lib.rsgpu_split_len(self._h, data_len, C.byref(n)))
+        return n.value
+
+    def split(self, data, stream=None):
+        """k zero-padded device shards holding the bytes of the uint8 device tensor `data`."""
+        import torch
+
+        n = data.numel()
+        L = self.split_len(n)
+        shards = [torch.empty(L, dtype=torch.uint8, device=data.device) for _ in range(self._k)]
+        arr = (Slice * self._k)(*[Slice(t.data_ptr(), L) for t in shards])
+        _check(_lib.rsgpu_split(self._h, data.data_ptr() if n else None, n, arr, self._k, _stream(stream)))
+        return shards
+
+    def join(self, shards, out_len, stream=None):
+        import torch
+
+        arr = (CSlice * len(shards))(*[CSlice(*_ptr_len(t)) for t in shards])
+        cap = min(out_len, sum(t.numel() for t in shards))
+        out = torch.empty(max(cap, 1), dtype=torch.uint8, device=shards[0].device if shards else "cuda")
+        w = _sz()
+        _check(_lib.rsgpu_join(self._h, arr, len(shards), out_len, out.data_ptr(), C.byref(w),
+                               _stream(stream)))
+        return out[:w.value]
 
     # ---- incremental encode (src/core/encode.rs:892-1132)
     def encode_single(self, i_data, shards, stream=None):
