@@ -363,6 +363,16 @@ int rsgpu_mlp_loss_grad(rsgpu_mlp* mlp, const float* x, const float* t, const fl
 int rsgpu_mlp_train(rsgpu_mlp* mlp, const float* x, const float* t, const float* w, size_t batch,
                     int32_t steps, float lr, float* loss, void* stream);
 
+/* Installs the dispatcher network's decisions as the codec's variant table: one
+ * forward pass over every (op, erasures, size bucket) of this codec, argmin over
+ * the variants valid for each call.  `envelope` (NULL = no limit) is int64
+ * [4][6]: k_min, k_max, m_min, m_max, shard_len_min, shard_len_max of the trained
+ * shapes per kind (classic GF8, classic GF16, Leopard GF8, Leopard GF16; k_max 0 =
+ * none); calls outside it keep the default tiling.  The network must have 14
+ * inputs (the features of rsgpu/dispatch.py) and rsgpu_variant_count() outputs. */
+int rsgpu_codec_install_dispatch(rsgpu_codec* codec, rsgpu_mlp* net, const int64_t* envelope,
+                                 void* stream);
+
 /* ---- decode-matrix cache statistics (ReconstructionCacheStats, codec.rs:401-426) ---- */
 int rsgpu_cache_stats(const rsgpu_codec* codec, uint64_t* requests, uint64_t* hits,
                       uint64_t* misses, uint64_t* entries);

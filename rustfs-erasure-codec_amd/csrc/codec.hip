@@ -1251,6 +1251,90 @@ int rsgpu_codec_set_variant_table(rsgpu_codec* h, const int16_t* table, size_t n
   return RSGPU_OK;
 }
 
+// Network inputs of one call; the same 14 features as features() in
+// rsgpu/dispatch.py (float64 arithmetic, rounded to float at the end).
+static void dispatch_features(int kind, size_t k, size_t m, size_t len, int op, size_t e, float* f) {
+  const double l2 = (std::log2((double)std::max<size_t>(len, 1)) - 16.0) / 6.0;
+  const double v[14] = {kind == 1 ? 1.0 : 0.0, kind == 2 ? 1.0 : 0.0, kind == 3 ? 1.0 : 0.0,
+                        std::log2((double)k) / 12.0, m > 0 ? std::log2((double)m) / 10.0 : 0.0,
+                        std::log2((double)(k + m)) / 12.0, l2, l2 * l2,
+                        op == 0 ? 1.0 : 0.0, op == 1 ? 1.0 : 0.0, op == 2 ? 1.0 : 0.0,
+                        (double)e / (double)std::max<size_t>(m, 1), std::log2(1.0 + (double)e) / 10.0,
+                        (std::log2((double)std::max<size_t>(len, 1)) + std::log2((double)(k + m)) - 24.0) / 6.0};
+  for (int i = 0; i < 14; ++i) f[i] = (float)v[i];
+}
+
+// Fills the codec's variant table from the dispatcher network: one forward pass
+// over every (op, erasures, size bucket) and the argmin over the variants valid
+// for that call.  `envelope` ([4][6] int64: k_min, k_max, m_min, m_max, len_min,
+// len_max per kind classic-GF8, classic-GF16, Leopard GF8, Leopard GF16; a row
+// with k_max == 0 is unknown) limits it to trained shapes; other entries keep the
+// default tiling.
+int rsgpu_codec_install_dispatch(rsgpu_codec* h, rsgpu_mlp* net, const int64_t* envelope, void* stream) {
+  if (!h || !net) return RSGPU_ERR_INVALID_ARGUMENT;
+  Codec& c = h->c;
+  int32_t n_in = 0, n_hidden = 0, n_out = 0;
+  size_t n_params = 0;
+  int rc = rsgpu_mlp_dims(net, &n_in, &n_hidden, &n_out, &n_params);
+  if (rc) return rc;
+  if (n_in != 14 || n_out != kNumVariants) return RSGPU_ERR_INVALID_ARGUMENT;
+  const int kind = c.family == RSGPU_FAMILY_LEOPARD_GF8    ? 2
+                   : c.family == RSGPU_FAMILY_LEOPARD_GF16 ? 3
+                   : c.field == RSGPU_FIELD_GF16           ? 1
+                                                           : 0;
+  struct Row {
+    int op;
+    size_t e;
+    int b;
+    size_t len;
+  };
+  std::vector<Row> rows;
+  for (int op = 0; op < kNumOps; ++op)
+    for (size_t e = 0; e <= c.m; ++e) {
+      if ((op < 2) != (e == 0)) continue;
+      for (int b = 0; b < kNumBuckets; ++b)
+        rows.push_back({op, e, b, (size_t)std::llround(std::pow(2.0, (b + kBucket0) / 2.0))});
+    }
+  std::vector<float> x(rows.size() * 14), y(rows.size() * (size_t)n_out);
+  for (size_t i = 0; i < rows.size(); ++i)
+    dispatch_features(kind, c.k, c.m, rows[i].len, rows[i].op, rows[i].e, &x[i * 14]);
+  hipStream_t s = (hipStream_t)stream;
+  DeviceBuffer dx, dy;
+  HIP_TRY(dx.alloc(x.size() * 4));
+  HIP_TRY(dy.alloc(y.size() * 4));
+  HIP_TRY(hipMemcpyAsync(dx.p, x.data(), x.size() * 4, hipMemcpyHostToDevice, s));
+  if ((rc = rsgpu_mlp_forward(net, (const float*)dx.p, rows.size(), (float*)dy.p, s))) return rc;
+  HIP_TRY(hipMemcpyAsync(y.data(), dy.p, y.size() * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const int64_t* env = envelope ? envelope + 6 * kind : nullptr;
+  auto inside = [&](size_t len) {
+    if (!env) return true;
+    if (env[1] == 0) return false;
+    return (int64_t)c.k >= env[0] && (int64_t)c.k <= env[1] && (int64_t)c.m >= env[2] &&
+           (int64_t)c.m <= env[3] && (int64_t)len >= env[4] / 2 && (int64_t)len <= env[5] * 2;
+  };
+  const size_t ne = c.m + 1;
+  std::vector<int16_t> tab((size_t)kNumOps * ne * kNumBuckets, -1);
+  for (size_t i = 0; i < rows.size(); ++i) {
+    const Row& r = rows[i];
+    if (!inside(r.len)) continue;
+    int best = -1;
+    float bv = 0;
+    for (int v = 0; v < kNumVariants; ++v) {
+      if (!variant_valid(c, r.op, r.e, v)) continue;
+      const float t = y[i * n_out + v];
+      if (best < 0 || t < bv) best = v, bv = t;
+    }
+    tab[((size_t)r.op * ne + r.e) * kNumBuckets + r.b] = (int16_t)best;
+  }
+  // encode / verify rows are keyed by e = 0; copy them so every erasure index resolves
+  for (int op = 0; op < 2; ++op)
+    for (size_t e = 1; e < ne; ++e)
+      std::memcpy(&tab[((size_t)op * ne + e) * kNumBuckets], &tab[(size_t)op * ne * kNumBuckets],
+                  kNumBuckets * sizeof(int16_t));
+  return rsgpu_codec_set_variant_table(h, tab.data(), ne);
+}
+
 int rsgpu_codec_variant_for(const rsgpu_codec* h, int32_t op, size_t erasures, size_t shard_len,
                             int32_t* variant) {
   if (!h || !variant || op < 0 || op >= kNumOps) return RSGPU_ERR_INVALID_ARGUMENT;

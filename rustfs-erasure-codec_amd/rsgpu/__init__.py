@@ -607,6 +607,7 @@ _lib.rsgpu_mlp_set_params.argtypes = [_vp, _f32p, _sz]
 _lib.rsgpu_mlp_forward.argtypes = [_vp, _vp, _sz, _vp, _vp]
 _lib.rsgpu_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _sz, _f32p, _f32p, _sz, _vp]
 _lib.rsgpu_mlp_train.argtypes = [_vp, _vp, _vp, _vp, _sz, C.c_int32, C.c_float, _f32p, _vp]
+_lib.rsgpu_codec_install_dispatch.argtypes = [_vp, _vp, C.POINTER(C.c_int64), _vp]
 
 
 class Mlp:

@@ -23,6 +23,7 @@ src/core/parallel.rs:65-127).  Here the choice is learned:
     python -m rsgpu.dispatch train --data timings.npz --out dispatch_mi355x.npz
 """
 import argparse
+import ctypes as C
 import math
 import os
 import sys
@@ -349,6 +350,21 @@ class Dispatcher:
         table[0, 1:, :] = table[0, 0, :]
         table[1, 1:, :] = table[1, 0, :]
         return table
+
+    def envelope_array(self):
+        """The envelope as the C ABI's int64 [4][6] array (kinds without data: zeros)."""
+        env = np.zeros((len(KINDS), 6), np.int64)
+        for kind, v in self.envelope.items():
+            env[KINDS.index(kind)] = v
+        return env
+
+    def install_native(self, rs, stream=None):
+        """Same table as install(), computed by the library (rsgpu_codec_install_dispatch):
+        features, forward pass and argmin in C++ / HIP, no Python per entry."""
+        env = np.ascontiguousarray(self.envelope_array())
+        _check(lib().rsgpu_codec_install_dispatch(rs._h, self.mlp._h,
+                                                  env.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                  _stream(stream)))
 
     def install(self, rs, kind, autotune_calls=()):
         """Installs the network's table on `rs`.  Calls in `autotune_calls`

@@ -162,3 +162,23 @@ def test_installed_tables_keep_outputs_exact():
             rs.reconstruct(work)
             for i in erased:
                 assert (to_host(work[i]) == full[i]).all(), (kind, erased, i)
+
+
+def test_native_install_matches_python_table():
+    """rsgpu_codec_install_dispatch (features, forward and argmin in the library)
+    installs exactly the table the Python dispatcher computes."""
+    announce("dispatcher native install")
+    disp = D.Dispatcher.load()
+    for kind, k, m in [("gf8", 10, 4), ("gf16", 6, 3), ("leopard_gf8", 32, 8), ("leopard_gf16", 64, 16),
+                       ("gf8", 200, 30)]:
+        rs = D.make_codec(kind, k, m)
+        table = disp.table_for(rs, kind)
+        disp.install_native(rs)
+        n_ops, n_buckets, b0 = rsgpu.dispatch_dims()
+        for op in range(n_ops):
+            for e in range(m + 1):
+                for b in range(n_buckets):
+                    ln = int(round(2.0 ** ((b + b0) / 2.0)))
+                    assert rs.variant_for(op, e, ln) == (table[op, e, b] if table[op, e, b] < 0 or
+                                                         rs.variant_valid(op, e, table[op, e, b]) else -1), \
+                        (kind, op, e, b)
