@@ -324,9 +324,9 @@ struct Word<true> {
   uint32_t lo, hi;
 };
 
-// G consecutive words of one LDS row.  G = 4 moves them as 16-byte LDS accesses
-// (consecutive lanes take consecutive 16-byte slots: conflict free); the row
-// stride C must then be a multiple of 4.
+// G consecutive words of one LDS row (G = 1, 2 or 4; the row stride C is a
+// multiple of G).  G = 4 moves them as 16-byte LDS accesses (consecutive lanes
+// take consecutive 16-byte slots: conflict free).
 template <bool G16, int G>
 __device__ __forceinline__ void lds_getv(const uint32_t* lds, uint32_t idx, Word<G16> (&w)[G]) {
   if constexpr (G == 4 && !G16) {
@@ -337,6 +337,12 @@ __device__ __forceinline__ void lds_getv(const uint32_t* lds, uint32_t idx, Word
     uint4 t1 = *reinterpret_cast<const uint4*>(lds + 2 * idx + 4);
     w[0].lo = t0.x, w[0].hi = t0.y, w[1].lo = t0.z, w[1].hi = t0.w;
     w[2].lo = t1.x, w[2].hi = t1.y, w[3].lo = t1.z, w[3].hi = t1.w;
+  } else if constexpr (G == 2 && !G16) {
+    uint2 t = *reinterpret_cast<const uint2*>(lds + idx);
+    w[0].v = t.x, w[1].v = t.y;
+  } else if constexpr (G == 2 && G16) {
+    uint4 t = *reinterpret_cast<const uint4*>(lds + 2 * idx);
+    w[0].lo = t.x, w[0].hi = t.y, w[1].lo = t.z, w[1].hi = t.w;
   } else {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -356,6 +362,10 @@ __device__ __forceinline__ void lds_putv(uint32_t* lds, uint32_t idx, const Word
   } else if constexpr (G == 4 && G16) {
     *reinterpret_cast<uint4*>(lds + 2 * idx) = make_uint4(w[0].lo, w[0].hi, w[1].lo, w[1].hi);
     *reinterpret_cast<uint4*>(lds + 2 * idx + 4) = make_uint4(w[2].lo, w[2].hi, w[3].lo, w[3].hi);
+  } else if constexpr (G == 2 && !G16) {
+    *reinterpret_cast<uint2*>(lds + idx) = make_uint2(w[0].v, w[1].v);
+  } else if constexpr (G == 2 && G16) {
+    *reinterpret_cast<uint4*>(lds + 2 * idx) = make_uint4(w[0].lo, w[0].hi, w[1].lo, w[1].hi);
   } else {
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -540,8 +550,8 @@ __device__ __forceinline__ void gloadv(const uint8_t* shard, uint64_t word, uint
     }
   }
   w[0] = word < words ? gload1<G16>(shard, word) : Word<G16>{};
-  if constexpr (G > 1) {
-    w[1] = word + 1 < words ? gload1<G16>(shard, word + 1) : Word<G16>{};
+  if constexpr (G > 1) w[1] = word + 1 < words ? gload1<G16>(shard, word + 1) : Word<G16>{};
+  if constexpr (G > 2) {
     w[2] = word + 2 < words ? gload1<G16>(shard, word + 2) : Word<G16>{};
     w[3] = word + 3 < words ? gload1<G16>(shard, word + 3) : Word<G16>{};
   }
@@ -566,6 +576,8 @@ __device__ __forceinline__ void gstorev(uint8_t* shard, uint64_t word, uint64_t 
   if (word < words) gstore1<G16>(shard, word, w[0]);
   if constexpr (G > 1) {
     if (word + 1 < words) gstore1<G16>(shard, word + 1, w[1]);
+  }
+  if constexpr (G > 2) {
     if (word + 2 < words) gstore1<G16>(shard, word + 2, w[2]);
     if (word + 3 < words) gstore1<G16>(shard, word + 3, w[3]);
   }
@@ -638,6 +650,7 @@ template <bool G16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void leo_encode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
   if (a.C % 4 == 0) leo_encode_body<G16, 4>(a, lds);
+  else if (a.C % 2 == 0) leo_encode_body<G16, 2>(a, lds);
   else leo_encode_body<G16, 1>(a, lds);
 }
 
@@ -708,6 +721,7 @@ template <bool G16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void leo_decode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
   if (a.C % 4 == 0) leo_decode_body<G16, 4>(a, lds);
+  else if (a.C % 2 == 0) leo_decode_body<G16, 2>(a, lds);
   else leo_decode_body<G16, 1>(a, lds);
 }
 
@@ -801,6 +815,7 @@ static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uin
   if (C < 1) C = 1;
   if (C > words) C = (uint32_t)words;
   if (C >= 4) C &= ~(uint64_t)3;  // whole 4-word vectors (16-byte LDS / global accesses)
+  else if (C >= 2) C &= ~(uint64_t)1;
   return (uint32_t)C;
 }
 

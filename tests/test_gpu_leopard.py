@@ -76,6 +76,30 @@ def test_reconstruct_random_patterns_match_oracle(bits, k, m):
             assert (to_host(work[i]) == full[i]).all(), (trial, i, sorted(missing))
 
 
+@pytest.mark.parametrize("k,m,length", [(4096, 1024, 128), (10000, 2000, 64)])
+def test_gf16_wide_codecs_match_oracle(k, m, length):
+    """GF16 shapes whose decode transform (8192 and 16384 lanes) leaves room in
+    LDS for only 2 words and 1 word per lane: the narrow-tile kernel paths."""
+    announce(f"leopard gf16 wide {k}+{m} len={length}")
+    rng = np.random.default_rng(k + m)
+    data = [rng.integers(0, 256, length, dtype=np.uint8) for _ in range(k)]
+    rc, par = O.leo_encode(16, k, m, data)
+    assert rc == 0
+    full = np.stack(data + list(par))
+    rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf16")
+    dev = torch.from_numpy(full).cuda()
+    dev[k:] = 0
+    rs.encode(list(dev))
+    assert (dev[k:].cpu().numpy() == full[k:]).all()
+    assert rs.verify(list(dev))
+    for e in (m, m // 3, 1):
+        missing = set(rng.choice(k + m, e, replace=False).tolist())
+        work = [None if i in missing else dev[i] for i in range(k + m)]
+        rs.reconstruct(work)
+        for i in sorted(missing):
+            assert (to_host(work[i]) == full[i]).all(), (e, i)
+
+
 def test_every_pattern_small_gf8_and_gf16():
     announce("leopard all patterns 5+3")
     for bits in (8, 16):
