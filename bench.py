@@ -3,7 +3,7 @@
 Workload: 512 stripes of RS 10+4 (GF(2^8), Vandermonde) with 1 MiB shards of uniform random
 bytes, device-resident.  One step encodes all 512 stripes.
 
-  value               : batched encode -- the 512 stripes in ONE persistent-kernel launch
+  value               : batched encode -- the 512 stripes in ONE batch-kernel launch
                         replayed from a hipGraph (data bytes k*S per second)
   encode_plain_gbs    : the same stripes launched one stripe at a time
   verify_*/reconstruct_*: verify (register compare) and reconstruct with 1..4 random erasures
@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--stripes", type=int, default=512)
     ap.add_argument("--shard", type=int, default=1 << 20)
     ap.add_argument("--cpu-stripes", type=int, default=96)
@@ -368,7 +368,7 @@ def main():
     st.rs.set_variant_table(None)
     assert torch.equal(st.buf[:, k:], ref_parity), "dispatched encode differs"
 
-    # ---- batched encode: one persistent launch, replayed from a hipGraph
+    # ---- batched encode: one batch-kernel launch, replayed from a hipGraph
     b_enc = st.batch("encode", sp)
     tune = b_enc.autotune(reps=3)
     st.buf[:, k:].zero_()
@@ -494,7 +494,7 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": f"RS 10+4 GF(2^8) encode of {N} stripes x {S // 1024} KiB shards, "
-                               "device-resident, batched: one persistent launch per step "
+                               "device-resident, batched: one batch-kernel launch per step "
                                "(hipGraph replay); plain per-stripe launches reported alongside",
                    "stripes": N, "shard_bytes": S, "data_shards": k, "parity_shards": m,
                    "shard_pitch_bytes": pitch},

@@ -220,13 +220,13 @@ int rsgpu_reconstruct_host(const rsgpu_codec* codec, uint8_t* const* shards, uin
 int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k,
                      const rsgpu_cslice* in, const rsgpu_slice* out, void* stream);
 
-/* ---- batches: many stripe operations in one persistent launch.
+/* ---- batches: many stripe operations in one kernel launch.
  * The reference runs each stripe call separately and parallelises inside it
  * (encode_sep_par encode.rs:1177, execute_classic_plan_parallel
  * reconstruct.rs:229, ParallelPolicy parallel.rs:65-264); a batch is that
  * parallel policy for many stripes at once.  Each add_* validates exactly like
  * the single-stripe call of the same name and records the op; classic GF(2^8)
- * ops run in one persistent kernel, tower-field and Leopard ops through their
+ * ops run in one batch kernel, tower-field and Leopard ops through their
  * own kernels on the same stream.  Reconstruct ops enqueue any decode-row solve
  * on `stream` when added and mark their targets present.  Verify ops OR a
  * nonzero value into `device_flag` on mismatch (the caller zeroes it). ---- */
@@ -241,7 +241,8 @@ int rsgpu_batch_add_verify(rsgpu_batch* batch, const rsgpu_codec* codec, const r
 int rsgpu_batch_add_reconstruct(rsgpu_batch* batch, const rsgpu_codec* codec, rsgpu_slot* shards,
                                 size_t n_shards, int32_t data_only, void* stream);
 /* Kernel shape: inputs per pipelined load group (4, 6, 8 or 12), 16-byte vectors
- * per lane (1 or 2; 1 for 12-input groups) and resident workgroups per CU (1..64). */
+ * per lane (1 or 2; 1 for 12-input groups) and resident workgroups per CU (1..64;
+ * 0 launches one workgroup per tile instead of a persistent grid). */
 int rsgpu_batch_set_tune(rsgpu_batch* batch, int32_t inputs_per_group, int32_t vectors_per_lane,
                          int32_t blocks_per_cu);
 /* Times every kernel shape on this batch (`reps` runs each) and keeps the fastest. */

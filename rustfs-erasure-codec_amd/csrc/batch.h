@@ -1,11 +1,11 @@
 // batch.h -- many classic stripes (mixed shapes, shard sizes and operations)
-// executed by one persistent kernel launch.
+// executed by one kernel launch.
 #pragma once
 #include "internal.h"
 
 namespace rsgpu {
 
-// One stripe operation as the persistent kernel sees it (device memory).
+// One stripe operation as the batch kernel sees it (device memory).
 struct BatchStripe {
   uint64_t n16;            // 16-byte vectors per shard
   uint64_t tile0;          // first global tile index of this stripe
@@ -17,14 +17,15 @@ struct BatchStripe {
   uint16_t mode, tower;
 };
 
-// Shape of the persistent kernel: inputs per pipelined load group (kb), 16-byte
+// Shape of the batch kernel: inputs per pipelined load group (kb), 16-byte
 // vectors per lane (nv; tiles are kBlock * nv vectors per shard), resident
-// workgroups per CU, and the accumulator rows compiled in (4 or 8, from the
-// widest descriptor in the batch).
+// workgroups per CU of a persistent grid (0: one workgroup per tile, the
+// fastest grid on every workload measured), and the accumulator rows compiled
+// in (4 or 8, from the widest descriptor in the batch).
 struct BatchTune {
   int kb = 8;
   int nv = 1;
-  int blocks_per_cu = 8;
+  int blocks_per_cu = 0;
   int rmax = 4;
 };
 

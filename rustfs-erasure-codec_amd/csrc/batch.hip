@@ -1,10 +1,11 @@
-// batch.hip -- the persistent ragged-batch kernel and its host object.
+// batch.hip -- the ragged-batch kernel and its host object.
 //
 // Thousands of classic GF(2^8) stripe operations (encode, verify, reconstruct;
 // any mix of k, r and shard length) are flattened into one table of stripe
 // descriptors plus one table of shard pointers in device memory.  Every stripe
 // is cut into tiles of kBlock 16-byte vectors per shard; the tiles of all
-// stripes form one global index space that a persistent grid strides through.
+// stripes form one global index space; the grid strides through it (one
+// workgroup per tile by default, or a persistent grid of a few per CU).
 // A workgroup maps its tile to the stripe by a wave-uniform binary search over
 // the stripes' first-tile offsets (scalar loads), then runs the same
 // register-accumulator matrix apply as the single-stripe kernel with runtime
@@ -173,7 +174,7 @@ __device__ __forceinline__ void finish(const Cursor& c, uint32_t (&acc)[RMAX][NV
       for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
 }
 
-// Software-pipelined persistent kernel: while a wave computes one chunk (KB
+// Software-pipelined grid-stride kernel: while a wave computes one chunk (KB
 // inputs of its current tile) the 128-bit loads of its next chunk -- the next
 // input group, or the first group of its next tile, possibly in another
 // stripe -- are already in flight in the second register buffer.
@@ -237,8 +238,14 @@ static void launch_kb(int kb, int nv, unsigned grid, hipStream_t s, const BatchS
 hipError_t launch_batch(const BatchStripe* d_stripes, int n_stripes, const void* const* d_ptrs,
                         uint64_t total_tiles, const BatchTune& tune, int n_cu, hipStream_t s) {
   if (total_tiles == 0 || n_stripes == 0) return hipSuccess;
-  uint64_t grid = (uint64_t)n_cu * tune.blocks_per_cu;
+  // blocks_per_cu == 0: one workgroup per tile, handed out in tile order by the
+  // hardware dispatcher, so the tiles in flight stay a compact window of the
+  // shards (DRAM page locality); the grid is capped so that grid x kBlock
+  // threads fit 32 bits, and the kernel strides over any remainder.
+  uint64_t grid = tune.blocks_per_cu > 0 ? (uint64_t)n_cu * tune.blocks_per_cu : total_tiles;
   if (grid > total_tiles) grid = total_tiles;
+  const uint64_t max_grid = (UINT64_C(1) << 32) / kBlock - 1;
+  if (grid > max_grid) grid = max_grid;
   if (tune.rmax <= 4)
     launch_kb<4>(tune.kb, tune.nv, (unsigned)grid, s, d_stripes, n_stripes, d_ptrs, total_tiles);
   else
@@ -260,7 +267,7 @@ struct rsgpu_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
-  // ops the persistent kernel does not take (tower field, Leopard): kept and run
+  // ops the batch kernel does not take (tower field, Leopard): kept and run
   // through their own entry points on the batch's stream.
   struct Deferred {
     const rsgpu_codec* codec;
@@ -392,13 +399,13 @@ int rsgpu_batch_clear(rsgpu_batch* b) {
   return RSGPU_OK;
 }
 
-static bool persistent_ok(const rsgpu_codec* c) {
+static bool batch_kernel_ok(const rsgpu_codec* c) {
   return c->c.family == RSGPU_FAMILY_CLASSIC && c->c.field == RSGPU_FIELD_GF8;
 }
 
 int rsgpu_batch_add_encode(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_slice* shards, size_t n) {
   if (!b || !c || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
-  if (!persistent_ok(c)) {
+  if (!batch_kernel_ok(c)) {
     rsgpu_batch::Deferred d{c, 0, std::vector<rsgpu_slice>(shards, shards + n), {}, {}, nullptr, 0, {}};
     b->deferred.push_back(std::move(d));
     b->plan.n_ops++;
@@ -413,7 +420,7 @@ int rsgpu_batch_add_encode(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_sli
 int rsgpu_batch_add_verify(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_cslice* shards,
                            size_t n, int32_t* device_flag) {
   if (!b || !c || !shards || !device_flag) return RSGPU_ERR_INVALID_ARGUMENT;
-  if (!persistent_ok(c)) {
+  if (!batch_kernel_ok(c)) {
     rsgpu_batch::Deferred d{c, 1, {}, std::vector<rsgpu_cslice>(shards, shards + n), {}, device_flag, 0, {}};
     b->deferred.push_back(std::move(d));
     b->plan.n_ops++;
@@ -428,7 +435,7 @@ int rsgpu_batch_add_verify(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_csl
 int rsgpu_batch_add_reconstruct(rsgpu_batch* b, const rsgpu_codec* c, rsgpu_slot* shards, size_t n,
                                 int32_t data_only, void* stream) {
   if (!b || !c || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
-  if (!persistent_ok(c)) {
+  if (!batch_kernel_ok(c)) {
     rsgpu_batch::Deferred d{c, 2, {}, {}, std::vector<rsgpu_slot>(shards, shards + n), nullptr,
                             data_only ? 1 : 0, {}};
     for (size_t i = 0; i < n; ++i) d.present0.push_back(shards[i].present);
@@ -446,7 +453,7 @@ int rsgpu_batch_set_tune(rsgpu_batch* b, int32_t inputs_per_group, int32_t vecto
                          int32_t blocks_per_cu) {
   const int kb = inputs_per_group, nv = vectors_per_lane;
   if (!b || (kb != 4 && kb != 6 && kb != 8 && kb != 12) || (nv != 1 && nv != 2) ||
-      (kb == 12 && nv != 1) || blocks_per_cu < 1 || blocks_per_cu > 64)
+      (kb == 12 && nv != 1) || blocks_per_cu < 0 || blocks_per_cu > 64)
     return RSGPU_ERR_INVALID_ARGUMENT;
   if (nv != b->tune.nv) b->dirty = true;
   b->tune.nv = nv;
@@ -530,7 +537,7 @@ int rsgpu_batch_autotune(rsgpu_batch* b, void* stream, int32_t reps, int32_t* be
   if (!b || reps < 1) return RSGPU_ERR_INVALID_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
   static const int shapes[][2] = {{4, 1}, {6, 1}, {8, 1}, {12, 1}, {4, 2}, {6, 2}};
-  static const int bpcs[] = {2, 4, 6, 8, 12, 16};
+  static const int bpcs[] = {2, 4, 6, 8, 12, 16, 32, 0};
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return RSGPU_ERR_HIP;
   float best = 1e30f;

@@ -650,7 +650,7 @@ int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int
 }
 
 // ------------------------------------------------------------ batch stripes
-// Adds one stripe operation to a persistent-kernel batch: descriptors of at
+// Adds one stripe operation to a batch: descriptors of at
 // most kMaxOut outputs each, plus byte-kernel jobs for unaligned views and for
 // the length % 16 tail.
 static int add_stripe(BatchPlan& plan, const std::vector<const uint8_t*>& in,

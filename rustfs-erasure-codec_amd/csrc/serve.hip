@@ -9,7 +9,7 @@
 //     ((k+m) * shard_len: inputs resident in place plus the outputs the server
 //     allocates) fits under the budget and their outputs fit in the arena;
 //   * continuous batching: whenever one of the batch slots (a stream with its
-//     own persistent-kernel batch) is idle, every admitted, not yet launched
+//     own batch-kernel batch) is idle, every admitted, not yet launched
 //     request is packed into one batch and launched; batches on different slots
 //     overlap on the device, so requests that arrive while others run are
 //     admitted and start as soon as memory and a slot free up;

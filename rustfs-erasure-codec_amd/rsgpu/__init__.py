@@ -221,9 +221,9 @@ class ReedSolomon:
         """rows: (n_rows, row_len * elem_bytes) uint8 array, GF16 elements as [x, c] pairs."""
         return cls(data_shards, parity_shards, matrix_mode="vandermonde", _custom=rows, **opts)
 
-    def __del__(self):
+    def __del__(self, _free=_lib.rsgpu_codec_free):  # bound now: module globals are gone at exit
         if getattr(self, "_h", None):
-            _lib.rsgpu_codec_free(self._h)
+            _free(self._h)
             self._h = None
 
     # ---- shape
@@ -621,9 +621,9 @@ class Mlp:
         _check(_lib.rsgpu_mlp_dims(self._h, C.byref(a), C.byref(hdim), C.byref(b), C.byref(n)))
         self.n_in, self.n_hidden, self.n_out, self.n_params = a.value, hdim.value, b.value, n.value
 
-    def __del__(self):
+    def __del__(self, _free=_lib.rsgpu_mlp_free):  # bound now: module globals are gone at exit
         if getattr(self, "_h", None):
-            _lib.rsgpu_mlp_free(self._h)
+            _free(self._h)
             self._h = None
 
     def get_params(self):
@@ -685,7 +685,7 @@ _lib.rsgpu_batch_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
 
 
 class Batch:
-    """Many stripe operations executed by one persistent kernel launch."""
+    """Many stripe operations executed by one batch-kernel launch."""
 
     def __init__(self):
         h = C.c_void_p()
@@ -693,9 +693,9 @@ class Batch:
         self._h = h
         self._keep = []
 
-    def __del__(self):
+    def __del__(self, _free=_lib.rsgpu_batch_free):  # bound now: module globals are gone at exit
         if getattr(self, "_h", None):
-            _lib.rsgpu_batch_free(self._h)
+            _free(self._h)
             self._h = None
 
     def clear(self):

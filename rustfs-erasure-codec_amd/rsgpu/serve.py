@@ -58,9 +58,9 @@ class Server:
         _check(_L.rsgpu_server_new(C.byref(cfg), C.byref(self._h)))
         self._keep = []
 
-    def close(self):
+    def close(self, _free=_L.rsgpu_server_free):  # bound now: module globals are gone at exit
         if getattr(self, "_h", None):
-            _L.rsgpu_server_free(self._h)
+            _free(self._h)
             self._h = None
 
     def __del__(self):

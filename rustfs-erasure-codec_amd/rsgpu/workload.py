@@ -2,7 +2,7 @@
 
 ragged(): thousands of stripes with mixed (k, m), log-uniform shard sizes and a mix of
 encode / verify / reconstruct, laid out in one device allocation, with helpers to run
-them one stripe per launch ("plain") or as one persistent batch.
+them one stripe per launch ("plain") or as one batch-kernel launch.
 """
 import ctypes as C
 import math

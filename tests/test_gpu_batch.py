@@ -1,4 +1,4 @@
-"""GPU parity of the persistent ragged-batch kernel: every stripe of a mixed batch
+"""GPU parity of the ragged-batch kernel: every stripe of a mixed batch
 (shapes, shard sizes, encode/verify/reconstruct, erasures) equals the oracle,
 also when the batch is replayed from a hipGraph and after autotuning."""
 import numpy as np
@@ -115,6 +115,27 @@ def test_autotune_keeps_outputs_bit_exact():
     assert st["ops"] == len(items)
 
 
+@pytest.mark.parametrize("kb,nv,bpc", [(8, 1, 0), (4, 2, 0), (12, 1, 0), (6, 1, 1), (6, 2, 32)])
+def test_every_launch_shape_is_bit_exact(kb, nv, bpc):
+    """Persistent grids of any size and the one-workgroup-per-tile grid (0)."""
+    announce(f"batch tune kb={kb} nv={nv} blocks_per_cu={bpc}")
+    _, codecs, mats, items = _mixed_batch(11, 30, max_len=1 << 15, tails=True)
+    batch = rsgpu.Batch()
+    outs = []
+    for it in items:
+        k, m, n = it["k"], it["m"], it["length"]
+        sh = [torch.from_numpy(d).cuda() for d in it["data"]] + \
+             [torch.full((n,), 0xA5, dtype=torch.uint8, device="cuda") for _ in range(m)]
+        batch.add_encode(codecs[(k, m)], sh)
+        outs.append((sh, it))
+    batch.set_tune(kb, nv, bpc)
+    batch.run()
+    torch.cuda.synchronize()
+    for sh, it in outs:
+        for j in range(it["m"]):
+            assert (to_host(sh[it["k"] + j]) == it["parity"][j]).all(), (it["k"], it["m"], it["length"], j)
+
+
 def test_batch_with_tower_and_leopard_codecs_defers():
     announce("batch deferred tower op")
     k, m, n = 5, 3, 256
@@ -134,7 +155,7 @@ def test_batch_with_tower_and_leopard_codecs_defers():
 
 def test_ragged_workload_plain_equals_batched():
     """The ragged mix (mixed shapes, 4 KiB-256 KiB shards, encode/verify/reconstruct with
-    0..m erasures): the persistent batch writes the same bytes as per-stripe launches."""
+    0..m erasures): the batch kernel writes the same bytes as per-stripe launches."""
     announce("ragged workload plain == batched")
     from rsgpu import workload
 
