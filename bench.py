@@ -58,6 +58,8 @@ def parse():
                     help="request rates (req/s) for the serving section ('' skips)")
     ap.add_argument("--leopard-only", action="store_true",
                     help="only the Leopard section (for profiling its kernels)")
+    ap.add_argument("--mfma-only", action="store_true",
+                    help="only the int8 matrix-core vs VALU encode comparison")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the tuned batch encode launches plus a 1 GiB device copy "
                          "(the reference kernel for scaling rocprofv3 byte counters)")
@@ -259,6 +261,49 @@ def leopard_section(torch, rsgpu, DSP, disp):
     return out
 
 
+def gf2_mfma_section(torch, rsgpu):
+    """The int8 matrix-core formulation (bit-expanded GF(2) product) against the VALU
+    apply kernels on single large stripes: device time per encode (HIP events), the
+    best of all 15 apply variants and the default one, outputs compared byte for byte."""
+    out = {}
+    for k, m, S in ((10, 4, 64 << 20), (24, 8, 32 << 20), (31, 8, 32 << 20)):
+        rs = rsgpu.ReedSolomon(k, m)
+        g = torch.Generator(device="cuda").manual_seed(k * 100 + m)
+        data = [torch.randint(0, 256, (S,), dtype=torch.uint8, device="cuda", generator=g)
+                for _ in range(k)]
+        par_v = [torch.zeros(S, dtype=torch.uint8, device="cuda") for _ in range(m)]
+        par_x = [torch.zeros(S, dtype=torch.uint8, device="cuda") for _ in range(m)]
+
+        def dev_ms(fn):
+            fn()
+            return float(event_durations(torch, lambda i: fn(), 5).mean())
+
+        rs.set_variant(-1)
+        t_default = dev_ms(lambda: rs.encode_sep(data, par_v))
+        best_v, t_best = -1, t_default
+        for v in range(rsgpu.variant_count()):
+            if not rs.variant_valid("encode", 0, v):
+                continue
+            rs.set_variant(v)
+            t = dev_ms(lambda: rs.encode_sep(data, par_v))
+            if t < t_best:
+                best_v, t_best = v, t
+        rs.set_variant(-1)
+        t_mfma = dev_ms(lambda: rs.encode_sep_gf2mfma(data, par_x))
+        torch.cuda.synchronize()
+        same = all(torch.equal(a, b) for a, b in zip(par_v, par_x))
+        moved = (k + m) * S
+        out[f"{k}+{m}_{S >> 20}MiB"] = {
+            "vector_best_variant": best_v, "vector_best_ms": round(t_best, 4),
+            "vector_default_ms": round(t_default, 4), "mfma_ms": round(t_mfma, 4),
+            "vector_best_moved_gbs": round(moved / t_best / 1e6, 1),
+            "mfma_moved_gbs": round(moved / t_mfma / 1e6, 1),
+            "mfma_vs_vector_best": round(t_best / t_mfma, 3), "identical": same}
+        del data, par_v, par_x, rs
+        torch.cuda.empty_cache()
+    return out
+
+
 def host_section(rsgpu):
     """PCIe-inclusive rates (never `value`): the host-pointer entry point (copy in,
     encode, copy out per call) and the block-stream encode (4 MiB blocks through
@@ -332,6 +377,14 @@ def main():
         sys.exit(2)
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.mfma_only:
+        import torch
+
+        import rsgpu
+
+        torch.cuda.init()
+        print(json.dumps({"gf2_mfma": gf2_mfma_section(torch, rsgpu)}))
         return
     if args.leopard_only:
         import torch
@@ -471,6 +524,7 @@ def main():
         del b_rg, rg
 
     leopard = leopard_section(torch, rsgpu, DSP, disp) if args.leopard else None
+    gf2_mfma = gf2_mfma_section(torch, rsgpu)
     dispatch_live = None
     if args.dispatch_live:
         dispatch_live = DSP.evaluate_live(torch, disp, DSP.sample_specs(args.dispatch_live,
@@ -515,6 +569,7 @@ def main():
         "ragged": ragged,
         "dispatch_live": dispatch_live,
         "leopard": leopard,
+        "gf2_mfma": gf2_mfma,
         "serving": serving,
         "host_boundary": host,
         "stripe_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),

@@ -130,6 +130,14 @@ int rsgpu_encode(const rsgpu_codec* codec, const rsgpu_slice* shards, size_t n_s
 /* ReedSolomon::encode_sep */
 int rsgpu_encode_sep(const rsgpu_codec* codec, const rsgpu_cslice* data, size_t n_data,
                      const rsgpu_slice* parity, size_t n_parity, void* stream);
+/* Same outputs as rsgpu_encode_sep, computed as a GF(2) bit-matrix product on the
+ * int8 matrix cores (bit-expanded data, parity of the int32 sums).  An evaluated
+ * alternative to the VALU kernels with no counterpart in the reference: classic
+ * GF(2^8) codecs with k <= 31 and m <= 8, 4-byte-aligned shards whose length is a
+ * multiple of 4 (else RSGPU_ERR_INVALID_ARGUMENT); other codecs return
+ * RSGPU_UNSUPPORTED_CODEC_FAMILY. */
+int rsgpu_encode_sep_gf2mfma(const rsgpu_codec* codec, const rsgpu_cslice* data, size_t n_data,
+                             const rsgpu_slice* parity, size_t n_parity, void* stream);
 
 /* ---- incremental encode: src/core/encode.rs:892-1132 (classic codecs only;
  * Leopard codecs return RSGPU_UNSUPPORTED_CODEC_FAMILY) ---- */

@@ -19,15 +19,6 @@ void set_hip_error(hipError_t e, const char* where) {
   t_hip_error = std::string(where) + ": " + hipGetErrorString(e);
 }
 
-#define HIP_TRY(expr)                              \
-  do {                                             \
-    hipError_t _e = (expr);                        \
-    if (_e != hipSuccess) {                        \
-      set_hip_error(_e, #expr);                    \
-      return _e == hipErrorOutOfMemory ? RSGPU_ERR_OUT_OF_MEMORY : RSGPU_ERR_HIP; \
-    }                                              \
-  } while (0)
-
 DeviceBuffer::~DeviceBuffer() {
   if (p) (void)hipFree(p);
 }

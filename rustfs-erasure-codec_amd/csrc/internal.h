@@ -159,12 +159,25 @@ struct Codec {
   // codec of a Leopard GF(2^8) codec), with a device copy of log + exp.
   std::shared_ptr<const FieldTables> ftab;
   DeviceBuffer d_ftab;
+  // A operands of the GF(2) int8-MFMA encode (gf2_mfma.hip), built on first use
+  // (guarded by mu).
+  mutable DeviceBuffer d_gf2_bits;
 };
 
 int words_per_coef(int tower);  // kTabWords or 4 * kTabWords
 
 // thread-local last HIP error
 void set_hip_error(hipError_t e, const char* where);
+
+// Returns the status of a failed HIP call from the enclosing C-ABI function.
+#define HIP_TRY(expr)                                                             \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      ::rsgpu::set_hip_error(_e, #expr);                                          \
+      return _e == hipErrorOutOfMemory ? RSGPU_ERR_OUT_OF_MEMORY : RSGPU_ERR_HIP; \
+    }                                                                             \
+  } while (0)
 
 }  // namespace rsgpu
 

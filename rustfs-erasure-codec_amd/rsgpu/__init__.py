@@ -102,6 +102,7 @@ _lib.rsgpu_codec_info.argtypes = [_vp, C.POINTER(_sz), C.POINTER(_sz), _i32p, _i
 _lib.rsgpu_codec_matrix.argtypes = [_vp, _u8p, _sz]
 _lib.rsgpu_encode.argtypes = [_vp, C.POINTER(Slice), _sz, _vp]
 _lib.rsgpu_encode_sep.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _vp]
+_lib.rsgpu_encode_sep_gf2mfma.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _vp]
 _lib.rsgpu_verify.argtypes = [_vp, C.POINTER(CSlice), _sz, _i32p, _vp]
 _lib.rsgpu_verify_async.argtypes = [_vp, C.POINTER(CSlice), _sz, _vp, _vp]
 _lib.rsgpu_verify_with_buffer.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _i32p, _vp]
@@ -259,6 +260,13 @@ class ReedSolomon:
         d = (CSlice * len(data))(*[CSlice(*_ptr_len(s)) for s in data])
         p = (Slice * len(parity))(*[Slice(*_ptr_len(s)) for s in parity])
         _check(_lib.rsgpu_encode_sep(self._h, d, len(data), p, len(parity), _stream(stream)))
+
+    def encode_sep_gf2mfma(self, data, parity, stream=None):
+        """encode_sep through the int8 matrix-core formulation (bit-expanded GF(2)
+        product); classic GF(2^8), k <= 31, m <= 8, lengths a multiple of 4."""
+        d = (CSlice * len(data))(*[CSlice(*_ptr_len(s)) for s in data])
+        p = (Slice * len(parity))(*[Slice(*_ptr_len(s)) for s in parity])
+        _check(_lib.rsgpu_encode_sep_gf2mfma(self._h, d, len(data), p, len(parity), _stream(stream)))
 
     # ---- verify
     def verify(self, shards, stream=None):
