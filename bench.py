@@ -258,6 +258,25 @@ def leopard_section(torch, rsgpu, DSP, disp):
                 del case
         out[name] = rows
         del rs
+    # the two Leopard GF8 shapes the reference publishes (1 MiB shards, data GiB/s on
+    # Apple M-series NEON: docs/benchmark-results-2026-06-04.md:53-54)
+    for name, k, m, pub_gib in (("gf8_10+4_1MiB", 10, 4, 10.47), ("gf8_32+16_1MiB", 32, 16, 7.04)):
+        rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf8")
+        S = 1 << 20
+        rows = {"reference_published_encode_gbs": round(pub_gib * 2 ** 30 / 1e9, 2)}
+        for op, e in (("encode", 0), ("reconstruct", m)):
+            case = DSP.OpCase(torch, rs, "leopard_gf8", k, m, S, op, e, seed=k + e)
+            rs.set_variant_table(None)
+            t_def = case.time([-1])[-1]
+            disp.install(rs, "leopard_gf8")
+            v = rs.variant_for(op, e, S)
+            t_dsp = case.time([-1])[-1]
+            rs.set_variant_table(None)
+            rows[op] = {"default_gbs": round(k * S / t_def / 1e3, 2),
+                        "dispatched_gbs": round(k * S / t_dsp / 1e3, 2), "dispatched_variant": v}
+            del case
+        out[name] = rows
+        del rs
     return out
 
 

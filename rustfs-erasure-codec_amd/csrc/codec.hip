@@ -22,6 +22,48 @@ void set_hip_error(hipError_t e, const char* where) {
 DeviceBuffer::~DeviceBuffer() {
   if (p) (void)hipFree(p);
 }
+DecodePool::~DecodePool() {
+  for (auto& sl : all) {
+    if (sl.ready) (void)hipEventDestroy(sl.ready);
+    (void)hipFree(sl.mem);
+  }
+}
+
+hipError_t DecodePool::acquire(Slot& out) {
+  std::lock_guard<std::mutex> g(mu);
+  if (free_slots.empty() && retired.size() >= kRecycleBatch) {
+    hipError_t e = hipDeviceSynchronize();  // every queued reader of a retired slot is done
+    if (e != hipSuccess) return e;
+    free_slots.insert(free_slots.end(), retired.begin(), retired.end());
+    retired.clear();
+  }
+  if (!free_slots.empty()) {
+    out = free_slots.back();
+    free_slots.pop_back();
+    return hipSuccess;
+  }
+  Slot sl{nullptr, nullptr};
+  hipError_t e = hipMalloc(&sl.mem, slot_bytes);
+  if (e != hipSuccess) return e;
+  e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    (void)hipFree(sl.mem);
+    return e;
+  }
+  all.push_back(sl);
+  out = sl;
+  return hipSuccess;
+}
+
+void DecodePool::release(const Slot& s) {
+  std::lock_guard<std::mutex> g(mu);
+  retired.push_back(s);
+}
+
+DecodeEntry::~DecodeEntry() {
+  if (pool && slot.mem) pool->release(slot);
+}
+
 hipError_t DeviceBuffer::alloc(size_t n) {
   if (p) (void)hipFree(p);
   p = nullptr;
@@ -541,6 +583,14 @@ static bool custom_solvable(const Codec& c, const Plan& plan) {
   return gauss_jordan(F, w, e, e);
 }
 
+// Orders a read of `e` on stream s after its solve when the solve was enqueued
+// on another stream and has not completed yet.
+static int wait_ready(const DecodeEntry& e, hipStream_t s) {
+  if (e.solve_stream == s || hipEventQuery(e.slot.ready) == hipSuccess) return RSGPU_OK;
+  HIP_TRY(hipStreamWaitEvent(s, e.slot.ready, 0));
+  return RSGPU_OK;
+}
+
 static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan& plan,
                             hipStream_t s, std::shared_ptr<DecodeEntry>& out) {
   const std::string key = plan_key(c, shards, plan);
@@ -551,19 +601,27 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
     ++c.hits;
     c.lru.splice(c.lru.begin(), c.lru, it->second);
     out = it->second->second;
-    return RSGPU_OK;
+    return wait_ready(*out, s);
   }
   ++c.misses;
   if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable(c, plan)) return RSGPU_INVALID_CUSTOM_MATRIX;
   const bool tower = c.field == RSGPU_FIELD_GF16;
+  const size_t nt = plan.targets.size();
+  if (!c.decode_pool) {  // one slot size per codec: tables and rows of m targets
+    c.decode_pool = std::make_shared<DecodePool>();
+    const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
+    c.decode_pool->slot_bytes = (max_tab + c.m * c.k * 2 + 255) / 256 * 256;
+  }
   auto e = std::make_shared<DecodeEntry>();
   e->valid = plan.valid;
   e->targets = plan.targets;
-  const size_t nt = plan.targets.size();
-  const size_t tab_bytes = nt * c.k * words_per_coef(tower) * 4;
-  HIP_TRY(e->tables.alloc(tab_bytes));
-  HIP_TRY(e->rows.alloc(nt * c.k * 2));
-  e->bytes = tab_bytes + nt * c.k * 2 + key.size();
+  HIP_TRY(c.decode_pool->acquire(e->slot));
+  e->pool = c.decode_pool;
+  const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
+  e->tables = (uint32_t*)e->slot.mem;
+  e->rows = (uint8_t*)e->slot.mem + max_tab;
+  e->bytes = c.decode_pool->slot_bytes + key.size();
+  e->solve_stream = s;
   DecodeJob job{};
   job.k = (int)c.k;
   job.m = (int)c.m;
@@ -575,12 +633,13 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
   for (size_t i = 0; i < nt; ++i) job.targets[i] = (uint16_t)plan.targets[i];
   int32_t* st = thread_flag(c.device);
   if (!st) return RSGPU_ERR_OUT_OF_MEMORY;
-  hipError_t he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, (uint8_t*)e->rows.p,
-                                     (uint32_t*)e->tables.p, st + 1, s);
+  hipError_t he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, e->rows, e->tables,
+                                     st + 1, s);
   if (he != hipSuccess) {
     set_hip_error(he, "decode_rows");
     return he == hipErrorInvalidValue ? RSGPU_ERR_UNSUPPORTED : RSGPU_ERR_HIP;
   }
+  HIP_TRY(hipEventRecord(e->slot.ready, s));
   out = e;
   if (!c.use_cache) return RSGPU_OK;
   size_t cap = c.cache_capacity;
@@ -623,11 +682,11 @@ static int reconstruct_impl(const Codec& c, rsgpu_slot* shards, size_t n, int go
   std::vector<uint8_t*> out(plan.targets.size());
   for (size_t i = 0; i < c.k; ++i) in[i] = shards[plan.valid[i]].ptr;
   for (size_t t = 0; t < plan.targets.size(); ++t) out[t] = shards[plan.targets[t]].ptr;
-  rc = run_apply((const uint32_t*)e->tables.p, (int)c.k, (int)plan.targets.size(),
+  rc = run_apply(e->tables, (int)c.k, (int)plan.targets.size(),
                  c.field == RSGPU_FIELD_GF16, in, out, plan.shard_len, kWrite, nullptr, s, variant);
   if (rc) return rc;
-  // An entry evicted meanwhile is released when `e` goes out of scope; hipFree
-  // in its destructor waits for the queued kernels that still read it.
+  // An entry evicted meanwhile retires its slot when `e` goes out of scope; the
+  // slot is reused only after a device-wide synchronisation.
   for (int t : plan.targets) shards[t].present = 1;
   return RSGPU_OK;
 }
@@ -736,7 +795,7 @@ int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int g
   std::vector<uint8_t*> out(p.targets.size());
   for (size_t i = 0; i < c.k; ++i) in[i] = shards[p.valid[i]].ptr;
   for (size_t t = 0; t < p.targets.size(); ++t) out[t] = shards[p.targets[t]].ptr;
-  if ((rc = add_stripe(plan, in, out, (const uint32_t*)e->tables.p, p.shard_len, kWrite, nullptr)))
+  if ((rc = add_stripe(plan, in, out, e->tables, p.shard_len, kWrite, nullptr)))
     return rc;
   plan.keep.push_back(e);
   for (int t : p.targets) shards[t].present = 1;

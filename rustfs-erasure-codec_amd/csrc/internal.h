@@ -111,13 +111,40 @@ struct DeviceBuffer {
   hipError_t alloc(size_t n);
 };
 
+// Device slots for decode entries of one codec, each with its own event.  A
+// released slot retires: kernels already queued may still read it.  Retired
+// slots become free again after one device-wide synchronisation, taken only
+// when no free slot is left and at least kRecycleBatch have retired, so a
+// stream of cache misses costs neither a hipMalloc nor a synchronising hipFree
+// per entry.
+struct DecodePool {
+  static constexpr size_t kRecycleBatch = 64;
+  struct Slot {
+    void* mem;
+    hipEvent_t ready;
+  };
+  std::mutex mu;
+  size_t slot_bytes = 0;
+  std::vector<Slot> all, free_slots, retired;
+  ~DecodePool();
+  hipError_t acquire(Slot& out);
+  void release(const Slot& s);
+};
+
 // One cached decode plan: the composed rows for a (present, targets) pattern.
+// `slot` holds the tables [n_targets][k][words] followed by the rows
+// [n_targets][k]; `slot.ready` is recorded on `solve_stream` behind the solve,
+// and a call on another stream waits for it before reading the entry.
 struct DecodeEntry {
-  DeviceBuffer tables;     // [n_targets][k][words]
-  DeviceBuffer rows;       // [n_targets][k] elements
+  std::shared_ptr<DecodePool> pool;
+  DecodePool::Slot slot{nullptr, nullptr};
+  uint32_t* tables = nullptr;
+  uint8_t* rows = nullptr;
+  hipStream_t solve_stream = nullptr;
   std::vector<int> valid;  // survivor indices
   std::vector<int> targets;
   size_t bytes = 0;
+  ~DecodeEntry();
 };
 
 struct VariantTable {
@@ -146,6 +173,7 @@ struct Codec {
       index;
   mutable size_t cache_bytes = 0;
   mutable uint64_t requests = 0, hits = 0, misses = 0;
+  mutable std::shared_ptr<DecodePool> decode_pool;  // created on the first miss
   // Leopard device state (leopard.hip)
   std::shared_ptr<void> leopard;
   // Kernel-variant dispatch (codec.hip): a forced variant (>= 0), else the

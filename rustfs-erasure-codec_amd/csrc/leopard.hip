@@ -805,12 +805,15 @@ std::shared_ptr<const FieldTables> leopard8_field_tables() {
 
 static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopard.get()); }
 
-// Column tile of a launch: as many words as fit LDS for `rows` lanes, at most 64,
-// or the smaller `want` a dispatched variant asks for.
+// Column tile of a launch: as many words as fit LDS for `rows` lanes, at most
+// max(64, 4096 / rows) -- a layer then has rows * C / 8 >= 512 four-word items
+// for the 256 threads even when the codec has few work lanes -- or the smaller
+// `want` a dispatched variant asks for.
 static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0) {
   const uint32_t wb = s.gf16 ? 8 : 4;
   uint64_t C = s.lds_limit / ((uint64_t)rows * wb);
-  if (C > 64) C = 64;
+  const uint64_t cap = std::max<uint64_t>(64, 4096 / rows);
+  if (C > cap) C = cap;
   if (want && C > want) C = want;
   if (C < 1) C = 1;
   if (C > words) C = (uint32_t)words;

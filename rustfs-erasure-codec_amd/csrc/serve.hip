@@ -201,17 +201,27 @@ void admit(rsgpu_server* S, uint64_t t) {
   }
 }
 
-// Packs every admitted request into free batch slots (one batch per slot).
+// Packs every admitted request into free batch slots (one batch per slot).  A
+// burst is shared evenly over the free slots, at least kMinShare requests each,
+// so that the first batch launches once its share is built and the device works
+// on it while the host builds the next; a few pending requests stay one batch.
+constexpr size_t kMinShare = 64;
+
 int launch(rsgpu_server* S, uint64_t t) {
+  size_t free_slots = 0;
+  for (auto& o : S->slots) free_slots += o->busy ? 0 : 1;
   for (size_t si = 0; si < S->slots.size(); ++si) {
     BatchSlot& B = *S->slots[si];
     if (S->pending.empty()) break;
     if (B.busy) continue;
+    const size_t share = std::max((S->pending.size() + free_slots - 1) / free_slots, kMinShare);
+    const size_t limit = std::min<size_t>(share, (size_t)S->cfg.max_batch_ops);
+    --free_slots;
     int rc = rsgpu_batch_clear(B.batch);
     if (rc) return rc;
     B.tags.clear();
     int n_flags = 0;
-    while (!S->pending.empty() && (int)B.tags.size() < S->cfg.max_batch_ops) {
+    while (!S->pending.empty() && B.tags.size() < limit) {
       Req& q = S->reqs.at(S->pending.front());
       S->pending.pop_front();
       const rsgpu_codec* c = q.r.codec;
