@@ -1439,70 +1439,6 @@ int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k, con
 }
 
 // ---- host-pointer variants
-static int stage_shards(size_t n, size_t len, DeviceBuffer& buf, std::vector<uint8_t*>& dptr) {
-  HIP_TRY(buf.alloc(n * ((len + 255) & ~size_t(255))));
-  dptr.resize(n);
-  for (size_t i = 0; i < n; ++i) dptr[i] = (uint8_t*)buf.p + i * ((len + 255) & ~size_t(255));
-  return RSGPU_OK;
-}
-
-int rsgpu_encode_host(const rsgpu_codec* h, uint8_t* const* shards, size_t n, size_t len) {
-  if (!h || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
-  const Codec& c = h->c;
-  if (n < c.k + c.m) return RSGPU_TOO_FEW_SHARDS;
-  if (n > c.k + c.m) return RSGPU_TOO_MANY_SHARDS;
-  if (len == 0) return RSGPU_EMPTY_SHARD;
-  DeviceBuffer buf;
-  std::vector<uint8_t*> d;
-  int rc = stage_shards(n, len, buf, d);
-  if (rc) return rc;
-  for (size_t i = 0; i < c.k; ++i) HIP_TRY(hipMemcpy(d[i], shards[i], len, hipMemcpyHostToDevice));
-  std::vector<rsgpu_slice> sl(n);
-  for (size_t i = 0; i < n; ++i) sl[i] = {d[i], len};
-  if ((rc = rsgpu_encode(h, sl.data(), n, nullptr))) return rc;
-  for (size_t i = c.k; i < n; ++i) HIP_TRY(hipMemcpy(shards[i], d[i], len, hipMemcpyDeviceToHost));
-  return RSGPU_OK;
-}
-
-int rsgpu_verify_host(const rsgpu_codec* h, const uint8_t* const* shards, size_t n, size_t len,
-                      int32_t* ok) {
-  if (!h || !shards || !ok) return RSGPU_ERR_INVALID_ARGUMENT;
-  if (len == 0) return RSGPU_EMPTY_SHARD;
-  DeviceBuffer buf;
-  std::vector<uint8_t*> d;
-  int rc = stage_shards(n, len, buf, d);
-  if (rc) return rc;
-  for (size_t i = 0; i < n; ++i) HIP_TRY(hipMemcpy(d[i], shards[i], len, hipMemcpyHostToDevice));
-  std::vector<rsgpu_cslice> sl(n);
-  for (size_t i = 0; i < n; ++i) sl[i] = {d[i], len};
-  return rsgpu_verify(h, sl.data(), n, ok, nullptr);
-}
-
-int rsgpu_reconstruct_host(const rsgpu_codec* h, uint8_t* const* shards, uint8_t* present,
-                           size_t n, size_t len, int32_t data_only) {
-  if (!h || !shards || !present) return RSGPU_ERR_INVALID_ARGUMENT;
-  if (len == 0) return RSGPU_EMPTY_SHARD;
-  DeviceBuffer buf;
-  std::vector<uint8_t*> d;
-  int rc = stage_shards(n, len, buf, d);
-  if (rc) return rc;
-  std::vector<rsgpu_slot> sl(n);
-  for (size_t i = 0; i < n; ++i) {
-    sl[i] = {d[i], len, present[i] ? 1 : 0};
-    if (present[i]) HIP_TRY(hipMemcpy(d[i], shards[i], len, hipMemcpyHostToDevice));
-  }
-  rc = data_only ? rsgpu_reconstruct_data(h, sl.data(), n, nullptr)
-                 : rsgpu_reconstruct(h, sl.data(), n, nullptr);
-  if (rc) return rc;
-  HIP_TRY(hipDeviceSynchronize());
-  for (size_t i = 0; i < n; ++i)
-    if (!present[i] && sl[i].present) {
-      HIP_TRY(hipMemcpy(shards[i], d[i], len, hipMemcpyDeviceToHost));
-      present[i] = 1;
-    }
-  return RSGPU_OK;
-}
-
 int rsgpu_cache_stats(const rsgpu_codec* h, uint64_t* requests, uint64_t* hits, uint64_t* misses,
                       uint64_t* entries) {
   if (!h) return RSGPU_ERR_INVALID_ARGUMENT;

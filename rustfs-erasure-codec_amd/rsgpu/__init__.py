@@ -355,6 +355,13 @@ class ReedSolomon:
             if s is not a:
                 s[...] = a
 
+    def verify_host(self, shards):
+        arrs = [np.ascontiguousarray(s, np.uint8) for s in shards]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        ok = C.c_int32(0)
+        _check(_lib.rsgpu_verify_host(self._h, ptrs, len(arrs), arrs[0].size, C.byref(ok)))
+        return bool(ok.value)
+
     def reconstruct_host(self, shards, present, data_only=False):
         arrs = [np.ascontiguousarray(s, np.uint8) for s in shards]
         ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
