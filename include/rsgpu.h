@@ -171,7 +171,10 @@ int rsgpu_verify_with_buffer(const rsgpu_codec* codec, const rsgpu_cslice* shard
                              size_t n_shards, const rsgpu_slice* buffer, size_t n_buffer,
                              int32_t* ok, void* stream);
 
-/* ---- reconstruct: src/core/reconstruct.rs:539-790 ---- */
+/* ---- reconstruct: src/core/reconstruct.rs:539-790 ----
+ * Classic codecs solve for at most 128 missing data shards per call (the
+ * inversion runs in one workgroup); more return RSGPU_ERR_UNSUPPORTED.  Missing
+ * parity shards are not limited. */
 /* ReedSolomon::reconstruct (all missing shards). */
 int rsgpu_reconstruct(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards, void* stream);
 /* ReedSolomon::reconstruct_data (missing data shards only). */

@@ -148,8 +148,6 @@ struct DevField {
   }
 };
 
-constexpr int kMaxSolve = 128;  // missing data rows solvable in one workgroup
-
 __global__ __launch_bounds__(256) void decode_rows_kernel(const uint16_t* __restrict__ matrix,
                                                           const DecodeJob job,
                                                           uint8_t* __restrict__ rows_out,
@@ -282,6 +280,180 @@ hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, ui
   size_t lds = ((size_t)e * (e + p + e) + (size_t)job.n_targets * job.k) * sizeof(uint16_t);
   hipLaunchKernelGGL(decode_rows_kernel, dim3(1), dim3(256), lds, s, d_matrix, job, d_rows,
                      d_tables, d_status);
+  return hipGetLastError();
+}
+
+// Writes coefficient v of output element idx: its byte(s) and v_perm tables.
+__device__ __forceinline__ void put_coef(uint16_t v, size_t idx, bool tower, const DevField& F,
+                                         const uint8_t* lg, const uint8_t* ex, uint8_t* rows_out,
+                                         uint32_t* tables_out) {
+  const int W = tower ? 4 * kTabWords : kTabWords;
+  uint32_t* dst = tables_out + idx * W;
+  if (tower) {
+    rows_out[2 * idx] = (uint8_t)(v >> 8);
+    rows_out[2 * idx + 1] = (uint8_t)v;
+    uint8_t cx = v >> 8, cc = v & 0xff;
+    uint8_t co[4] = {(uint8_t)(cc ^ d_mul8(lg, ex, 2, cx)), cx, d_mul8(lg, ex, 128, cx), cc};
+    for (int q = 0; q < 4; ++q) make_perm_table(co[q], dst + q * kTabWords);
+  } else {
+    rows_out[idx] = (uint8_t)v;
+    make_perm_table_with([&](uint8_t a, uint8_t b) { return (uint8_t)F.mul(a, b); }, (uint8_t)v, dst);
+  }
+}
+
+// C^-1 of C[r][c] = matrix[par_surv[r]][missing[c]] (e x e) by Gauss-Jordan on
+// [C | I] in LDS, written row-major to cinv.
+__global__ __launch_bounds__(256) void decode_inv_kernel(const uint16_t* __restrict__ matrix,
+                                                         const DecodeBigJob job,
+                                                         uint16_t* __restrict__ cinv,
+                                                         int32_t* __restrict__ status) {
+  __shared__ uint8_t lg[256];
+  __shared__ uint8_t ex[512];
+  __shared__ int pivot_row;
+  extern __shared__ uint16_t wm[];  // [e][2e]
+  const int tid = threadIdx.x, e = job.e, cols = 2 * e;
+  for (int i = tid; i < 256; i += blockDim.x) lg[i] = job.field ? job.field[i] : kGf8.log[i];
+  for (int i = tid; i < 512; i += blockDim.x) ex[i] = job.field ? job.field[256 + i] : kGf8.exp[i];
+  for (int idx = tid; idx < e * cols; idx += blockDim.x) {
+    const int r = idx / cols, c = idx % cols;
+    wm[idx] = c < e ? matrix[(size_t)job.par_surv[r] * job.k + job.missing[c]]
+                    : (c - e == r ? (uint16_t)job.one : (uint16_t)0);
+  }
+  __syncthreads();
+  DevField F{lg, ex, job.tower != 0};
+  for (int r = 0; r < e; ++r) {
+    if (tid == 0) {
+      int pr = -1;
+      for (int rb = r; rb < e; ++rb)
+        if (wm[rb * cols + r] != 0) {
+          pr = rb;
+          break;
+        }
+      pivot_row = pr;
+    }
+    __syncthreads();
+    const int pr = pivot_row;
+    if (pr < 0) {  // uniform: every thread leaves
+      if (tid == 0) *status = RSGPU_INVALID_CUSTOM_MATRIX;
+      return;
+    }
+    if (pr != r)
+      for (int c = tid; c < cols; c += blockDim.x) {
+        const uint16_t t = wm[r * cols + c];
+        wm[r * cols + c] = wm[pr * cols + c];
+        wm[pr * cols + c] = t;
+      }
+    __syncthreads();
+    const uint16_t scale = F.inv(wm[r * cols + r]);
+    __syncthreads();
+    for (int c = tid; c < cols; c += blockDim.x) wm[r * cols + c] = F.mul(scale, wm[r * cols + c]);
+    __syncthreads();
+    for (int idx = tid; idx < e * cols; idx += blockDim.x) {
+      const int rb = idx / cols, c = idx % cols;
+      if (rb == r || c == r) continue;
+      const uint16_t f = wm[rb * cols + r];
+      if (f) wm[idx] ^= F.mul(f, wm[r * cols + c]);
+    }
+    __syncthreads();
+    for (int rb = tid; rb < e; rb += blockDim.x)
+      if (rb != r) wm[rb * cols + r] = 0;
+    __syncthreads();
+  }
+  for (int idx = tid; idx < e * e; idx += blockDim.x) cinv[idx] = wm[(idx / e) * cols + e + idx % e];
+  if (tid == 0) *status = RSGPU_OK;
+}
+
+struct DecodeTargets {
+  int32_t t0, n;                     // first output row of this launch, rows in it
+  uint16_t target[kBigTargetChunk];  // shard index of each row
+};
+
+// Row t0 + blockIdx.y, columns blockIdx.x * 256 + threadIdx.x.  With w = u C^-1
+// (u = unit row j for missing data j, else the parity row restricted to the
+// missing columns), the coefficient on present data d is base(d) + sum_i w[i] *
+// matrix[par_surv[i]][d] (base = the parity row's own coefficient, 0 for data
+// targets) and the coefficient on parity survivor i is w[i].
+__global__ __launch_bounds__(256) void decode_rows_big_kernel(const uint16_t* __restrict__ matrix,
+                                                              const DecodeBigJob job,
+                                                              const DecodeTargets tg,
+                                                              const uint16_t* __restrict__ cinv,
+                                                              uint8_t* __restrict__ rows_out,
+                                                              uint32_t* __restrict__ tables_out) {
+  __shared__ uint8_t lg[256];
+  __shared__ uint8_t ex[512];
+  __shared__ uint16_t w[kMaxSolve];
+  const int tid = threadIdx.x, k = job.k, e = job.e, p = k - e;
+  for (int i = tid; i < 256; i += blockDim.x) lg[i] = job.field ? job.field[i] : kGf8.log[i];
+  for (int i = tid; i < 512; i += blockDim.x) ex[i] = job.field ? job.field[256 + i] : kGf8.exp[i];
+  __syncthreads();
+  DevField F{lg, ex, job.tower != 0};
+  const int target = tg.target[blockIdx.y];
+  const uint16_t* prow = matrix + (size_t)target * k;
+  if (target < k) {
+    int j = 0;
+    while (j < e && job.missing[j] != target) ++j;  // a data target is a missing shard
+    for (int i = tid; i < e; i += blockDim.x) w[i] = j < e ? cinv[j * e + i] : (uint16_t)0;
+  } else {
+    for (int i = tid; i < e; i += blockDim.x) {
+      uint16_t v = 0;
+      for (int j = 0; j < e; ++j) v ^= F.mul(prow[job.missing[j]], cinv[j * e + i]);
+      w[i] = v;
+    }
+  }
+  __syncthreads();
+  const int col = blockIdx.x * blockDim.x + tid;
+  if (col >= k) return;
+  uint16_t v;
+  if (col < p) {
+    int d = col;  // the col-th present data index
+    for (int j = 0; j < e; ++j)
+      if (job.missing[j] <= d) ++d;
+    v = target < k ? 0 : prow[d];
+    for (int i = 0; i < e; ++i) v ^= F.mul(w[i], matrix[(size_t)job.par_surv[i] * k + d]);
+  } else {
+    v = w[col - p];
+  }
+  put_coef(v, (size_t)(tg.t0 + blockIdx.y) * k + col, job.tower != 0, F, lg, ex, rows_out, tables_out);
+}
+
+hipError_t launch_decode_big(const uint16_t* d_matrix, const DecodeBigJob& job,
+                             const std::vector<int>& targets, uint16_t* d_cinv, uint8_t* d_rows,
+                             uint32_t* d_tables, int32_t* d_status, hipStream_t s) {
+  if (job.e > kMaxSolve) return hipErrorInvalidValue;
+  if (job.e > 0) {
+    const size_t lds = (size_t)job.e * 2 * job.e * sizeof(uint16_t);
+    hipLaunchKernelGGL(decode_inv_kernel, dim3(1), dim3(256), lds, s, d_matrix, job, d_cinv, d_status);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  for (size_t t0 = 0; t0 < targets.size(); t0 += kBigTargetChunk) {
+    DecodeTargets tg{};
+    tg.t0 = (int32_t)t0;
+    tg.n = (int32_t)std::min<size_t>(kBigTargetChunk, targets.size() - t0);
+    for (int i = 0; i < tg.n; ++i) tg.target[i] = (uint16_t)targets[t0 + i];
+    const dim3 grid((unsigned)((job.k + 255) / 256), (unsigned)tg.n);
+    hipLaunchKernelGGL(decode_rows_big_kernel, grid, dim3(256), 0, s, d_matrix, job, tg, d_cinv,
+                       d_rows, d_tables);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
+__global__ __launch_bounds__(kBlock) void compare_rows_kernel(const CompareArgs c) {
+  const uint8_t* a = c.a[blockIdx.y];
+  const uint8_t* b = c.b[blockIdx.y];
+  bool bad = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < c.len;
+       i += (uint64_t)gridDim.x * kBlock)
+    bad |= a[i] != b[i];
+  if (bad) atomicOr(c.flag, 1);
+}
+
+hipError_t launch_compare_rows(const CompareArgs& c, hipStream_t s) {
+  if (c.rows <= 0 || c.len == 0) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((c.len + kBlock - 1) / kBlock, 1024);
+  hipLaunchKernelGGL(compare_rows_kernel, dim3((unsigned)blocks, (unsigned)c.rows), dim3(kBlock), 0, s, c);
   return hipGetLastError();
 }
 

@@ -489,19 +489,17 @@ static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int
   rc = run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m, tower, in, tmp, len,
                  kWrite, nullptr, s, variant);
   if (rc) return rc;
-  // identity rows compare scratch vs stored parity: apply the unit matrix in verify mode
-  std::vector<uint16_t> eye(c.m * c.m, 0);
-  for (size_t j = 0; j < c.m; ++j) eye[j * c.m + j] = 1;
-  std::vector<uint32_t> ht;
-  build_tables_host(eye.data(), (int)c.m, (int)c.m, tower, ht);
-  DeviceBuffer dt;
-  HIP_TRY(dt.alloc(ht.size() * 4));
-  HIP_TRY(hipMemcpyAsync(dt.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, s));
-  std::vector<const uint8_t*> tin(tmp.begin(), tmp.end());
-  if (c.m > (size_t)kMaxIn) return RSGPU_ERR_UNSUPPORTED;
-  rc = run_apply((const uint32_t*)dt.p, (int)c.m, (int)c.m, tower, tin, out, len, kVerify, d_flag, s);
-  HIP_TRY(hipStreamSynchronize(s));  // scratch buffers are freed on return
-  return rc;
+  // compare the recomputed parity with the stored parity, kMaxOut rows a launch
+  for (size_t r0 = 0; r0 < c.m; r0 += kMaxOut) {
+    CompareArgs ca{};
+    ca.rows = (int32_t)std::min<size_t>(kMaxOut, c.m - r0);
+    for (int j = 0; j < ca.rows; ++j) ca.a[j] = tmp[r0 + j], ca.b[j] = out[r0 + j];
+    ca.len = len;
+    ca.flag = d_flag;
+    HIP_TRY(launch_compare_rows(ca, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));  // the scratch buffer is freed on return
+  return RSGPU_OK;
 }
 
 // ------------------------------------------------------------ reconstruction
@@ -607,34 +605,54 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
   if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable(c, plan)) return RSGPU_INVALID_CUSTOM_MATRIX;
   const bool tower = c.field == RSGPU_FIELD_GF16;
   const size_t nt = plan.targets.size();
-  if (!c.decode_pool) {  // one slot size per codec: tables and rows of m targets
+  const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
+  const size_t max_e = std::min<size_t>(std::min(c.k, c.m), kMaxSolve);
+  if (!c.decode_pool) {  // one slot size per codec: tables and rows of m targets, C^-1
     c.decode_pool = std::make_shared<DecodePool>();
-    const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
-    c.decode_pool->slot_bytes = (max_tab + c.m * c.k * 2 + 255) / 256 * 256;
+    c.decode_pool->slot_bytes = (max_tab + c.m * c.k * 2 + max_e * max_e * 2 + 255) / 256 * 256;
   }
   auto e = std::make_shared<DecodeEntry>();
   e->valid = plan.valid;
   e->targets = plan.targets;
   HIP_TRY(c.decode_pool->acquire(e->slot));
   e->pool = c.decode_pool;
-  const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
   e->tables = (uint32_t*)e->slot.mem;
   e->rows = (uint8_t*)e->slot.mem + max_tab;
   e->bytes = c.decode_pool->slot_bytes + key.size();
   e->solve_stream = s;
-  DecodeJob job{};
-  job.k = (int)c.k;
-  job.m = (int)c.m;
-  job.n_targets = (int)nt;
-  job.tower = tower ? 1 : 0;
-  job.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
-  job.one = c.ftab ? c.ftab->one : 1;
-  for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)plan.valid[i];
-  for (size_t i = 0; i < nt; ++i) job.targets[i] = (uint16_t)plan.targets[i];
   int32_t* st = thread_flag(c.device);
   if (!st) return RSGPU_ERR_OUT_OF_MEMORY;
-  hipError_t he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, e->rows, e->tables,
-                                     st + 1, s);
+  hipError_t he;
+  if (c.k > 256 || nt > 256) {  // beyond DecodeJob's lists
+    DecodeBigJob bj{};
+    bj.k = (int)c.k;
+    bj.tower = tower ? 1 : 0;
+    bj.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
+    bj.one = c.ftab ? c.ftab->one : 1;
+    size_t p = 0;
+    while (p < c.k && plan.valid[p] < (int)c.k) ++p;
+    bj.e = (int)(c.k - p);
+    if (bj.e > kMaxSolve) return RSGPU_ERR_UNSUPPORTED;
+    for (size_t d = 0, kn = 0, j = 0; d < c.k; ++d) {
+      if (kn < p && plan.valid[kn] == (int)d) ++kn;
+      else bj.missing[j++] = (uint16_t)d;
+    }
+    for (int i = 0; i < bj.e; ++i) bj.par_surv[i] = (uint16_t)plan.valid[p + i];
+    uint16_t* cinv = (uint16_t*)(e->rows + c.m * c.k * 2);
+    he = launch_decode_big((const uint16_t*)c.d_matrix.p, bj, plan.targets, cinv, e->rows,
+                           e->tables, st + 1, s);
+  } else {
+    DecodeJob job{};
+    job.k = (int)c.k;
+    job.m = (int)c.m;
+    job.n_targets = (int)nt;
+    job.tower = tower ? 1 : 0;
+    job.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
+    job.one = c.ftab ? c.ftab->one : 1;
+    for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)plan.valid[i];
+    for (size_t i = 0; i < nt; ++i) job.targets[i] = (uint16_t)plan.targets[i];
+    he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, e->rows, e->tables, st + 1, s);
+  }
   if (he != hipSuccess) {
     set_hip_error(he, "decode_rows");
     return he == hipErrorInvalidValue ? RSGPU_ERR_UNSUPPORTED : RSGPU_ERR_HIP;
@@ -712,7 +730,8 @@ static int add_stripe(BatchPlan& plan, const std::vector<const uint8_t*>& in,
   for (auto p : out) vec_ok &= aligned16(p);
   const uint64_t n16 = vec_ok ? len / 16 : 0;
   const size_t tail = (size_t)n16 * 16;
-  if (tail < len && k > kMaxIn) return RSGPU_ERR_UNSUPPORTED;
+  // a verify tail compares the complete sum, so its inputs cannot be split
+  if (tail < len && k > kMaxIn && mode == kVerify) return RSGPU_ERR_UNSUPPORTED;
   for (int r0 = 0; r0 < r; r0 += kMaxOut) {
     const int rn = std::min(kMaxOut, r - r0);
     if (n16) {
@@ -730,19 +749,22 @@ static int add_stripe(BatchPlan& plan, const std::vector<const uint8_t*>& in,
       for (int j = 0; j < rn; ++j) plan.ptrs.push_back(out[r0 + j]);
       plan.stripes.push_back(s);
     }
-    if (tail < len) {
+    // the len % 16 tail (or all of an unaligned stripe) in byte jobs of at most
+    // kMaxIn inputs; after the first, each accumulates into the outputs
+    for (int k0 = 0; tail < len && k0 < k; k0 += kMaxIn) {
       ByteArgs b{};
-      for (int i = 0; i < k; ++i) b.in[i] = in[i];
+      const int kn = std::min(kMaxIn, k - k0);
+      for (int i = 0; i < kn; ++i) b.in[i] = in[k0 + i];
       for (int j = 0; j < rn; ++j) b.out[j] = out[r0 + j];
       b.tables = tables + (size_t)r0 * k * kTabWords;
       b.flag = flag;
       b.begin = tail;
       b.end = len;
-      b.k = k;
+      b.k = kn;
       b.tab_row_stride = k * kTabWords;
-      b.tab_col0 = 0;
+      b.tab_col0 = k0;
       b.r = rn;
-      b.mode = mode;
+      b.mode = (k0 == 0 || mode == kVerify) ? mode : kAccumulate;
       b.tower = 0;
       plan.byte_jobs.push_back(b);
     }
@@ -873,10 +895,6 @@ static int codec_common(size_t k, size_t m, const rsgpu_options* opts, Codec& c,
   c.use_cache = o.inversion_cache != 0;
   c.cache_capacity = o.inversion_cache_capacity ? o.inversion_cache_capacity : derive_cache_capacity(k, m);
   c.cache_max_bytes = o.inversion_cache_max_bytes ? o.inversion_cache_max_bytes : (8u << 20);
-  if (c.family == RSGPU_FAMILY_CLASSIC && k > 255 && c.field == RSGPU_FIELD_GF16) {
-    // device decode solve holds the 16-bit survivor table in 256-entry arrays
-    return RSGPU_ERR_UNSUPPORTED;
-  }
   return RSGPU_OK;
 }
 

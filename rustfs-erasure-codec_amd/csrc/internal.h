@@ -94,8 +94,35 @@ struct DecodeJob {
   const uint8_t* field;   // device FieldTables log[256] + exp[512] (nullptr: 0x11d field)
   int32_t one;            // the field's multiplicative identity
 };
+// Sets *flag when row r of a differs from row r of b anywhere in [0, len).
+struct CompareArgs {
+  const uint8_t* a[kMaxOut];
+  const uint8_t* b[kMaxOut];
+  uint64_t len;
+  int32_t* flag;
+  int32_t rows;
+};
+hipError_t launch_compare_rows(const CompareArgs& c, hipStream_t s);
+
 hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
                               uint32_t* d_tables, int32_t* d_status, hipStream_t s);
+
+// Decode rows of codecs beyond DecodeJob's 256-entry lists (classic GF(2^16) with
+// k > 256 or more than 256 targets): C^-1 of the missing-data system in one
+// workgroup, then every (target, column) coefficient in a grid.  Survivors are
+// the present data (every index below k not in `missing`) followed by
+// `par_surv`, as make_plan's first-k-present rule orders them.
+constexpr int kMaxSolve = 128;        // missing data rows solvable in one workgroup
+constexpr int kBigTargetChunk = 1024;  // targets per rows launch
+struct DecodeBigJob {
+  int32_t k, e, tower, one;
+  const uint8_t* field;          // as in DecodeJob
+  uint16_t missing[kMaxSolve];   // missing data indices, ascending
+  uint16_t par_surv[kMaxSolve];  // parity survivors solving for them, ascending
+};
+hipError_t launch_decode_big(const uint16_t* d_matrix, const DecodeBigJob& job,
+                             const std::vector<int>& targets, uint16_t* d_cinv, uint8_t* d_rows,
+                             uint32_t* d_tables, int32_t* d_status, hipStream_t s);
 
 // Tables for a host-known row set (encode rows, apply_rows primitive).
 void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out,

@@ -1,0 +1,93 @@
+"""GPU parity at the classic codecs' large sizes: GF(2^8) at the 256-shard limit
+and GF(2^16) with more than 256 data shards or reconstruct targets (the decode
+solve then runs as C^-1 in one workgroup plus a grid over the coefficients),
+against the oracle's encode and reconstruct."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import announce, rand_shards, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+rsgpu = pytest.importorskip("rsgpu")
+
+F = {"gf8": O.GF8, "gf16": O.GF16}
+
+
+def _check(field, k, m, n, erasure_sets, seed):
+    rng = np.random.default_rng(seed)
+    rc, mat = O.build_matrix(F[field], O.VANDERMONDE, k, m)
+    assert rc == 0
+    data = rand_shards(rng, k, n)
+    full = data + list(O.encode(F[field], mat, k, m, data))
+    rs = rsgpu.ReedSolomon(k, m, field=field)
+    dev = torch.from_numpy(np.stack(full)).cuda()
+    par = dev[k:].clone().zero_()
+    rs.encode_sep(list(dev[:k]), list(par))
+    assert torch.equal(par, dev[k:])
+    assert rs.verify(list(dev))
+    bad = dev.clone()
+    bad[k + m - 1, n - 1] ^= 1  # last byte of the last parity row
+    assert not rs.verify(list(bad))
+    for missing in erasure_sets:
+        work = [None if i in missing else dev[i] for i in range(k + m)]
+        rs.reconstruct(work)
+        for i in sorted(missing):
+            assert (to_host(work[i]) == full[i]).all(), (len(missing), i)
+
+
+@pytest.mark.parametrize("k,m", [(128, 128), (200, 56), (255, 1), (1, 255)])
+def test_gf8_at_the_shard_limit(k, m):
+    announce(f"gf8 limit {k}+{m}")
+    rng = np.random.default_rng(k)
+    sets = [set(range(min(m, k))),  # as many data shards as parity can solve
+            set(rng.choice(k + m, m, replace=False).tolist()),
+            {k + m - 1}]
+    _check("gf8", k, m, 1024 + 7, sets, k * 7 + m)
+
+
+@pytest.mark.parametrize("k,m,n_missing_data,n_missing_parity",
+                         [(300, 40, 40, 0), (300, 40, 17, 23), (300, 40, 0, 40),
+                          (100, 300, 30, 270), (100, 1100, 10, 1050)])
+def test_gf16_beyond_256(k, m, n_missing_data, n_missing_parity):
+    announce(f"gf16 large {k}+{m} missing {n_missing_data}d {n_missing_parity}p")
+    rng = np.random.default_rng(k + m + n_missing_data)
+    missing = set(rng.choice(k, n_missing_data, replace=False).tolist()) | \
+        set((k + rng.choice(m, n_missing_parity, replace=False)).tolist())
+    _check("gf16", k, m, 256 + 2, [missing], k * 3 + m)
+
+
+def test_gf16_more_missing_data_than_one_solve_is_unsupported():
+    announce("gf16 large unsupported solve size")
+    k, m, n = 300, 200, 64
+    rs = rsgpu.ReedSolomon(k, m, field="gf16")
+    sh = [torch.zeros(n, dtype=torch.uint8, device="cuda") for _ in range(k + m)]
+    work = [None if i < 150 else sh[i] for i in range(k + m)]  # 150 data rows > 128
+    with pytest.raises(rsgpu.RsError) as ei:
+        rs.reconstruct(work)
+    assert ei.value.name == "Unsupported"
+
+
+def test_batch_wide_stripes_with_tails():
+    """Batch encode of stripes wider than one 64-input byte job, at lengths with a
+    len % 16 tail: the tail runs as chained byte jobs."""
+    announce("batch wide stripes with tails")
+    batch = rsgpu.Batch()
+    cases = []
+    for k, m, n in ((100, 4, 1000 + 5), (70, 8, 4096 + 9), (65, 3, 33)):
+        rng = np.random.default_rng(k + n)
+        rc, mat = O.build_matrix(O.GF8, O.VANDERMONDE, k, m)
+        data = rand_shards(rng, k, n)
+        exp = O.encode(O.GF8, mat, k, m, data)
+        rs = rsgpu.ReedSolomon(k, m)
+        sh = [to_dev(d) for d in data] + [torch.full((n,), 0x3C, dtype=torch.uint8, device="cuda")
+                                          for _ in range(m)]
+        batch.add_encode(rs, sh)
+        cases.append((rs, sh, exp, k, m))
+    batch.run()
+    torch.cuda.synchronize()
+    for rs, sh, exp, k, m in cases:
+        for j in range(m):
+            assert (to_host(sh[k + j]) == exp[j]).all(), (k, m, j)
