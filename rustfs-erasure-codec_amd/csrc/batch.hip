@@ -15,6 +15,7 @@
 // in a hipGraph and replayed.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 
 #include "batch.h"
 #include "kernel_common.h"
@@ -262,6 +263,15 @@ struct rsgpu_batch {
   BatchPlan plan;
   BatchTune tune;
   DeviceBuffer d_stripes, d_ptrs;
+  // Buffers the tables outgrew, freed with the batch: a hipFree waits for the
+  // whole device, and queued launches may still read them.
+  std::vector<std::unique_ptr<DeviceBuffer>> outgrown;
+  // Pinned copy source of the tables, so uploads are asynchronous; `staged` is
+  // recorded behind the copies out of it.
+  uint8_t* h_stage = nullptr;
+  size_t h_stage_bytes = 0;
+  hipEvent_t staged = nullptr;
+  hipStream_t stage_stream = nullptr;
   bool dirty = true;
   int device = 0, n_cu = 256;
   hipGraph_t graph = nullptr;
@@ -284,6 +294,11 @@ struct rsgpu_batch {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (graph_stream) (void)hipStreamDestroy(graph_stream);
+    if (staged) {
+      (void)hipEventSynchronize(staged);
+      (void)hipEventDestroy(staged);
+    }
+    if (h_stage) (void)hipHostFree(h_stage);
   }
 };
 
@@ -318,24 +333,50 @@ static void retile(rsgpu_batch* b) {
 }
 
 static int upload(rsgpu_batch* b, hipStream_t s) {
-  if (!b->dirty) return RSGPU_OK;
+  if (!b->dirty) {  // a launch on another stream than the upload's waits for its copies
+    if (b->staged && b->stage_stream != s && hipStreamWaitEvent(s, b->staged, 0) != hipSuccess)
+      return RSGPU_ERR_HIP;
+    return RSGPU_OK;
+  }
   retile(b);
   auto& p = b->plan;
-  size_t sb = std::max<size_t>(p.stripes.size(), 1) * sizeof(BatchStripe);
-  size_t pb = std::max<size_t>(p.ptrs.size(), 1) * sizeof(void*);
-  // grow geometrically: a reallocation frees the old buffer, which waits for the device
-  if (b->d_stripes.bytes < sb && b->d_stripes.alloc(std::max(sb, 2 * b->d_stripes.bytes)) != hipSuccess)
-    return RSGPU_ERR_OUT_OF_MEMORY;
-  if (b->d_ptrs.bytes < pb && b->d_ptrs.alloc(std::max(pb, 2 * b->d_ptrs.bytes)) != hipSuccess)
-    return RSGPU_ERR_OUT_OF_MEMORY;
+  const size_t sb = std::max<size_t>(p.stripes.size(), 1) * sizeof(BatchStripe);
+  const size_t pb = std::max<size_t>(p.ptrs.size(), 1) * sizeof(void*);
+  // grow geometrically; an outgrown buffer is parked, not freed
+  auto grow = [&](DeviceBuffer& d, size_t need) -> bool {
+    if (d.bytes >= need) return true;
+    auto old = std::make_unique<DeviceBuffer>();
+    std::swap(old->p, d.p);
+    std::swap(old->bytes, d.bytes);
+    const size_t want = std::max(need, 2 * old->bytes);
+    if (old->p) b->outgrown.push_back(std::move(old));
+    return d.alloc(want) == hipSuccess;
+  };
+  if (!grow(b->d_stripes, sb) || !grow(b->d_ptrs, pb)) return RSGPU_ERR_OUT_OF_MEMORY;
+  if (!b->staged && hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess)
+    return RSGPU_ERR_HIP;
+  if (hipEventSynchronize(b->staged) != hipSuccess) return RSGPU_ERR_HIP;  // last copies are done
+  const size_t stage = sb + pb;
+  if (b->h_stage_bytes < stage) {
+    const size_t want = std::max(stage, 2 * b->h_stage_bytes);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    b->h_stage = nullptr;
+    b->h_stage_bytes = 0;
+    if (hipHostMalloc((void**)&b->h_stage, want, hipHostMallocDefault) != hipSuccess)
+      return RSGPU_ERR_OUT_OF_MEMORY;
+    b->h_stage_bytes = want;
+  }
+  std::memcpy(b->h_stage, p.stripes.data(), p.stripes.size() * sizeof(BatchStripe));
+  std::memcpy(b->h_stage + sb, p.ptrs.data(), p.ptrs.size() * sizeof(void*));
   if (!p.stripes.empty() &&
-      hipMemcpyAsync(b->d_stripes.p, p.stripes.data(), p.stripes.size() * sizeof(BatchStripe),
+      hipMemcpyAsync(b->d_stripes.p, b->h_stage, p.stripes.size() * sizeof(BatchStripe),
                      hipMemcpyHostToDevice, s) != hipSuccess)
     return RSGPU_ERR_HIP;
-  if (!p.ptrs.empty() && hipMemcpyAsync(b->d_ptrs.p, p.ptrs.data(), p.ptrs.size() * sizeof(void*),
+  if (!p.ptrs.empty() && hipMemcpyAsync(b->d_ptrs.p, b->h_stage + sb, p.ptrs.size() * sizeof(void*),
                                         hipMemcpyHostToDevice, s) != hipSuccess)
     return RSGPU_ERR_HIP;
-  if (hipStreamSynchronize(s) != hipSuccess) return RSGPU_ERR_HIP;  // host vectors may change later
+  if (hipEventRecord(b->staged, s) != hipSuccess) return RSGPU_ERR_HIP;
+  b->stage_stream = s;
   b->dirty = false;
   invalidate_graph(b);
   return RSGPU_OK;

@@ -148,14 +148,14 @@ struct DevField {
   }
 };
 
-__global__ __launch_bounds__(256) void decode_rows_kernel(const uint16_t* __restrict__ matrix,
-                                                          const DecodeJob job,
-                                                          uint8_t* __restrict__ rows_out,
-                                                          uint32_t* __restrict__ tables_out,
-                                                          int32_t* __restrict__ status) {
+// One decode solve by one workgroup: [C | B | I] Gauss-Jordan in `smem` ([e][cols]
+// work matrix, then [n_targets][k] rows), then the rows' bytes and v_perm tables.
+__device__ __forceinline__ void decode_rows_body(const uint16_t* __restrict__ matrix,
+                                                 const DecodeJob& job, uint8_t* __restrict__ rows_out,
+                                                 uint32_t* __restrict__ tables_out,
+                                                 int32_t* __restrict__ status, uint16_t* smem) {
   __shared__ uint8_t lg[256];
   __shared__ uint8_t ex[512];
-  extern __shared__ uint16_t smem[];  // [e][cols] work matrix, then [n_targets][k] rows
   const int tid = threadIdx.x;
   for (int i = tid; i < 256; i += blockDim.x) lg[i] = job.field ? job.field[i] : kGf8.log[i];
   for (int i = tid; i < 512; i += blockDim.x) ex[i] = job.field ? job.field[256 + i] : kGf8.exp[i];
@@ -271,15 +271,46 @@ __global__ __launch_bounds__(256) void decode_rows_kernel(const uint16_t* __rest
   if (tid == 0) *status = RSGPU_OK;
 }
 
-hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
-                              uint32_t* d_tables, int32_t* d_status, hipStream_t s) {
+__global__ __launch_bounds__(256) void decode_rows_kernel(const uint16_t* __restrict__ matrix,
+                                                          const DecodeJob job,
+                                                          uint8_t* __restrict__ rows_out,
+                                                          uint32_t* __restrict__ tables_out,
+                                                          int32_t* __restrict__ status) {
+  extern __shared__ uint16_t smem[];
+  decode_rows_body(matrix, job, rows_out, tables_out, status, smem);
+}
+
+// Many independent solves in one launch: workgroup b runs jobs[b].
+__global__ __launch_bounds__(256) void decode_rows_multi_kernel(const uint16_t* __restrict__ matrix,
+                                                                const DecodeMultiJob* __restrict__ jobs,
+                                                                int32_t* __restrict__ status) {
+  extern __shared__ uint16_t smem[];
+  const DecodeMultiJob& J = jobs[blockIdx.x];
+  decode_rows_body(matrix, J.job, J.rows, J.tables, status, smem);
+}
+
+size_t decode_rows_lds(const DecodeJob& job) {
   int p = 0;
   while (p < job.k && job.valid[p] < job.k) ++p;
-  int e = job.k - p;
-  if (e > kMaxSolve) return hipErrorInvalidValue;
-  size_t lds = ((size_t)e * (e + p + e) + (size_t)job.n_targets * job.k) * sizeof(uint16_t);
+  const size_t e = (size_t)(job.k - p);
+  if (e > (size_t)kMaxSolve) return 0;
+  return (e * (e + p + e) + (size_t)job.n_targets * job.k) * sizeof(uint16_t);
+}
+
+hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
+                              uint32_t* d_tables, int32_t* d_status, hipStream_t s) {
+  const size_t lds = decode_rows_lds(job);
+  if (!lds) return hipErrorInvalidValue;
   hipLaunchKernelGGL(decode_rows_kernel, dim3(1), dim3(256), lds, s, d_matrix, job, d_rows,
                      d_tables, d_status);
+  return hipGetLastError();
+}
+
+hipError_t launch_decode_rows_multi(const uint16_t* d_matrix, const DecodeMultiJob* d_jobs, int n,
+                                    size_t lds, int32_t* d_status, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(decode_rows_multi_kernel, dim3((unsigned)n), dim3(256), lds, s, d_matrix, d_jobs,
+                     d_status);
   return hipGetLastError();
 }
 

@@ -23,10 +23,9 @@ DeviceBuffer::~DeviceBuffer() {
   if (p) (void)hipFree(p);
 }
 DecodePool::~DecodePool() {
-  for (auto& sl : all) {
+  for (auto& sl : all)
     if (sl.ready) (void)hipEventDestroy(sl.ready);
-    (void)hipFree(sl.mem);
-  }
+  for (void* p : slabs) (void)hipFree(p);
 }
 
 hipError_t DecodePool::acquire(Slot& out) {
@@ -42,16 +41,22 @@ hipError_t DecodePool::acquire(Slot& out) {
     free_slots.pop_back();
     return hipSuccess;
   }
-  Slot sl{nullptr, nullptr};
-  hipError_t e = hipMalloc(&sl.mem, slot_bytes);
+  // at most 64 MiB per slab (large GF(2^16) codecs have multi-MB slots)
+  const size_t n_slots = std::max<size_t>(1, std::min(next_slab, (size_t(64) << 20) / slot_bytes));
+  void* slab = nullptr;
+  hipError_t e = hipMalloc(&slab, n_slots * slot_bytes);
   if (e != hipSuccess) return e;
-  e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
-  if (e != hipSuccess) {
-    (void)hipFree(sl.mem);
-    return e;
+  slabs.push_back(slab);
+  for (size_t i = 0; i < n_slots; ++i) {
+    Slot sl{(uint8_t*)slab + i * slot_bytes, nullptr};
+    e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
+    if (e != hipSuccess) return e;  // the slots made so far stay usable
+    all.push_back(sl);
+    free_slots.push_back(sl);
   }
-  all.push_back(sl);
-  out = sl;
+  next_slab = std::min(next_slab * 2, kMaxSlab);
+  out = free_slots.back();
+  free_slots.pop_back();
   return hipSuccess;
 }
 
@@ -581,6 +586,62 @@ static bool custom_solvable(const Codec& c, const Plan& plan) {
   return gauss_jordan(F, w, e, e);
 }
 
+// Inserts a new entry at the front of the LRU, evicting from the back to the
+// capacity and byte budget (with c.mu held).  A victim still read by queued
+// kernels stays alive through the shared_ptr of the caller that launched them;
+// its slot retires when the last owner lets go.
+static void cache_insert(const Codec& c, const std::string& key, const std::shared_ptr<DecodeEntry>& e) {
+  if (!c.use_cache) return;
+  const size_t cap = c.cache_capacity;
+  while (!c.lru.empty() && (c.index.size() >= cap || c.cache_bytes + e->bytes > c.cache_max_bytes)) {
+    auto& victim = c.lru.back();
+    c.cache_bytes -= victim.second->bytes;
+    c.index.erase(victim.first);
+    c.lru.pop_back();
+  }
+  if (e->bytes <= c.cache_max_bytes) {
+    c.lru.emplace_front(key, e);
+    c.index[key] = c.lru.begin();
+    c.cache_bytes += e->bytes;
+  }
+}
+
+// The one-workgroup solve's job for `plan` (k <= 256, at most 256 targets).
+static void fill_decode_job(const Codec& c, const Plan& plan, DecodeJob& job) {
+  job.k = (int)c.k;
+  job.m = (int)c.m;
+  job.n_targets = (int)plan.targets.size();
+  job.tower = c.field == RSGPU_FIELD_GF16 ? 1 : 0;
+  job.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
+  job.one = c.ftab ? c.ftab->one : 1;
+  for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)plan.valid[i];
+  for (size_t i = 0; i < plan.targets.size(); ++i) job.targets[i] = (uint16_t)plan.targets[i];
+}
+
+// A decode entry for `plan` on a pool slot (with c.mu held); its solve is
+// enqueued on s by the caller, who records slot.ready behind it.
+static int new_decode_entry(const Codec& c, const Plan& plan, const std::string& key, hipStream_t s,
+                            std::shared_ptr<DecodeEntry>& out) {
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
+  const size_t max_e = std::min<size_t>(std::min(c.k, c.m), kMaxSolve);
+  if (!c.decode_pool) {  // one slot size per codec: tables and rows of m targets, C^-1
+    c.decode_pool = std::make_shared<DecodePool>();
+    c.decode_pool->slot_bytes = (max_tab + c.m * c.k * 2 + max_e * max_e * 2 + 255) / 256 * 256;
+  }
+  auto e = std::make_shared<DecodeEntry>();
+  e->valid = plan.valid;
+  e->targets = plan.targets;
+  HIP_TRY(c.decode_pool->acquire(e->slot));
+  e->pool = c.decode_pool;
+  e->tables = (uint32_t*)e->slot.mem;
+  e->rows = (uint8_t*)e->slot.mem + max_tab;
+  e->bytes = c.decode_pool->slot_bytes + key.size();
+  e->solve_stream = s;
+  out = std::move(e);
+  return RSGPU_OK;
+}
+
 // Orders a read of `e` on stream s after its solve when the solve was enqueued
 // on another stream and has not completed yet.
 static int wait_ready(const DecodeEntry& e, hipStream_t s) {
@@ -605,21 +666,9 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
   if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable(c, plan)) return RSGPU_INVALID_CUSTOM_MATRIX;
   const bool tower = c.field == RSGPU_FIELD_GF16;
   const size_t nt = plan.targets.size();
-  const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
-  const size_t max_e = std::min<size_t>(std::min(c.k, c.m), kMaxSolve);
-  if (!c.decode_pool) {  // one slot size per codec: tables and rows of m targets, C^-1
-    c.decode_pool = std::make_shared<DecodePool>();
-    c.decode_pool->slot_bytes = (max_tab + c.m * c.k * 2 + max_e * max_e * 2 + 255) / 256 * 256;
-  }
-  auto e = std::make_shared<DecodeEntry>();
-  e->valid = plan.valid;
-  e->targets = plan.targets;
-  HIP_TRY(c.decode_pool->acquire(e->slot));
-  e->pool = c.decode_pool;
-  e->tables = (uint32_t*)e->slot.mem;
-  e->rows = (uint8_t*)e->slot.mem + max_tab;
-  e->bytes = c.decode_pool->slot_bytes + key.size();
-  e->solve_stream = s;
+  std::shared_ptr<DecodeEntry> e;
+  int rc = new_decode_entry(c, plan, key, s, e);
+  if (rc) return rc;
   int32_t* st = thread_flag(c.device);
   if (!st) return RSGPU_ERR_OUT_OF_MEMORY;
   hipError_t he;
@@ -643,14 +692,7 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
                            e->tables, st + 1, s);
   } else {
     DecodeJob job{};
-    job.k = (int)c.k;
-    job.m = (int)c.m;
-    job.n_targets = (int)nt;
-    job.tower = tower ? 1 : 0;
-    job.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
-    job.one = c.ftab ? c.ftab->one : 1;
-    for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)plan.valid[i];
-    for (size_t i = 0; i < nt; ++i) job.targets[i] = (uint16_t)plan.targets[i];
+    fill_decode_job(c, plan, job);
     he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, e->rows, e->tables, st + 1, s);
   }
   if (he != hipSuccess) {
@@ -659,22 +701,88 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
   }
   HIP_TRY(hipEventRecord(e->slot.ready, s));
   out = e;
-  if (!c.use_cache) return RSGPU_OK;
-  size_t cap = c.cache_capacity;
-  while (!c.lru.empty() && (c.index.size() >= cap || c.cache_bytes + e->bytes > c.cache_max_bytes)) {
-    auto& victim = c.lru.back();
-    // A victim still read by queued kernels keeps its buffers alive through
-    // the shared_ptr held by the caller that launched them; the last owner
-    // frees it after synchronising on its event.
-    c.cache_bytes -= victim.second->bytes;
-    c.index.erase(victim.first);
-    c.lru.pop_back();
+  cache_insert(c, key, e);
+  return RSGPU_OK;
+}
+
+// ---------------------------------------------- decode solves ahead of a batch
+PresolveScratch::~PresolveScratch() {
+  if (done) (void)hipEventSynchronize(done);
+  if (pinned) (void)hipHostFree(pinned);
+  if (done) (void)hipEventDestroy(done);
+}
+
+int PresolveScratch::ensure(size_t bytes) {
+  if (bytes <= cap) return RSGPU_OK;
+  if (pinned) (void)hipHostFree(pinned);
+  pinned = nullptr;
+  cap = 0;
+  HIP_TRY(hipHostMalloc((void**)&pinned, bytes, hipHostMallocDefault));
+  HIP_TRY(dev.alloc(bytes));
+  cap = bytes;
+  return RSGPU_OK;
+}
+
+int presolve_decode(const std::vector<PresolveCall>& calls, hipStream_t s, PresolveScratch& scr) {
+  std::vector<const Codec*> order;  // codecs in first-seen order
+  for (auto& cl : calls)
+    if (std::find(order.begin(), order.end(), cl.codec) == order.end()) order.push_back(cl.codec);
+  if (!scr.done) HIP_TRY(hipEventCreateWithFlags(&scr.done, hipEventDisableTiming));
+  HIP_TRY(hipEventSynchronize(scr.done));  // the previous call's jobs were read
+  // room for every call's job up front: no reallocation while copies are queued
+  int rc0 = scr.ensure(calls.size() * sizeof(DecodeMultiJob) + order.size() * 256);
+  if (rc0) return rc0;
+  size_t off = 0;
+  for (const Codec* cp : order) {
+    const Codec& c = *cp;
+    if (c.family != RSGPU_FAMILY_CLASSIC || c.k > 256) continue;
+    std::lock_guard<std::mutex> g(c.mu);  // held until the new entries' events are recorded
+    std::vector<DecodeMultiJob> jobs;
+    std::vector<std::pair<std::string, std::shared_ptr<DecodeEntry>>> made;
+    size_t lds = 0;
+    for (auto& cl : calls) {
+      if (cl.codec != cp) continue;
+      Plan plan;
+      if (make_plan(c, cl.shards, cl.n, 0, nullptr, 0, plan) || plan.all_present ||
+          plan.targets.empty() || plan.targets.size() > 256)
+        continue;  // errors are reported by the call itself
+      std::string key = plan_key(c, cl.shards, plan);
+      if (c.index.count(key)) continue;
+      bool dup = false;
+      for (auto& mk : made) dup |= mk.first == key;
+      if (dup) continue;
+      if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable(c, plan)) continue;
+      DecodeMultiJob J{};
+      fill_decode_job(c, plan, J.job);
+      const size_t l = decode_rows_lds(J.job);
+      if (!l) continue;
+      lds = std::max(lds, l);
+      std::shared_ptr<DecodeEntry> e;
+      int rc = new_decode_entry(c, plan, key, s, e);
+      if (rc) return rc;
+      J.rows = e->rows;
+      J.tables = e->tables;
+      jobs.push_back(J);
+      made.emplace_back(std::move(key), std::move(e));
+    }
+    if (jobs.empty()) continue;
+    const size_t bytes = jobs.size() * sizeof(DecodeMultiJob);
+    std::memcpy(scr.pinned + off, jobs.data(), bytes);
+    HIP_TRY(hipMemcpyAsync((uint8_t*)scr.dev.p + off, scr.pinned + off, bytes, hipMemcpyHostToDevice, s));
+    int32_t* st = thread_flag(c.device);
+    if (!st) return RSGPU_ERR_OUT_OF_MEMORY;
+    HIP_TRY(launch_decode_rows_multi((const uint16_t*)c.d_matrix.p,
+                                     (const DecodeMultiJob*)((uint8_t*)scr.dev.p + off),
+                                     (int)jobs.size(), lds, st + 1, s));
+    off += (bytes + 255) / 256 * 256;
+    for (auto& mk : made) {
+      HIP_TRY(hipEventRecord(mk.second->slot.ready, s));
+      c.requests++;
+      c.misses++;
+      cache_insert(c, mk.first, mk.second);
+    }
   }
-  if (e->bytes <= c.cache_max_bytes) {
-    c.lru.emplace_front(key, e);
-    c.index[key] = c.lru.begin();
-    c.cache_bytes += e->bytes;
-  }
+  HIP_TRY(hipEventRecord(scr.done, s));
   return RSGPU_OK;
 }
 

@@ -106,6 +106,16 @@ hipError_t launch_compare_rows(const CompareArgs& c, hipStream_t s);
 
 hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
                               uint32_t* d_tables, int32_t* d_status, hipStream_t s);
+// Dynamic LDS of one solve (0: more than kMaxSolve missing data rows).
+size_t decode_rows_lds(const DecodeJob& job);
+// Independent solves of one codec in one launch (a job array in device memory).
+struct DecodeMultiJob {
+  DecodeJob job;
+  uint8_t* rows;
+  uint32_t* tables;
+};
+hipError_t launch_decode_rows_multi(const uint16_t* d_matrix, const DecodeMultiJob* d_jobs, int n,
+                                    size_t lds, int32_t* d_status, hipStream_t s);
 
 // Decode rows of codecs beyond DecodeJob's 256-entry lists (classic GF(2^16) with
 // k > 256 or more than 256 targets): C^-1 of the missing-data system in one
@@ -141,17 +151,22 @@ struct DeviceBuffer {
 // Device slots for decode entries of one codec, each with its own event.  A
 // released slot retires: kernels already queued may still read it.  Retired
 // slots become free again after one device-wide synchronisation, taken only
-// when no free slot is left and at least kRecycleBatch have retired, so a
-// stream of cache misses costs neither a hipMalloc nor a synchronising hipFree
-// per entry.
+// when no free slot is left and at least kRecycleBatch have retired.  New
+// slots come in slabs of one hipMalloc each (doubling up to kMaxSlab slots and
+// 64 MiB), so
+// a stream of cache misses costs neither a hipMalloc nor a synchronising
+// hipFree per entry.
 struct DecodePool {
   static constexpr size_t kRecycleBatch = 64;
+  static constexpr size_t kMaxSlab = 256;
   struct Slot {
     void* mem;
     hipEvent_t ready;
   };
   std::mutex mu;
   size_t slot_bytes = 0;
+  size_t next_slab = 8;  // slots in the next slab
+  std::vector<void*> slabs;
   std::vector<Slot> all, free_slots, retired;
   ~DecodePool();
   hipError_t acquire(Slot& out);
@@ -220,6 +235,31 @@ struct Codec {
 };
 
 int words_per_coef(int tower);  // kTabWords or 4 * kTabWords
+
+// Staging for presolve_decode's job arrays, owned by the caller of one stream
+// and reused across calls; `done` is recorded behind the launches that read it.
+struct PresolveScratch {
+  uint8_t* pinned = nullptr;
+  size_t cap = 0;
+  DeviceBuffer dev;
+  hipEvent_t done = nullptr;
+  ~PresolveScratch();
+  int ensure(size_t bytes);
+};
+
+// A reconstruct call to be solved ahead: its codec and its k + m slots.
+struct PresolveCall {
+  const Codec* codec;
+  const rsgpu_slot* shards;
+  size_t n;
+};
+
+// Solves, on stream s, the decode rows of every pattern among `calls` (goal: all
+// missing shards) that the codec's cache does not hold yet: one launch per codec
+// with one workgroup per pattern.  The entries enter the cache as misses would;
+// calls then find them there.  Codecs outside the one-workgroup solve (Leopard,
+// more than 256 data shards) are left to the calls themselves.
+int presolve_decode(const std::vector<PresolveCall>& calls, hipStream_t s, PresolveScratch& scr);
 
 // thread-local last HIP error
 void set_hip_error(hipError_t e, const char* where);

@@ -95,6 +95,7 @@ struct BatchSlot {
   int32_t* h_flags = nullptr;
   std::vector<uint64_t> tags;
   bool busy = false;
+  PresolveScratch presolve;  // job arrays of the decode solves ahead of this slot's batches
 };
 
 }  // namespace
@@ -221,9 +222,20 @@ int launch(rsgpu_server* S, uint64_t t) {
     if (rc) return rc;
     B.tags.clear();
     int n_flags = 0;
-    while (!S->pending.empty() && B.tags.size() < limit) {
-      Req& q = S->reqs.at(S->pending.front());
+    std::vector<uint64_t> taken;
+    while (!S->pending.empty() && taken.size() < limit) {
+      taken.push_back(S->pending.front());
       S->pending.pop_front();
+    }
+    // the new erasure patterns among them, solved together ahead of the adds
+    std::vector<PresolveCall> calls;
+    for (uint64_t tag : taken) {
+      const Req& q = S->reqs.at(tag);
+      if (q.r.op == 2) calls.push_back({&q.r.codec->c, q.slots.data(), q.slots.size()});
+    }
+    if (!calls.empty() && (rc = presolve_decode(calls, B.stream, B.presolve))) return rc;
+    for (uint64_t tag : taken) {
+      Req& q = S->reqs.at(tag);
       const rsgpu_codec* c = q.r.codec;
       if (q.r.op == 0) {
         rc = rsgpu_batch_add_encode(B.batch, c, q.sl.data(), q.sl.size());

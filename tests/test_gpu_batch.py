@@ -196,3 +196,49 @@ def test_ragged_workload_plain_equals_batched():
         _, omat = O.build_matrix(O.GF8, O.VANDERMONDE, k, m)
         exp = O.encode(O.GF8, omat, k, m, sh[:k])
         assert all((sh[k + j] == exp[j]).all() for j in range(m)), (k, m)
+
+
+def test_growth_and_streams():
+    """A batch grown while an earlier run is still queued, rerun on another
+    stream without a new upload, then rebuilt: every run equals the oracle."""
+    announce("batch growth across streams")
+    _, codecs, mats, items = _mixed_batch(21, 120, max_len=1 << 14, tails=False)
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    batch = rsgpu.Batch()
+    outs = []
+
+    def add(it):
+        k, m, n = it["k"], it["m"], it["length"]
+        sh = [torch.from_numpy(d).cuda() for d in it["data"]] + \
+             [torch.zeros(n, dtype=torch.uint8, device="cuda") for _ in range(m)]
+        batch.add_encode(codecs[(k, m)], sh)
+        outs.append((sh, it))
+
+    def check():
+        for sh, it in outs:
+            for j in range(it["m"]):
+                assert (to_host(sh[it["k"] + j]) == it["parity"][j]).all()
+
+    torch.cuda.synchronize()
+    for it in items[:4]:
+        add(it)
+    batch.run(stream=sa)          # small upload on stream A
+    for it in items[4:]:
+        add(it)                   # the tables outgrow their buffers
+    batch.run(stream=sa)
+    torch.cuda.synchronize()
+    check()
+    for sh, it in outs:
+        for t in sh[it["k"]:]:
+            t.zero_()
+    torch.cuda.synchronize()
+    batch.run(stream=sb)          # no new upload: stream B waits for A's copies
+    torch.cuda.synchronize()
+    check()
+    batch.clear()
+    outs.clear()
+    for it in items[:50]:
+        add(it)
+    batch.run(stream=sb)
+    torch.cuda.synchronize()
+    check()

@@ -91,3 +91,24 @@ def test_batch_wide_stripes_with_tails():
     for rs, sh, exp, k, m in cases:
         for j in range(m):
             assert (to_host(sh[k + j]) == exp[j]).all(), (k, m, j)
+
+
+def test_gf16_field_order_restriction():
+    """correct_field_order_restriction (src/tests/galois_16.rs:91-102)."""
+    announce("gf16 field order")
+    order = 1 << 16
+    for args in ((order, 1), (1, order)):
+        with pytest.raises(rsgpu.RsError) as ei:
+            rsgpu.ReedSolomon(*args, field="gf16")
+        assert ei.value.name == "TooManyShards"
+    rs = rsgpu.ReedSolomon(1, order - 1, field="gf16")
+    n = 64
+    data = torch.arange(n, dtype=torch.uint8, device="cuda")
+    par = torch.zeros((order - 1, n), dtype=torch.uint8, device="cuda")
+    rs.encode_sep([data], list(par[:: 1]))
+    # one data shard: every parity row is that shard times its coefficient, and
+    # the first Vandermonde parity row of a 1-data code is the identity
+    rc, mat = O.build_matrix(O.GF16, O.VANDERMONDE, 1, 3)
+    exp = O.encode(O.GF16, mat, 1, 3, [data.cpu().numpy()])
+    for j in range(3):
+        assert (to_host(par[j]) == exp[j]).all(), j
