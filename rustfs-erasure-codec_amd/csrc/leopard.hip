@@ -325,16 +325,29 @@ struct Word<true> {
 };
 
 // G consecutive words of one LDS row (G = 1, 2 or 4; the row stride C is a
-// multiple of G).  G = 4 moves them as 16-byte LDS accesses (consecutive lanes
-// take consecutive 16-byte slots: conflict free).
+// multiple of G), at word index idx of a buffer.  G = 4 moves them as 16-byte
+// LDS accesses.  A GF16 group of four words is 32 bytes, so its two halves are
+// stored apart: groups are laid out in blocks of 16 (512 bytes), the words 0-1
+// of the 16 groups first, then their words 2-3.  Consecutive lanes (consecutive
+// groups) then read consecutive 16-byte slots in both accesses: conflict free.
+// Buffers in this layout start on a block and span whole blocks
+// (gf16_lds_dwords).
+__host__ __device__ constexpr uint32_t gf16_lds_dwords(uint32_t words) {  // whole 16-group blocks
+  return (words / 4 + 15) / 16 * 128;
+}
+__device__ __forceinline__ uint32_t gf16_half0(uint32_t idx) {
+  const uint32_t gi = idx >> 2;
+  return ((gi & ~15u) << 3) + ((gi & 15u) << 2);
+}
 template <bool G16, int G>
 __device__ __forceinline__ void lds_getv(const uint32_t* lds, uint32_t idx, Word<G16> (&w)[G]) {
   if constexpr (G == 4 && !G16) {
     uint4 t = *reinterpret_cast<const uint4*>(lds + idx);
     w[0].v = t.x, w[1].v = t.y, w[2].v = t.z, w[3].v = t.w;
   } else if constexpr (G == 4 && G16) {
-    uint4 t0 = *reinterpret_cast<const uint4*>(lds + 2 * idx);
-    uint4 t1 = *reinterpret_cast<const uint4*>(lds + 2 * idx + 4);
+    const uint32_t d0 = gf16_half0(idx);
+    uint4 t0 = *reinterpret_cast<const uint4*>(lds + d0);
+    uint4 t1 = *reinterpret_cast<const uint4*>(lds + d0 + 64);
     w[0].lo = t0.x, w[0].hi = t0.y, w[1].lo = t0.z, w[1].hi = t0.w;
     w[2].lo = t1.x, w[2].hi = t1.y, w[3].lo = t1.z, w[3].hi = t1.w;
   } else if constexpr (G == 2 && !G16) {
@@ -360,8 +373,9 @@ __device__ __forceinline__ void lds_putv(uint32_t* lds, uint32_t idx, const Word
   if constexpr (G == 4 && !G16) {
     *reinterpret_cast<uint4*>(lds + idx) = make_uint4(w[0].v, w[1].v, w[2].v, w[3].v);
   } else if constexpr (G == 4 && G16) {
-    *reinterpret_cast<uint4*>(lds + 2 * idx) = make_uint4(w[0].lo, w[0].hi, w[1].lo, w[1].hi);
-    *reinterpret_cast<uint4*>(lds + 2 * idx + 4) = make_uint4(w[2].lo, w[2].hi, w[3].lo, w[3].hi);
+    const uint32_t d0 = gf16_half0(idx);
+    *reinterpret_cast<uint4*>(lds + d0) = make_uint4(w[0].lo, w[0].hi, w[1].lo, w[1].hi);
+    *reinterpret_cast<uint4*>(lds + d0 + 64) = make_uint4(w[2].lo, w[2].hi, w[3].lo, w[3].hi);
   } else if constexpr (G == 2 && !G16) {
     *reinterpret_cast<uint2*>(lds + idx) = make_uint2(w[0].v, w[1].v);
   } else if constexpr (G == 2 && G16) {
@@ -596,7 +610,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
   const uint32_t C = a.C, m = a.m, ng = C / G;
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   uint32_t* A = lds;
-  uint32_t* Bv = lds + (size_t)m * C * (G16 ? 2 : 1);
+  uint32_t* Bv = lds + (G16 && G == 4 ? gf16_lds_dwords(m * C) : m * C * (G16 ? 2u : 1u));
   for (uint32_t g = 0; g < a.n_groups; ++g) {
     const LeoGroup Gp = a.groups[g];
     uint32_t* dst = g == 0 ? A : Bv;
@@ -811,9 +825,14 @@ static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopar
 // `want` a dispatched variant asks for.
 static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0) {
   const uint32_t wb = s.gf16 ? 8 : 4;
-  uint64_t C = s.lds_limit / ((uint64_t)rows * wb);
+  const uint64_t usable = s.lds_limit - 1024;  // room for the GF16 layout's block padding
+  uint64_t C = usable / ((uint64_t)rows * wb);
   const uint64_t cap = std::max<uint64_t>(64, 4096 / rows);
   if (C > cap) C = cap;
+  // Default: two workgroups per CU when that still leaves 8 words a lane -- a
+  // lone workgroup (one wave per SIMD) leaves its table loads' latency exposed.
+  const uint64_t half = usable / 2 / ((uint64_t)rows * wb);
+  if (!want && C > half && half >= 8) C = half;
   if (want && C > want) C = want;
   if (C < 1) C = 1;
   if (C > words) C = (uint32_t)words;
@@ -1039,7 +1058,8 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   a.flag = flag;
   a.mode = (uint32_t)mode;
   a.vec = all_aligned16(ptrs) ? 1 : 0;
-  const size_t lds = (size_t)2 * st.m * C * (st.gf16 ? 8 : 4);
+  const size_t lds = st.gf16 && C % 4 == 0 ? (size_t)2 * gf16_lds_dwords(st.m * C) * 4
+                                            : (size_t)2 * st.m * C * (st.gf16 ? 8 : 4);
   const unsigned grid = (unsigned)((words + C - 1) / C);
   if (st.gf16)
     hipLaunchKernelGGL(leo_encode_kernel<true>, dim3(grid), dim3(256), lds, s, a);
@@ -1188,7 +1208,8 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   a.fft_layer0 = e->n_ifft;
   a.fft_layers = e->n_fft;
   a.vec = all_aligned16(ptrs) ? 1 : 0;
-  const size_t lds = (size_t)st.n * C * (st.gf16 ? 8 : 4);
+  const size_t lds = st.gf16 && C % 4 == 0 ? (size_t)gf16_lds_dwords(st.n * C) * 4
+                                            : (size_t)st.n * C * (st.gf16 ? 8 : 4);
   const unsigned grid = (unsigned)((words + C - 1) / C);
   if (st.gf16)
     hipLaunchKernelGGL(leo_decode_kernel<true>, dim3(grid), dim3(256), lds, s, a);
