@@ -294,6 +294,8 @@ struct LeoState {
 };
 
 // ---------------------------------------------------------------- kernels
+constexpr uint32_t kLeoInline = 256;
+
 struct LeoArgs {
   const LeoLayer* layers;
   const LeoBlock* blocks;
@@ -311,7 +313,20 @@ struct LeoArgs {
   int32_t* flag;               // verify
   uint32_t mode;               // 0 encode, 1 verify
   uint32_t vec;                // every shard pointer 16-byte aligned
+  // Up to kLeoInline shards: pointers and flags travel in the kernel arguments
+  // (inline_args = 1) instead of through the staged copy above, which saves a
+  // host-to-device copy per call.
+  uint32_t inline_args;
+  const uint8_t* iptr[kLeoInline];
+  uint8_t ipresent[kLeoInline];
 };
+
+__device__ __forceinline__ const uint8_t* shard_ptr(const LeoArgs& a, uint32_t i) {
+  return a.inline_args ? a.iptr[i] : a.ptrs[i];
+}
+__device__ __forceinline__ uint8_t shard_flags(const LeoArgs& a, uint32_t i) {
+  return a.inline_args ? a.ipresent[i] : a.present[i];
+}
 
 template <bool G16>
 struct Word;
@@ -618,7 +633,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
       const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
       Word<G16> v[G];
       if (lane < Gp.mtrunc) {
-        gloadv<G16, G>(a.ptrs[Gp.first + lane], w0 + wq, a.words, a.vec != 0, v);
+        gloadv<G16, G>(shard_ptr(a, Gp.first + lane), w0 + wq, a.words, a.vec != 0, v);
       } else {
 #pragma unroll
         for (int q = 0; q < G; ++q) v[q] = Word<G16>{};
@@ -644,7 +659,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
   for (uint32_t it = threadIdx.x; it < a.parity * ng; it += blockDim.x) {
     const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
     if (w0 + wq >= a.words) continue;
-    uint8_t* out = const_cast<uint8_t*>(a.ptrs[a.k + lane]);
+    uint8_t* out = const_cast<uint8_t*>(shard_ptr(a, a.k + lane));
     Word<G16> v[G];
     lds_getv<G16, G>(A, lane * C + wq, v);
     if (a.mode == 0) {
@@ -681,8 +696,8 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds)
     int shard = -1;
     if (lane < a.parity) shard = (int)(a.k + lane);
     else if (lane >= m && lane < m + a.k) shard = (int)(lane - m);
-    if (shard >= 0 && (a.present[shard] & 1)) {
-      gloadv<G16, G>(a.ptrs[shard], w0 + wq, a.words, a.vec != 0, v);
+    if (shard >= 0 && (shard_flags(a, shard) & 1)) {
+      gloadv<G16, G>(shard_ptr(a, shard), w0 + wq, a.words, a.vec != 0, v);
       const Tab<G16> te = load_tab<G16>(a.err_tabs + (size_t)lane * TW);
 #pragma unroll
       for (int q = 0; q < G; ++q) v[q] = wmul<G16>(te, v[q]);
@@ -720,14 +735,14 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds)
   run_layers<G16, G>(lds, a.fft_layer0, a.fft_layers, a);
   for (uint32_t it = threadIdx.x; it < (a.k + a.parity) * ng; it += blockDim.x) {
     const uint32_t s = it / ng, wq = (it - s * ng) * G;
-    if ((a.present[s] & 3) != 2 || w0 + wq >= a.words) continue;  // missing and required
+    if ((shard_flags(a, s) & 3) != 2 || w0 + wq >= a.words) continue;  // missing and required
     const uint32_t lane = s >= a.k ? s - a.k : m + s;
     const Tab<G16> ti = load_tab<G16>(a.inv_tabs + (size_t)lane * TW);
     Word<G16> v[G];
     lds_getv<G16, G>(lds, lane * C + wq, v);
 #pragma unroll
     for (int q = 0; q < G; ++q) v[q] = wmul<G16>(ti, v[q]);
-    gstorev<G16, G>(const_cast<uint8_t*>(a.ptrs[s]), w0 + wq, a.words, a.vec != 0, v);
+    gstorev<G16, G>(const_cast<uint8_t*>(shard_ptr(a, s)), w0 + wq, a.words, a.vec != 0, v);
   }
 }
 
@@ -969,7 +984,23 @@ static int stage_args(const std::vector<const uint8_t*>& ptrs, const std::vector
   return RSGPU_OK;
 }
 
+// Puts the pointer table (and flags) of a launch into `a`: inline when it fits,
+// else staged on stream s (bytes > 0 then, to be released after the launch).
+static int pass_args(LeoArgs& a, const std::vector<const uint8_t*>& ptrs,
+                     const std::vector<uint8_t>* flags, hipStream_t s, size_t* off, size_t* bytes) {
+  *off = *bytes = 0;
+  if (ptrs.size() <= kLeoInline) {
+    a.inline_args = 1;
+    for (size_t i = 0; i < ptrs.size(); ++i) a.iptr[i] = ptrs[i];
+    if (flags)
+      for (size_t i = 0; i < flags->size() && i < kLeoInline; ++i) a.ipresent[i] = (*flags)[i];
+    return RSGPU_OK;
+  }
+  return stage_args(ptrs, flags, s, &a.ptrs, flags ? &a.present : nullptr, off, bytes);
+}
+
 static int release_args(size_t off, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return RSGPU_OK;  // passed inline
   hipError_t e = arg_ring().commit(off, bytes, s);
   if (e != hipSuccess) {
     set_hip_error(e, "leopard arg ring");
@@ -1041,7 +1072,7 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   const uint32_t C = tile_words(st, 2 * st.m, words, variant_tile(variant));
   LeoArgs a{};
   size_t off = 0, bytes = 0;
-  int rc = stage_args(ptrs, nullptr, s, &a.ptrs, nullptr, &off, &bytes);
+  int rc = pass_args(a, ptrs, nullptr, s, &off, &bytes);
   if (rc) return rc;
   a.layers = (const LeoLayer*)st.enc.layers.p;
   a.blocks = (const LeoBlock*)st.enc.blocks.p;
@@ -1190,7 +1221,7 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   for (size_t i = 0; i < total; ++i) ptrs[i] = shards[i].ptr;
   LeoArgs a{};
   size_t off = 0, bytes = 0;
-  if ((rc = stage_args(ptrs, &flags, s, &a.ptrs, &a.present, &off, &bytes))) return rc;
+  if ((rc = pass_args(a, ptrs, &flags, s, &off, &bytes))) return rc;
   a.layers = (const LeoLayer*)e->plan.layers.p;
   a.blocks = (const LeoBlock*)e->plan.blocks.p;
   a.tables = (const uint32_t*)e->plan.tables.p;
