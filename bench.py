@@ -58,6 +58,9 @@ def parse():
                     help="request rates (req/s) for the serving section ('' skips)")
     ap.add_argument("--leopard-only", action="store_true",
                     help="only the Leopard section (for profiling its kernels)")
+    ap.add_argument("--leopard-case", nargs=4, metavar=("FAMILY", "K", "M", "SHARD"),
+                    help="one Leopard shape (e.g. leopard_gf8 10 4 1048576): encode and "
+                         "reconstruct with m erasures, default kernel choice")
     ap.add_argument("--mfma-only", action="store_true",
                     help="only the int8 matrix-core vs VALU encode comparison")
     ap.add_argument("--roofline-only", action="store_true",
@@ -412,6 +415,21 @@ def main():
         sys.exit(2)
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.leopard_case:
+        import torch
+
+        import rsgpu
+        from rsgpu import dispatch as DSP
+
+        torch.cuda.init()
+        fam, k, m, S = args.leopard_case[0], *map(int, args.leopard_case[1:])
+        rs = rsgpu.ReedSolomon(k, m, codec_family=fam)
+        out = {}
+        for op, e in (("encode", 0), ("reconstruct", m)):
+            t = DSP.OpCase(torch, rs, fam, k, m, S, op, e, seed=1).time([-1])[-1]
+            out[op] = {"us": round(t, 1), "data_gbs": round(k * S / t / 1e3, 1)}
+        print(json.dumps({"leopard_case": {"family": fam, "k": k, "m": m, "shard": S, **out}}))
         return
     if args.mfma_only:
         import torch
