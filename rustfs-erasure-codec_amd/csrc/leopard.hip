@@ -475,61 +475,99 @@ __device__ __forceinline__ void inv2(Word<G16>& x, Word<G16>& y, const Tab<G16>&
   wxor(x, wmul<G16>(t, y));
 }
 
+typedef const __attribute__((address_space(4))) uint32_t* cword_t;
+
+template <bool G16>
+__device__ __forceinline__ Tab<G16> load_tab_c(cword_t p) {
+  Tab<G16> t;
+  if constexpr (G16) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) t.a[j][q] = p[j * kTabWords + q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) t.a[q] = p[q];
+  }
+  return t;
+}
+
+// The items of one layer.  UNI: every wave's 64 items lie in one block (the
+// layer's cnt * ng is a multiple of 64 and waves start at multiples of 64), so
+// the block and its twiddle tables are read once per wave with scalar loads.
+template <bool G16, int G, bool UNI>
+__device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, const LeoArgs& a) {
+  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
+  const uint32_t C = a.C, ng = a.C / G;
+  const uint32_t items = L.n_blocks * L.cnt * ng;
+  for (uint32_t it = threadIdx.x; it < items; it += blockDim.x) {
+    const uint32_t bf = it / ng, wq = (it - bf * ng) * G;
+    uint32_t blk = bf / L.cnt;
+    if constexpr (UNI) blk = __builtin_amdgcn_readfirstlane(blk);
+    LeoBlock B;
+    if constexpr (UNI) {
+      const cword_t q = (cword_t)(a.blocks + L.block0 + blk);
+      B.r = q[0], B.cnt = q[1], B.t01 = q[2], B.t23 = q[3], B.t02 = q[4];
+    } else {
+      B = a.blocks[L.block0 + blk];
+    }
+    const uint32_t i = B.r + (bf - blk * L.cnt);
+    const uint32_t d = L.dist;
+    auto tab = [&](uint32_t t) {
+      if constexpr (UNI) return load_tab_c<G16>((cword_t)a.tables + (size_t)t * TW);
+      else return load_tab<G16>(a.tables + (size_t)t * TW);
+    };
+    const Tab<G16> t01 = tab(B.t01);
+    if (L.radix == 2) {
+      Word<G16> x[G], y[G];
+      lds_getv<G16, G>(lds, i * C + wq, x);
+      lds_getv<G16, G>(lds, (i + d) * C + wq, y);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (L.inverse) inv2<G16>(x[g], y[g], t01);
+        else fwd2<G16>(x[g], y[g], t01);
+      }
+      lds_putv<G16, G>(lds, i * C + wq, x);
+      lds_putv<G16, G>(lds, (i + d) * C + wq, y);
+    } else {
+      const Tab<G16> t23 = tab(B.t23);
+      const Tab<G16> t02 = tab(B.t02);
+      Word<G16> x0[G], x1[G], x2[G], x3[G];
+      lds_getv<G16, G>(lds, i * C + wq, x0);
+      lds_getv<G16, G>(lds, (i + d) * C + wq, x1);
+      lds_getv<G16, G>(lds, (i + 2 * d) * C + wq, x2);
+      lds_getv<G16, G>(lds, (i + 3 * d) * C + wq, x3);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (L.inverse) {  // ifft_dit4 (ops.rs:556-586)
+          inv2<G16>(x0[g], x1[g], t01);
+          inv2<G16>(x2[g], x3[g], t23);
+          inv2<G16>(x1[g], x3[g], t02);
+          inv2<G16>(x0[g], x2[g], t02);
+        } else {  // fft_dit4 (ops.rs:516-548)
+          fwd2<G16>(x0[g], x2[g], t02);
+          fwd2<G16>(x1[g], x3[g], t02);
+          fwd2<G16>(x0[g], x1[g], t01);
+          fwd2<G16>(x2[g], x3[g], t23);
+        }
+      }
+      lds_putv<G16, G>(lds, i * C + wq, x0);
+      lds_putv<G16, G>(lds, (i + d) * C + wq, x1);
+      lds_putv<G16, G>(lds, (i + 2 * d) * C + wq, x2);
+      lds_putv<G16, G>(lds, (i + 3 * d) * C + wq, x3);
+    }
+  }
+}
+
 // Runs layers [l0, l0 + nl) over the LDS lanes.  An item is one butterfly on G
 // consecutive words, so its twiddle tables are loaded once for G words.
 template <bool G16, int G>
 __device__ __forceinline__ void run_layers(uint32_t* lds, uint32_t l0, uint32_t nl, const LeoArgs& a) {
-  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
-  const uint32_t C = a.C, ng = a.C / G;
+  const uint32_t ng = a.C / G;
   for (uint32_t l = l0; l < l0 + nl; ++l) {
     const LeoLayer L = a.layers[l];
-    const uint32_t items = L.n_blocks * L.cnt * ng;
-    for (uint32_t it = threadIdx.x; it < items; it += blockDim.x) {
-      const uint32_t bf = it / ng, wq = (it - bf * ng) * G;
-      const uint32_t blk = bf / L.cnt;
-      const LeoBlock B = a.blocks[L.block0 + blk];
-      const uint32_t i = B.r + (bf - blk * L.cnt);
-      const uint32_t d = L.dist;
-      const Tab<G16> t01 = load_tab<G16>(a.tables + (size_t)B.t01 * TW);
-      if (L.radix == 2) {
-        Word<G16> x[G], y[G];
-        lds_getv<G16, G>(lds, i * C + wq, x);
-        lds_getv<G16, G>(lds, (i + d) * C + wq, y);
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          if (L.inverse) inv2<G16>(x[g], y[g], t01);
-          else fwd2<G16>(x[g], y[g], t01);
-        }
-        lds_putv<G16, G>(lds, i * C + wq, x);
-        lds_putv<G16, G>(lds, (i + d) * C + wq, y);
-      } else {
-        const Tab<G16> t23 = load_tab<G16>(a.tables + (size_t)B.t23 * TW);
-        const Tab<G16> t02 = load_tab<G16>(a.tables + (size_t)B.t02 * TW);
-        Word<G16> x0[G], x1[G], x2[G], x3[G];
-        lds_getv<G16, G>(lds, i * C + wq, x0);
-        lds_getv<G16, G>(lds, (i + d) * C + wq, x1);
-        lds_getv<G16, G>(lds, (i + 2 * d) * C + wq, x2);
-        lds_getv<G16, G>(lds, (i + 3 * d) * C + wq, x3);
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          if (L.inverse) {  // ifft_dit4 (ops.rs:556-586)
-            inv2<G16>(x0[g], x1[g], t01);
-            inv2<G16>(x2[g], x3[g], t23);
-            inv2<G16>(x1[g], x3[g], t02);
-            inv2<G16>(x0[g], x2[g], t02);
-          } else {  // fft_dit4 (ops.rs:516-548)
-            fwd2<G16>(x0[g], x2[g], t02);
-            fwd2<G16>(x1[g], x3[g], t02);
-            fwd2<G16>(x0[g], x1[g], t01);
-            fwd2<G16>(x2[g], x3[g], t23);
-          }
-        }
-        lds_putv<G16, G>(lds, i * C + wq, x0);
-        lds_putv<G16, G>(lds, (i + d) * C + wq, x1);
-        lds_putv<G16, G>(lds, (i + 2 * d) * C + wq, x2);
-        lds_putv<G16, G>(lds, (i + 3 * d) * C + wq, x3);
-      }
-    }
+    if ((L.cnt * ng) % 64 == 0) layer_items<G16, G, true>(lds, L, a);
+    else layer_items<G16, G, false>(lds, L, a);
     __syncthreads();
   }
 }

@@ -233,7 +233,8 @@ int launch(rsgpu_server* S, uint64_t t) {
       const Req& q = S->reqs.at(tag);
       if (q.r.op == 2) calls.push_back({&q.r.codec->c, q.slots.data(), q.slots.size()});
     }
-    if (!calls.empty() && (rc = presolve_decode(calls, B.stream, B.presolve))) return rc;
+    // best effort: on failure each add solves its own pattern (and reports its error)
+    if (!calls.empty()) (void)presolve_decode(calls, B.stream, B.presolve);
     for (uint64_t tag : taken) {
       Req& q = S->reqs.at(tag);
       const rsgpu_codec* c = q.r.codec;
