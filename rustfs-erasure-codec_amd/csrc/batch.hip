@@ -332,6 +332,26 @@ static void retile(rsgpu_batch* b) {
   b->tune.rmax = rmax <= 4 ? 4 : 8;
 }
 
+namespace rsgpu {
+int batch_reserve(rsgpu_batch* b, size_t n_stripes, size_t n_ptrs) {
+  const size_t sb = std::max<size_t>(n_stripes, 1) * sizeof(BatchStripe);
+  const size_t pb = std::max<size_t>(n_ptrs, 1) * sizeof(void*);
+  if (b->d_stripes.bytes < sb && b->d_stripes.alloc(sb) != hipSuccess) return RSGPU_ERR_OUT_OF_MEMORY;
+  if (b->d_ptrs.bytes < pb && b->d_ptrs.alloc(pb) != hipSuccess) return RSGPU_ERR_OUT_OF_MEMORY;
+  if (b->h_stage_bytes < sb + pb) {
+    if (b->staged && hipEventSynchronize(b->staged) != hipSuccess) return RSGPU_ERR_HIP;
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    b->h_stage = nullptr;
+    b->h_stage_bytes = 0;
+    if (hipHostMalloc((void**)&b->h_stage, sb + pb, hipHostMallocDefault) != hipSuccess)
+      return RSGPU_ERR_OUT_OF_MEMORY;
+    b->h_stage_bytes = sb + pb;
+  }
+  b->dirty = true;
+  return RSGPU_OK;
+}
+}  // namespace rsgpu
+
 static int upload(rsgpu_batch* b, hipStream_t s) {
   if (!b->dirty) {  // a launch on another stream than the upload's waits for its copies
     if (b->staged && b->stage_stream != s && hipStreamWaitEvent(s, b->staged, 0) != hipSuccess)

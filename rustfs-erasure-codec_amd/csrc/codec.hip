@@ -28,6 +28,16 @@ DecodePool::~DecodePool() {
   for (void* p : slabs) (void)hipFree(p);
 }
 
+// A slot's event is made on its first use (a slab's worth at once would stall
+// the call that grows the pool).  Free and retired copies of a slot carry it.
+hipError_t DecodePool::ready_event(Slot& sl) {
+  if (sl.ready) return hipSuccess;
+  hipError_t e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
+  if (e != hipSuccess) return e;
+  all.push_back(sl);
+  return hipSuccess;
+}
+
 hipError_t DecodePool::acquire(Slot& out) {
   std::lock_guard<std::mutex> g(mu);
   if (free_slots.empty() && retired.size() >= kRecycleBatch) {
@@ -39,7 +49,7 @@ hipError_t DecodePool::acquire(Slot& out) {
   if (!free_slots.empty()) {
     out = free_slots.back();
     free_slots.pop_back();
-    return hipSuccess;
+    return ready_event(out);
   }
   // at most 64 MiB per slab (large GF(2^16) codecs have multi-MB slots)
   const size_t n_slots = std::max<size_t>(1, std::min(next_slab, (size_t(64) << 20) / slot_bytes));
@@ -47,17 +57,11 @@ hipError_t DecodePool::acquire(Slot& out) {
   hipError_t e = hipMalloc(&slab, n_slots * slot_bytes);
   if (e != hipSuccess) return e;
   slabs.push_back(slab);
-  for (size_t i = 0; i < n_slots; ++i) {
-    Slot sl{(uint8_t*)slab + i * slot_bytes, nullptr};
-    e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
-    if (e != hipSuccess) return e;  // the slots made so far stay usable
-    all.push_back(sl);
-    free_slots.push_back(sl);
-  }
+  for (size_t i = 0; i < n_slots; ++i) free_slots.push_back(Slot{(uint8_t*)slab + i * slot_bytes, nullptr});
   next_slab = std::min(next_slab * 2, kMaxSlab);
   out = free_slots.back();
   free_slots.pop_back();
-  return hipSuccess;
+  return ready_event(out);
 }
 
 void DecodePool::release(const Slot& s) {
@@ -709,17 +713,25 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
 PresolveScratch::~PresolveScratch() {
   if (done) (void)hipEventSynchronize(done);
   if (pinned) (void)hipHostFree(pinned);
+  for (uint8_t* p : old_pinned) (void)hipHostFree(p);
   if (done) (void)hipEventDestroy(done);
 }
 
 int PresolveScratch::ensure(size_t bytes) {
   if (bytes <= cap) return RSGPU_OK;
-  if (pinned) (void)hipHostFree(pinned);
+  const size_t want = std::max<size_t>(std::max(bytes, 2 * cap), 256 << 10);
+  if (pinned) old_pinned.push_back(pinned);
   pinned = nullptr;
+  if (dev.p) {
+    auto o = std::make_unique<DeviceBuffer>();
+    std::swap(o->p, dev.p);
+    std::swap(o->bytes, dev.bytes);
+    old_dev.push_back(std::move(o));
+  }
   cap = 0;
-  HIP_TRY(hipHostMalloc((void**)&pinned, bytes, hipHostMallocDefault));
-  HIP_TRY(dev.alloc(bytes));
-  cap = bytes;
+  HIP_TRY(hipHostMalloc((void**)&pinned, want, hipHostMallocDefault));
+  HIP_TRY(dev.alloc(want));
+  cap = want;
   return RSGPU_OK;
 }
 
@@ -739,7 +751,9 @@ int presolve_decode(const std::vector<PresolveCall>& calls, hipStream_t s, Preso
     std::lock_guard<std::mutex> g(c.mu);  // held until the new entries' events are recorded
     std::vector<DecodeMultiJob> jobs;
     std::vector<std::pair<std::string, std::shared_ptr<DecodeEntry>>> made;
-    size_t lds = 0;
+    // the codec's new patterns first; a few are left to their calls, whose own
+    // solves need no job-array transfer
+    std::vector<std::pair<std::string, Plan>> fresh;
     for (auto& cl : calls) {
       if (cl.codec != cp) continue;
       Plan plan;
@@ -749,21 +763,26 @@ int presolve_decode(const std::vector<PresolveCall>& calls, hipStream_t s, Preso
       std::string key = plan_key(c, cl.shards, plan);
       if (c.index.count(key)) continue;
       bool dup = false;
-      for (auto& mk : made) dup |= mk.first == key;
+      for (auto& f : fresh) dup |= f.first == key;
       if (dup) continue;
       if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable(c, plan)) continue;
+      fresh.emplace_back(std::move(key), std::move(plan));
+    }
+    if (fresh.size() < kPresolveMin) continue;
+    size_t lds = 0;
+    for (auto& f : fresh) {
       DecodeMultiJob J{};
-      fill_decode_job(c, plan, J.job);
+      fill_decode_job(c, f.second, J.job);
       const size_t l = decode_rows_lds(J.job);
       if (!l) continue;
       lds = std::max(lds, l);
       std::shared_ptr<DecodeEntry> e;
-      int rc = new_decode_entry(c, plan, key, s, e);
+      int rc = new_decode_entry(c, f.second, f.first, s, e);
       if (rc) return rc;
       J.rows = e->rows;
       J.tables = e->tables;
       jobs.push_back(J);
-      made.emplace_back(std::move(key), std::move(e));
+      made.emplace_back(f.first, std::move(e));
     }
     if (jobs.empty()) continue;
     const size_t bytes = jobs.size() * sizeof(DecodeMultiJob);

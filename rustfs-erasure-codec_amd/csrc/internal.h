@@ -153,12 +153,12 @@ struct DeviceBuffer {
 // slots become free again after one device-wide synchronisation, taken only
 // when no free slot is left and at least kRecycleBatch have retired.  New
 // slots come in slabs of one hipMalloc each (doubling up to kMaxSlab slots and
-// 64 MiB), so
+// 64 MiB; a slot's event is made on its first use), so
 // a stream of cache misses costs neither a hipMalloc nor a synchronising
 // hipFree per entry.
 struct DecodePool {
   static constexpr size_t kRecycleBatch = 64;
-  static constexpr size_t kMaxSlab = 256;
+  static constexpr size_t kMaxSlab = 32;
   struct Slot {
     void* mem;
     hipEvent_t ready;
@@ -171,6 +171,7 @@ struct DecodePool {
   ~DecodePool();
   hipError_t acquire(Slot& out);
   void release(const Slot& s);
+  hipError_t ready_event(Slot& sl);  // with mu held
 };
 
 // One cached decode plan: the composed rows for a (present, targets) pattern.
@@ -238,10 +239,14 @@ int words_per_coef(int tower);  // kTabWords or 4 * kTabWords
 
 // Staging for presolve_decode's job arrays, owned by the caller of one stream
 // and reused across calls; `done` is recorded behind the launches that read it.
+// It grows geometrically, and outgrown buffers are freed only with the scratch
+// (a hipFree waits for the whole device, which would stall a serving loop).
 struct PresolveScratch {
   uint8_t* pinned = nullptr;
   size_t cap = 0;
   DeviceBuffer dev;
+  std::vector<std::unique_ptr<DeviceBuffer>> old_dev;
+  std::vector<uint8_t*> old_pinned;
   hipEvent_t done = nullptr;
   ~PresolveScratch();
   int ensure(size_t bytes);
@@ -260,6 +265,11 @@ struct PresolveCall {
 // calls then find them there.  Codecs outside the one-workgroup solve (Leopard,
 // more than 256 data shards) are left to the calls themselves.
 int presolve_decode(const std::vector<PresolveCall>& calls, hipStream_t s, PresolveScratch& scr);
+constexpr size_t kPresolveMin = 8;  // fewer new patterns of a codec: solved by their calls
+
+// Sizes a batch's descriptor staging for n_stripes descriptors and n_ptrs shard
+// pointers up front, so later uploads of that size allocate nothing.
+int batch_reserve(rsgpu_batch* b, size_t n_stripes, size_t n_ptrs);
 
 // thread-local last HIP error
 void set_hip_error(hipError_t e, const char* where);

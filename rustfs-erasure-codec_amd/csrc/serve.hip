@@ -323,7 +323,10 @@ int rsgpu_server_new(const rsgpu_server_config* cfg, rsgpu_server** out) {
         hipEventCreateWithFlags(&B.ev, hipEventDisableTiming) != hipSuccess ||
         B.d_flags.alloc((size_t)cfg->max_batch_ops * 4) != hipSuccess ||
         hipHostMalloc((void**)&B.h_flags, (size_t)cfg->max_batch_ops * 4, hipHostMallocDefault) != hipSuccess ||
-        rsgpu_batch_new(&B.batch) != RSGPU_OK) {
+        rsgpu_batch_new(&B.batch) != RSGPU_OK ||
+        // staging for a full batch up front: serving then allocates nothing
+        batch_reserve(B.batch, (size_t)cfg->max_batch_ops * 2, (size_t)cfg->max_batch_ops * 24) ||
+        B.presolve.ensure((size_t)cfg->max_batch_ops * sizeof(DecodeMultiJob) + 4096)) {
       delete S;
       return RSGPU_ERR_OUT_OF_MEMORY;
     }
