@@ -267,6 +267,130 @@ def test_classic_linearity():
     assert all(((x ^ y) == z).all() for x, y, z in zip(pa, pb, pab))
 
 
+# ------------------------------------- closed forms of the matrix constructions
+_FIELD_OPS = {O.GF8: (O.gf8_mul, O.gf8_div), O.GF16: (O.gf16_mul, O.gf16_div)}
+
+
+def _prod(mul, vals):
+    acc = 1
+    for v in vals:
+        acc = mul(acc, v)
+    return acc
+
+
+def _det(mul, div, a):
+    """Determinant by elimination (characteristic 2: row swaps keep the sign)."""
+    a = [list(map(int, row)) for row in a]
+    n, det = len(a), 1
+    for c in range(n):
+        p = next((r for r in range(c, n) if a[r][c]), None)
+        if p is None:
+            return 0
+        a[c], a[p] = a[p], a[c]
+        det = mul(det, a[c][c])
+        for r in range(c + 1, n):
+            f = div(a[r][c], a[c][c])
+            a[r] = [x ^ mul(f, y) for x, y in zip(a[r], a[c])]
+    return det
+
+
+@pytest.mark.parametrize("field", [O.GF8, O.GF16])
+@pytest.mark.parametrize("k,m", [(5, 3), (10, 4), (3, 1)])
+def test_vandermonde_parity_is_lagrange_basis(field, k, m):
+    """systematic rows of build_matrix (src/core/codec.rs:66-88): V * Vtop^-1 takes
+    data row i to the Lagrange basis polynomial L_i over the points nth(0..k-1), so
+    parity row r, column i is L_i(nth(r)) = prod_{j != i} (r + j) / (i + j)."""
+    mul, div = _FIELD_OPS[field]
+    rc, mat = O.build_matrix(field, O.VANDERMONDE, k, m)
+    assert rc == 0
+    assert mat[:k].tolist() == np.eye(k, dtype=int).tolist()
+    for r in range(k, k + m):
+        for i in range(k):
+            others = [j for j in range(k) if j != i]
+            want = div(_prod(mul, [r ^ j for j in others]), _prod(mul, [i ^ j for j in others]))
+            assert int(mat[r, i]) == want, (r, i)
+
+
+@pytest.mark.parametrize("field", [O.GF8, O.GF16])
+@pytest.mark.parametrize("k,m", [(5, 3), (10, 4), (4, 1)])
+def test_cauchy_parity_obeys_cauchy_determinant(field, k, m):
+    """build_cauchy_matrix (src/core/codec.rs:90-112): entry (r, c) is the inverse of
+    nth(r) + nth(c), and every square block of the parity rows has the Cauchy
+    determinant prod_{i<j}(x_i + x_j) prod_{i<j}(y_i + y_j) / prod_{i,j}(x_i + y_j)."""
+    mul, div = _FIELD_OPS[field]
+    rc, mat = O.build_matrix(field, O.CAUCHY, k, m)
+    assert rc == 0
+    assert mat[:k].tolist() == np.eye(k, dtype=int).tolist()
+    for r in range(k, k + m):
+        for c in range(k):
+            assert mul(int(mat[r, c]), r ^ c) == 1, (r, c)
+    for size in range(1, m + 1):
+        for rows in itertools.combinations(range(k, k + m), size):
+            for cols in itertools.combinations(range(k), size):
+                num = _prod(mul, [a ^ b for a, b in itertools.combinations(rows, 2)])
+                num = mul(num, _prod(mul, [a ^ b for a, b in itertools.combinations(cols, 2)]))
+                den = _prod(mul, [x ^ y for x in rows for y in cols])
+                block = [[mat[x, y] for y in cols] for x in rows]
+                assert _det(mul, div, block) == div(num, den), (rows, cols)
+
+
+@pytest.mark.parametrize("field", [O.GF8, O.GF16])
+@pytest.mark.parametrize("k,m", [(5, 3), (10, 4), (4, 1), (6, 2), (12, 4)])
+def test_jerasure_like_parity_closed_form(field, k, m):
+    """build_jerasure_like_matrix (src/core/codec.rs:114-197).  Its top k rows are the
+    untouched Vandermonde rows over nth(0..k-1), so the column reduction is a right
+    multiplication by Vtop^-1 and never swaps rows.  Before the scalings, parity row r
+    (r < k+m-1) is [L_i(nth(r))] and the last row, e_{k-1} * Vtop^-1, is the leading
+    coefficients 1 / prod_{j != i} (i + j).  Columns are then scaled so row k is all
+    ones, and rows after k so column 0 is one."""
+    mul, div = _FIELD_OPS[field]
+    rc, mat = O.build_matrix(field, O.JERASURE, k, m)
+    assert rc == 0
+    assert mat[:k].tolist() == np.eye(k, dtype=int).tolist()
+
+    def lagrange(r, i):
+        others = [j for j in range(k) if j != i]
+        return div(_prod(mul, [r ^ j for j in others]), _prod(mul, [i ^ j for j in others]))
+
+    raw = [[lagrange(r, i) for i in range(k)] for r in range(k, k + m - 1)]
+    raw.append([div(1, _prod(mul, [i ^ j for j in range(k) if j != i])) for i in range(k)])
+    col = [div(1, v) for v in raw[0]]
+    scaled = [[mul(v, s) for v, s in zip(row, col)] for row in raw]
+    want = [scaled[0]] + [[div(v, row[0]) for v in row] for row in scaled[1:]]
+    assert mat[k:].tolist() == want
+    assert mat[k].tolist() == [1] * k
+    assert all(int(mat[r, 0]) == 1 for r in range(k, k + m))
+
+
+def test_gf16_classic_encode_element_layout():
+    """A GF(2^16) element is two bytes, high byte first: one() = [0, 1] and
+    nth(n) = [n >> 8, n & 255] (src/galois_16.rs:55-65).  A unit element at
+    position e of data shard i therefore encodes to the matrix coefficient
+    (r, i) at position e of parity row r, high byte first; a general element
+    encodes to the scalar tower product, element by element."""
+    k, m, n = 6, 3, 16
+    rc, mat = O.build_matrix(O.GF16, O.VANDERMONDE, k, m)
+    assert rc == 0
+    for i in range(k):
+        data = [np.zeros(2 * n, np.uint8) for _ in range(k)]
+        data[i][2 * 5:2 * 5 + 2] = [0, 1]
+        parity = O.encode(O.GF16, mat, k, m, data)
+        for r in range(m):
+            coef = int(mat[k + r, i])
+            want = np.zeros(2 * n, np.uint8)
+            want[10:12] = [coef >> 8, coef & 255]
+            assert (parity[r] == want).all(), (i, r)
+    rng = np.random.default_rng(11)
+    data = [rng.integers(0, 256, 2 * n, dtype=np.uint8) for _ in range(k)]
+    parity = O.encode(O.GF16, mat, k, m, data)
+    for r in range(m):
+        for e in range(n):
+            acc = 0
+            for i in range(k):
+                acc ^= O.gf16_mul(int(mat[k + r, i]), (int(data[i][2 * e]) << 8) | int(data[i][2 * e + 1]))
+            assert (int(parity[r][2 * e]) << 8) | int(parity[r][2 * e + 1]) == acc, (r, e)
+
+
 # ---------------------------------------------------------------- GF(2^16) tower
 def test_gf16_tower_field_axioms():
     rng = np.random.default_rng(7)
@@ -332,6 +456,123 @@ def _leo_mul8(a, log_b, log, exp):
         return 0
     s = int(log[a]) + int(log_b)
     return int(exp[(s + (s >> 8)) & 0xFF])
+
+
+# Published Cantor bases of the Leopard fields (src/core/leopard_gf8/tables.rs:25,
+# src/core/leopard_gf16/tables.rs:24-27): field polynomials 0x11D and 0x1002D.
+LEO_CANTOR = {
+    8: (0x11D, [1, 214, 152, 146, 86, 200, 88, 230]),
+    16: (0x1002D, [0x0001, 0xACCA, 0x3C0E, 0x163E, 0xC582, 0xED2E, 0x914C, 0x4012,
+                   0x6C98, 0x10D8, 0x6A72, 0xB900, 0xFDB8, 0xFB34, 0xFF38, 0x991E]),
+}
+
+
+def _poly_mul(a, b, bits, poly):
+    """Carry-less product reduced by the field polynomial (textbook GF(2^n))."""
+    r = 0
+    while b:
+        if b & 1:
+            r ^= a
+        b >>= 1
+        a <<= 1
+        if a >> bits:
+            a ^= poly
+    return r
+
+
+def _leo_field(bits):
+    """Multiply and divide of a Leopard field through its log / exp tables."""
+    _, _, log, exp = O.leo8_tables() if bits == 8 else O.leo16_tables()
+    mod = (1 << bits) - 1
+    log = [int(v) for v in log]
+    exp = [int(v) for v in exp[:mod + 1]]
+
+    def mul(a, b):
+        return 0 if a == 0 or b == 0 else exp[(log[a] + log[b]) % mod]
+
+    def div(a, b):
+        assert b
+        return 0 if a == 0 else exp[(log[a] - log[b]) % mod]
+
+    return mul, div
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_leopard_cantor_basis_recurrence(bits):
+    """A Cantor basis has b_0 = 1 and b_i^2 + b_i = b_{i-1} in the polynomial field."""
+    poly, basis = LEO_CANTOR[bits]
+    assert basis[0] == 1
+    for i in range(1, bits):
+        assert _poly_mul(basis[i], basis[i], bits, poly) ^ basis[i] == basis[i - 1], i
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_leopard_tables_are_the_polynomial_field_in_cantor_coordinates(bits):
+    """Leopard stores element x as its coordinates in the Cantor basis: the map
+    phi(x) = XOR of b_i over the set bits i of x is a field isomorphism from the
+    table arithmetic onto GF(2)[X]/poly (init_luts, leopard_gf8/tables.rs:25-60)."""
+    poly, basis = LEO_CANTOR[bits]
+    size = 1 << bits
+    phi = [0] * size
+    for x in range(1, size):
+        low = x & -x
+        phi[x] = phi[x ^ low] ^ basis[low.bit_length() - 1]
+    assert sorted(phi) == list(range(size))
+    inv = [0] * size
+    for x, y in enumerate(phi):
+        inv[y] = x
+    mul, _ = _leo_field(bits)
+    rng = np.random.default_rng(bits)
+    if bits == 8:
+        pairs = itertools.product(range(256), repeat=2)
+    else:
+        pairs = [(int(a), int(b)) for a, b in rng.integers(0, size, (20000, 2))]
+        pairs += [(0, 5), (5, 0), (1, 0xFFFF), (0xFFFF, 0xFFFF)]
+    for a, b in pairs:
+        assert mul(a, b) == inv[_poly_mul(phi[a], phi[b], bits, poly)], (a, b)
+
+
+def _leo_elements(bits, shard):
+    if bits == 8:
+        return [int(v) for v in shard]
+    out = []
+    for base in range(0, shard.size, 64):
+        out += [int(shard[base + i]) | int(shard[base + 32 + i]) << 8 for i in range(32)]
+    return out
+
+
+@pytest.mark.parametrize("bits,k,m", [(16, 3, 2), (16, 10, 4), (16, 40, 16), (16, 33, 5),
+                                      (8, 3, 2), (8, 10, 4), (8, 17, 8), (8, 100, 28),
+                                      (8, 5, 3), (8, 128, 64)])
+def test_leopard_parity_is_coset_interpolation(bits, k, m):
+    """Closed form of the Leopard encoder (leopard_gf8/encode.rs, leopard_gf16/encode.rs).
+    With M = next power of two >= m and element x standing for the point w_x, data
+    shard c*M + t sits at the point M*(c+1) + t.  The IFFT over the coset
+    M*(c+1) + {0..M-1} and the FFT over {0..M-1} evaluate the degree < M interpolant
+    of chunk c (absent data shards count as zeros) at the parity points, so
+    parity i = sum_{c,t} d_{cM+t} prod_{s != t} (i + M(c+1) + s) / (t + s).
+    The GF(2^16) cases, whose parity bytes are pinned by the Go known answers,
+    confirm the form; the GF(2^8) cases then pin the GF(2^8) encoder."""
+    mul, div = _leo_field(bits)
+    M = 1 << (m - 1).bit_length()
+    rng = np.random.default_rng(bits * 1000 + k * 10 + m)
+    data = [rng.integers(0, 256, 64, dtype=np.uint8) for _ in range(k)]
+    rc, parity = O.leo_encode(bits, k, m, data)
+    assert rc == 0
+    elems = [_leo_elements(bits, d) for d in data]
+    got = [_leo_elements(bits, p) for p in parity]
+    for i in range(m):
+        want = [0] * len(elems[0])
+        for d in range(k):
+            c, t = divmod(d, M)
+            base = M * (c + 1)
+            coef = 1
+            for s in range(M):
+                if s != t:
+                    coef = mul(coef, div(i ^ (base + s), t ^ s))
+            for e, v in enumerate(elems[d]):
+                want[e] ^= mul(coef, v)
+        assert got[i] == want, i
 
 
 @pytest.mark.parametrize("bits", [8, 16])
