@@ -713,8 +713,8 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
   if (mismatch) atomicOr(a.flag, 1);
 }
 
-template <bool G16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void leo_encode_kernel(const LeoArgs a) {
+template <bool G16, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256, 2 * NT / 256))) void leo_encode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
   if (a.C % 4 == 0) leo_encode_body<G16, 4>(a, lds);
   else if (a.C % 2 == 0) leo_encode_body<G16, 2>(a, lds);
@@ -784,8 +784,8 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds)
   }
 }
 
-template <bool G16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void leo_decode_kernel(const LeoArgs a) {
+template <bool G16, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256, 2 * NT / 256))) void leo_decode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
   if (a.C % 4 == 0) leo_decode_body<G16, 4>(a, lds);
   else if (a.C % 2 == 0) leo_decode_body<G16, 2>(a, lds);
@@ -874,7 +874,7 @@ static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopar
 
 // Column tile of a launch: as many words as fit LDS for `rows` lanes, at most
 // max(64, 4096 / rows) -- a layer then has rows * C / 8 >= 512 four-word items
-// for the 256 threads even when the codec has few work lanes -- or the smaller
+// for the threads even when the codec has few work lanes -- or the smaller
 // `want` a dispatched variant asks for.
 static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0) {
   const uint32_t wb = s.gf16 ? 8 : 4;
@@ -892,6 +892,20 @@ static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uin
   if (C >= 4) C &= ~(uint64_t)3;  // whole 4-word vectors (16-byte LDS / global accesses)
   else if (C >= 2) C &= ~(uint64_t)1;
   return (uint32_t)C;
+}
+
+// Threads per workgroup.  The GF8 kernels take 72 VGPRs, so two 512-thread
+// workgroups share a CU at 4 waves per SIMD.  The GF16 kernels take about 137
+// (3 waves per SIMD at most): they stay at 256 threads while two workgroups fit
+// the LDS; a workgroup that fills the LDS alone runs 512 threads (encode) or 1024
+// (decode, built to 128 VGPRs), so that more than one wave per SIMD covers the
+// LDS and table-load latency.  Measured on MI355X against 256 threads: GF8 128+64
+// encode / reconstruct 672 -> 787 / 261 -> 357 GB/s, GF16 1024+256 reconstruct
+// 140 -> 207, GF16 4096+1024 encode 418 -> 505.
+static int leo_threads(const LeoState& s, size_t lds, bool decode) {
+  if (!s.gf16) return 512;
+  if (2 * lds <= s.lds_limit) return 256;
+  return decode ? 1024 : 512;
 }
 
 static bool all_aligned16(const std::vector<const uint8_t*>& ptrs) {
@@ -1130,10 +1144,10 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   const size_t lds = st.gf16 && C % 4 == 0 ? (size_t)2 * gf16_lds_dwords(st.m * C) * 4
                                             : (size_t)2 * st.m * C * (st.gf16 ? 8 : 4);
   const unsigned grid = (unsigned)((words + C - 1) / C);
-  if (st.gf16)
-    hipLaunchKernelGGL(leo_encode_kernel<true>, dim3(grid), dim3(256), lds, s, a);
-  else
-    hipLaunchKernelGGL(leo_encode_kernel<false>, dim3(grid), dim3(256), lds, s, a);
+  const int nt = leo_threads(st, lds, false);
+  if (!st.gf16) hipLaunchKernelGGL((leo_encode_kernel<false, 512>), dim3(grid), dim3(nt), lds, s, a);
+  else if (nt == 512) hipLaunchKernelGGL((leo_encode_kernel<true, 512>), dim3(grid), dim3(nt), lds, s, a);
+  else hipLaunchKernelGGL((leo_encode_kernel<true, 256>), dim3(grid), dim3(nt), lds, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_hip_error(e, "leo_encode_kernel");
@@ -1280,10 +1294,10 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   const size_t lds = st.gf16 && C % 4 == 0 ? (size_t)gf16_lds_dwords(st.n * C) * 4
                                             : (size_t)st.n * C * (st.gf16 ? 8 : 4);
   const unsigned grid = (unsigned)((words + C - 1) / C);
-  if (st.gf16)
-    hipLaunchKernelGGL(leo_decode_kernel<true>, dim3(grid), dim3(256), lds, s, a);
-  else
-    hipLaunchKernelGGL(leo_decode_kernel<false>, dim3(grid), dim3(256), lds, s, a);
+  const int nt = leo_threads(st, lds, true);
+  if (!st.gf16) hipLaunchKernelGGL((leo_decode_kernel<false, 512>), dim3(grid), dim3(nt), lds, s, a);
+  else if (nt == 1024) hipLaunchKernelGGL((leo_decode_kernel<true, 1024>), dim3(grid), dim3(nt), lds, s, a);
+  else hipLaunchKernelGGL((leo_decode_kernel<true, 256>), dim3(grid), dim3(nt), lds, s, a);
   hipError_t he = hipGetLastError();
   if (he != hipSuccess) {
     set_hip_error(he, "leo_decode_kernel");
