@@ -5,6 +5,7 @@ classic and Leopard padding rules (src/core/codec.rs:476-538, leopard.rs:75-98).
 import numpy as np
 import pytest
 
+from fixtures import golden
 from gpu_util import announce, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
@@ -12,22 +13,11 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 rsgpu = pytest.importorskip("rsgpu")
 
-# tests/encoding_format_identity.rs:10-27 (1+1 custom matrices [0x87] and [[0x12, 0x34]])
-GF8_ID_V1 = bytes([
-    0x52, 0x53, 0x45, 0x46, 0x4d, 0x54, 0x00, 0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00, 0x19, 0x72,
-    0x75, 0x73, 0x74, 0x66, 0x73, 0x2e, 0x67, 0x66, 0x38, 0x2e, 0x70, 0x6f, 0x6c, 0x79, 0x2d, 0x31,
-    0x31, 0x64, 0x2e, 0x75, 0x38, 0x2e, 0x76, 0x31, 0x00, 0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00,
-    0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00,
-    0x00, 0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00, 0x00, 0x00,
-    0x00, 0x00, 0x00, 0x01, 0x01, 0x87])
-TOWER_GF16_ID_V1 = bytes([
-    0x52, 0x53, 0x45, 0x46, 0x4d, 0x54, 0x00, 0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00, 0x28, 0x72,
-    0x75, 0x73, 0x74, 0x66, 0x73, 0x2e, 0x67, 0x66, 0x31, 0x36, 0x2e, 0x74, 0x6f, 0x77, 0x65, 0x72,
-    0x2d, 0x31, 0x31, 0x64, 0x2d, 0x78, 0x32, 0x2d, 0x30, 0x32, 0x78, 0x2d, 0x38, 0x30, 0x2e, 0x61,
-    0x78, 0x2d, 0x61, 0x63, 0x2e, 0x76, 0x31, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00, 0x00, 0x00, 0x00,
-    0x01, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00, 0x00,
-    0x00, 0x00, 0x01, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x02, 0x00, 0x00, 0x00, 0x00, 0x00,
-    0x00, 0x00, 0x01, 0x00, 0x01, 0x12, 0x34])
+# version-1 identifiers of the 1+1 custom matrices [0x87] (GF(2^8)) and [[0x12, 0x34]]
+# (tower GF(2^16)), tests/encoding_format_identity.rs:10-27
+_IDS = golden("format_identity_v1")
+GF8_ID_V1 = bytes.fromhex(_IDS["gf8_custom_1x1_0x87"])
+TOWER_GF16_ID_V1 = bytes.fromhex(_IDS["tower_gf16_custom_1x1_0x1234"])
 
 
 def test_stable_ids_match_reference_fixtures():

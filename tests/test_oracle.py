@@ -1,7 +1,8 @@
 """Pins the CPU oracle against the reference's own known answers.
 
-Every vector below is copied from a test of rustfs-erasure-codec 9.1.0 (file:line
-cited per test) or follows from a closed form of the mathematics.  The oracle is
+The known answers come from the tests of rustfs-erasure-codec 9.1.0, kept as data in
+tests/golden/reference_vectors.json with the file:line of each; the other checks follow
+from closed forms of the mathematics.  The oracle is
 the checker for every GPU parity test, so these run on the CPU only.
 """
 import itertools
@@ -10,57 +11,28 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from fixtures import golden, hexint
 
 # ------------------------------------------------------------------ GF(2^8)
-# src/galois_8/tests.rs:65-92 (the Backblaze log table)
-BACKBLAZE_LOG = [
-    0, 0, 1, 25, 2, 50, 26, 198, 3, 223, 51, 238, 27, 104, 199, 75, 4, 100, 224, 14, 52, 141, 239,
-    129, 28, 193, 105, 248, 200, 8, 76, 113, 5, 138, 101, 47, 225, 36, 15, 33, 53, 147, 142, 218,
-    240, 18, 130, 69, 29, 181, 194, 125, 106, 39, 249, 185, 201, 154, 9, 120, 77, 228, 114, 166, 6,
-    191, 139, 98, 102, 221, 48, 253, 226, 152, 37, 179, 16, 145, 34, 136, 54, 208, 148, 206, 143,
-    150, 219, 189, 241, 210, 19, 92, 131, 56, 70, 64, 30, 66, 182, 163, 195, 72, 126, 110, 107, 58,
-    40, 84, 250, 133, 186, 61, 202, 94, 155, 159, 10, 21, 121, 43, 78, 212, 229, 172, 115, 243,
-    167, 87, 7, 112, 192, 247, 140, 128, 99, 13, 103, 74, 222, 237, 49, 197, 254, 24, 227, 165,
-    153, 119, 38, 184, 180, 124, 17, 68, 146, 217, 35, 32, 137, 46, 55, 63, 209, 91, 149, 188, 207,
-    205, 144, 135, 151, 178, 220, 252, 190, 97, 242, 86, 211, 171, 20, 42, 93, 158, 132, 60, 57,
-    83, 71, 109, 65, 162, 31, 45, 67, 216, 183, 123, 164, 118, 196, 23, 73, 236, 127, 12, 111, 246,
-    108, 161, 59, 82, 41, 157, 85, 170, 251, 96, 134, 177, 187, 204, 62, 90, 203, 89, 95, 176, 156,
-    169, 160, 81, 11, 245, 22, 235, 122, 117, 44, 215, 79, 174, 213, 233, 230, 231, 173, 232, 116,
-    214, 244, 234, 168, 80, 88, 175,
-]
-
-
 def test_gf8_log_table_same_as_backblaze():
     log, _ = O.gf8_tables()
-    assert list(log) == BACKBLAZE_LOG
+    assert list(log) == golden("gf8_log_table")["log"]
 
 
 def test_gf8_known_answers():
-    # src/galois_8/tests.rs:209-279
-    assert O.gf8_mul(3, 4) == 12
-    assert O.gf8_mul(7, 7) == 21
-    assert O.gf8_mul(23, 45) == 41
-    inp = np.array([0, 1, 2, 3, 4, 5, 6, 10, 50, 100, 150, 174, 201, 255, 99, 32, 67, 85, 200, 199,
-                    198, 197, 196, 195, 194, 193, 192, 191, 190, 189, 188, 187, 186, 185], np.uint8)
-    out = O.gf8_mul_slice(25, inp)
-    assert list(out) == [0x0, 0x19, 0x32, 0x2b, 0x64, 0x7d, 0x56, 0xfa, 0xb8, 0x6d, 0xc7, 0x85,
-                         0xc3, 0x1f, 0x22, 0x7, 0x25, 0xfe, 0xda, 0x5d, 0x44, 0x6f, 0x76, 0x39,
-                         0x20, 0xb, 0x12, 0x11, 0x8, 0x23, 0x3a, 0x75, 0x6c, 0x47]
-    O.gf8_mul_slice(52, inp, out, xor=True)
-    assert list(out) == [0x0, 0x2d, 0x5a, 0x77, 0xb4, 0x99, 0xee, 0x2f, 0x79, 0xf2, 0x7, 0x51,
-                         0xd4, 0x19, 0x31, 0xc9, 0xf8, 0xfc, 0xf9, 0x4f, 0x62, 0x15, 0x38, 0xfb,
-                         0xd6, 0xa1, 0x8c, 0x96, 0xbb, 0xcc, 0xe1, 0x22, 0xf, 0x78]
-    out = O.gf8_mul_slice(177, inp)
-    assert list(out) == [0x0, 0xb1, 0x7f, 0xce, 0xfe, 0x4f, 0x81, 0x9e, 0x3, 0x6, 0xe8, 0x75,
-                         0xbd, 0x40, 0x36, 0xa3, 0x95, 0xcb, 0xc, 0xdd, 0x6c, 0xa2, 0x13, 0x23,
-                         0x92, 0x5c, 0xed, 0x1b, 0xaa, 0x64, 0xd5, 0xe5, 0x54, 0x9a]
-    O.gf8_mul_slice(117, inp, out, xor=True)
-    assert list(out) == [0x0, 0xc4, 0x95, 0x51, 0x37, 0xf3, 0xa2, 0xfb, 0xec, 0xc5, 0xd0, 0xc7,
-                         0x53, 0x88, 0xa3, 0xa5, 0x6, 0x78, 0x97, 0x9f, 0x5b, 0xa, 0xce, 0xa8,
-                         0x6c, 0x3d, 0xf9, 0xdf, 0x1b, 0x4a, 0x8e, 0xe8, 0x2c, 0x7d]
-    assert O.gf8_exp(2, 2) == 4
-    assert O.gf8_exp(5, 20) == 235
-    assert O.gf8_exp(13, 7) == 43
+    g = golden("gf8_arithmetic")
+    for x, y, want in g["mul"]:
+        assert O.gf8_mul(x, y) == want
+    for x, n, want in g["exp"]:
+        assert O.gf8_exp(x, n) == want
+    inp = np.array(g["slice_input"], np.uint8)
+    out = None
+    for step in g["slice_steps"]:
+        if step["xor"]:
+            O.gf8_mul_slice(step["c"], inp, out, xor=True)
+        else:
+            out = O.gf8_mul_slice(step["c"], inp)
+        assert list(out) == step["out"], step["c"]
 
 
 def _carryless_mod(a, b, poly=0x11D):
@@ -92,17 +64,13 @@ def test_gf8_div_exp_identities():
 
 # ------------------------------------------------------------------ matrices
 def test_matrix_inverse_known_answers():
-    # src/matrix.rs:716-745
-    rc, inv = O.matrix_invert(O.GF8, [[56, 23, 98], [3, 100, 200], [45, 201, 123]])
-    assert rc == 0
-    assert inv.tolist() == [[175, 133, 33], [130, 13, 245], [112, 35, 126]]
-    rc, inv = O.matrix_invert(O.GF8, [[1, 0, 0, 0, 0], [0, 1, 0, 0, 0], [0, 0, 0, 1, 0],
-                                      [0, 0, 0, 0, 1], [7, 7, 6, 6, 1]])
-    assert rc == 0
-    assert inv.tolist() == [[1, 0, 0, 0, 0], [0, 1, 0, 0, 0], [123, 123, 1, 122, 122],
-                            [0, 0, 1, 0, 0], [0, 0, 0, 1, 0]]
-    rc, _ = O.matrix_invert(O.GF8, [[1, 2], [1, 2]])
-    assert rc != 0
+    for case in golden("matrix_inverse")["cases"]:
+        rc, inv = O.matrix_invert(O.GF8, case["matrix"])
+        if case["inverse"] is None:
+            assert rc != 0
+        else:
+            assert rc == 0
+            assert inv.tolist() == case["inverse"]
 
 
 def test_systematic_matrix_rows():
@@ -124,44 +92,47 @@ def fixed_4x2():
 
 
 def test_golden_encode_4x2_incrementing_input():
-    # tests/golden_vectors.rs:46-66, tests/selftest.rs:24-57
+    g = golden("classic_4x2")
     _, mat = O.build_matrix(O.GF8, O.VANDERMONDE, 4, 2)
     data = fixed_4x2()
     parity = O.encode(O.GF8, mat, 4, 2, data)
-    assert parity[0].tolist() == list(range(64, 80))
-    assert parity[1].tolist() == list(range(80, 96))
-    assert O.checksum_shards(data + parity) == 0x9118ae5245b6279b
+    assert [p.tolist() for p in parity] == g["parity"]
+    assert O.checksum_shards(data + parity) == hexint(g["codeword_checksum"])
 
 
 def test_golden_reconstruct_4x2():
-    # tests/golden_vectors.rs:68-107
+    g = golden("classic_4x2")
     _, mat = O.build_matrix(O.GF8, O.VANDERMONDE, 4, 2)
     data = fixed_4x2()
     full = data + O.encode(O.GF8, mat, 4, 2, data)
+    # reconstruct_data with data shard 1 and parity shard 0 missing
+    missing = g["reconstruct_data_missing"]
     shards = [s.copy() for s in full]
-    shards[1][:] = 0
-    shards[4][:] = 0
-    present = [1, 0, 1, 1, 0, 1]
+    for i in missing:
+        shards[i][:] = 0
+    present = [0 if i in missing else 1 for i in range(6)]
     rc, out = O.reconstruct(O.GF8, mat, 4, 2, shards, present, data_only=True)
     assert rc == 0
     assert out[1].tolist() == list(range(16, 32))
-    recovered = [out[i] if i != 4 else np.zeros(0, np.uint8) for i in range(6)]
-    assert O.checksum_shards(recovered) == 0x961eb966d059c4cb
+    recovered = [out[i] if i < 4 or i not in missing else np.zeros(0, np.uint8) for i in range(6)]
+    assert O.checksum_shards(recovered) == hexint(g["reconstruct_data_checksum"])
+    # full reconstruct
+    missing = g["reconstruct_missing"]
     shards = [s.copy() for s in full]
-    shards[0][:] = 0
-    shards[5][:] = 0
-    rc, out = O.reconstruct(O.GF8, mat, 4, 2, shards, [0, 1, 1, 1, 1, 0])
+    for i in missing:
+        shards[i][:] = 0
+    rc, out = O.reconstruct(O.GF8, mat, 4, 2, shards, [0 if i in missing else 1 for i in range(6)])
     assert rc == 0 and all((a == b).all() for a, b in zip(out, full))
-    assert O.checksum_shards(out) == 0x9118ae5245b6279b
+    assert O.checksum_shards(out) == hexint(g["codeword_checksum"])
 
 
 def test_golden_verify_detects_corruption():
-    # tests/golden_vectors.rs:109-120
+    c = golden("classic_4x2")["verify_corrupt"]
     _, mat = O.build_matrix(O.GF8, O.VANDERMONDE, 4, 2)
     data = fixed_4x2()
     full = data + O.encode(O.GF8, mat, 4, 2, data)
     assert O.verify(O.GF8, mat, 4, 2, full)
-    full[5][3] ^= 0xFF
+    full[c["shard"]][c["byte"]] ^= c["xor"]
     assert not O.verify(O.GF8, mat, 4, 2, full)
 
 
@@ -177,30 +148,28 @@ def _split(data, k):
     return out
 
 
-@pytest.mark.parametrize("kind,expect", [
-    ("seeded", 0x81458d413eb7d9cc),
-    ("zeros", 0xb82e0d6749372cdb),
-    ("pattern", 0x0de38c74d7643ce3),
-])
-def test_golden_split_encode(kind, expect):
-    # tests/golden_vectors.rs:122-155 (61-byte payloads through split + encode)
+@pytest.mark.parametrize("kind", ["seeded", "zeros", "pattern"])
+def test_golden_split_encode(kind):
+    # 61-byte payloads through split + encode
+    g = golden("split_encode_4x2")
+    n = g["payload_bytes"]
     if kind == "seeded":
-        payload = O.smallrng_bytes(0xA5A51024, 61)
+        payload = O.smallrng_bytes(hexint(g["seeded"]["seed"]), n)
     elif kind == "zeros":
-        payload = np.zeros(61, np.uint8)
+        payload = np.zeros(n, np.uint8)
     else:
-        payload = np.resize(np.array([0xDE, 0xAD, 0xBE, 0xEF], np.uint8), 61)
+        payload = np.resize(np.array(g["pattern"]["repeat"], np.uint8), n)
     _, mat = O.build_matrix(O.GF8, O.VANDERMONDE, 4, 2)
     data = _split(payload, 4)
     parity = O.encode(O.GF8, mat, 4, 2, data)
     assert O.verify(O.GF8, mat, 4, 2, data + parity)
-    assert O.checksum_shards(data + parity) == expect
+    assert O.checksum_shards(data + parity) == hexint(g[kind]["checksum"])
 
 
 def test_qc_mix_known_answer():
-    # src/tests/mod.rs:95-105
-    seed = O.qc_seed([1, 12, 1, 1, 3])
-    assert O.qc_fill(1, seed ^ 0x2000).tolist() == [54]
+    g = golden("qc_mix")
+    seed = O.qc_seed(g["parts"])
+    assert O.qc_fill(len(g["fill"]), seed ^ hexint(g["xor"])).tolist() == g["fill"]
 
 
 def test_corrupt_positions_is_partial_permutation():
@@ -423,31 +392,28 @@ def test_leopard_gf8_table_shapes():
 
 
 def test_leopard_gf16_fft_skew_known_answer():
-    # src/core/leopard_gf16/tests.rs:342-360
+    want = golden("leopard_gf16")["fft_skew"]["first8"]
     skew, _, log, exp = O.leo16_tables()
-    assert skew[:8].tolist() == [0xFFFF, 0xFFFF, 0x5555, 0xFFFF, 0x4444, 0x5555, 0x8888, 0xFFFF]
+    assert skew[:8].tolist() == want
 
 
 def test_leopard_gf16_parity_known_answer():
-    # src/core/leopard_gf16/tests.rs:646-697 (Go-derived parity bytes)
+    # Go-derived parity bytes
+    want = golden("leopard_gf16")["parity_3x2"]["parity_first16"]
     data = [np.array([(i * 64 + j) & 0xFF for j in range(64)], np.uint8) for i in range(3)]
     rc, parity = O.leo_encode(16, 3, 2, data)
     assert rc == 0
-    assert parity[0][:16].tolist() == [59, 148, 199, 104, 147, 60, 111, 192, 227, 76, 31, 176,
-                                        75, 228, 183, 24]
-    assert parity[1][:16].tolist() == [251, 85, 5, 171, 87, 249, 169, 7, 43, 133, 213, 123, 135,
-                                        41, 121, 215]
+    assert [p[:16].tolist() for p in parity] == want
 
 
 def test_leopard_gf16_coefficients_known_answer():
-    # src/core/leopard_gf16/tests.rs:699-738
-    expected = [[3, 2, 5], [2, 3, 4]]
+    expected = golden("leopard_gf16")["coefficients_3x2"]["rows"]
     for d in range(3):
         data = [np.zeros(64, np.uint8) for _ in range(3)]
         data[d][0] = 1
         _, parity = O.leo_encode(16, 3, 2, data)
         for p in range(2):
-            got = int(parity[p][0]) | (int(parity[p][1]) << 8)
+            got = int(parity[p][0]) | (int(parity[p][32]) << 8)  # element 0: bytes 0 and 32
             assert got == expected[p][d]
 
 
