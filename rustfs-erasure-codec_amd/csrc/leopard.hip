@@ -713,8 +713,8 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
   if (mismatch) atomicOr(a.flag, 1);
 }
 
-template <bool G16, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256, 2 * NT / 256))) void leo_encode_kernel(const LeoArgs a) {
+template <bool G16, int NT, int MINW = NT / 256>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT / 256))) void leo_encode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
   if (a.C % 4 == 0) leo_encode_body<G16, 4>(a, lds);
   else if (a.C % 2 == 0) leo_encode_body<G16, 2>(a, lds);
@@ -784,8 +784,8 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds)
   }
 }
 
-template <bool G16, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256, 2 * NT / 256))) void leo_decode_kernel(const LeoArgs a) {
+template <bool G16, int NT, int MINW = NT / 256>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT / 256))) void leo_decode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
   if (a.C % 4 == 0) leo_decode_body<G16, 4>(a, lds);
   else if (a.C % 2 == 0) leo_decode_body<G16, 2>(a, lds);
@@ -896,16 +896,18 @@ static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uin
 
 // Threads per workgroup.  The GF8 kernels take 72 VGPRs, so two 512-thread
 // workgroups share a CU at 4 waves per SIMD.  The GF16 kernels take about 137
-// (3 waves per SIMD at most): they stay at 256 threads while two workgroups fit
-// the LDS; a workgroup that fills the LDS alone runs 512 threads (encode) or 1024
-// (decode, built to 128 VGPRs), so that more than one wave per SIMD covers the
-// LDS and table-load latency.  Measured on MI355X against 256 threads: GF8 128+64
-// encode / reconstruct 672 -> 787 / 261 -> 357 GB/s, GF16 1024+256 reconstruct
-// 140 -> 207, GF16 4096+1024 encode 418 -> 505.
+// (3 waves per SIMD at most).  GF16 encode runs 256 threads while two workgroups
+// fit the LDS and 512 when one fills it.  GF16 decode is built to 128 VGPRs (a
+// small spill) for 4 waves per SIMD: 512 threads while two workgroups fit, 1024
+// when one fills the LDS.  Measured on MI355X against 256 threads everywhere:
+// GF8 128+64 encode / reconstruct 672 -> 787 / 261 -> 357 GB/s, GF16 300+45
+// reconstruct 268 -> 315, GF16 1024+256 reconstruct 140 -> 207, GF16 4096+1024
+// encode 418 -> 505.
 static int leo_threads(const LeoState& s, size_t lds, bool decode) {
   if (!s.gf16) return 512;
-  if (2 * lds <= s.lds_limit) return 256;
-  return decode ? 1024 : 512;
+  const bool one_fills = 2 * lds > s.lds_limit;
+  if (decode) return one_fills ? 1024 : 512;
+  return one_fills ? 512 : 256;
 }
 
 static bool all_aligned16(const std::vector<const uint8_t*>& ptrs) {
@@ -1297,7 +1299,7 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   const int nt = leo_threads(st, lds, true);
   if (!st.gf16) hipLaunchKernelGGL((leo_decode_kernel<false, 512>), dim3(grid), dim3(nt), lds, s, a);
   else if (nt == 1024) hipLaunchKernelGGL((leo_decode_kernel<true, 1024>), dim3(grid), dim3(nt), lds, s, a);
-  else hipLaunchKernelGGL((leo_decode_kernel<true, 256>), dim3(grid), dim3(nt), lds, s, a);
+  else hipLaunchKernelGGL((leo_decode_kernel<true, 512, 4>), dim3(grid), dim3(nt), lds, s, a);
   hipError_t he = hipGetLastError();
   if (he != hipSuccess) {
     set_hip_error(he, "leo_decode_kernel");

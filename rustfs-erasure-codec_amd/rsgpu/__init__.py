@@ -560,7 +560,10 @@ class _StreamIO:
 
         def wr(_user, shard, buf, n):
             try:
-                got = self.writers[shard].write(C.string_at(buf, n))
+                # io's write() reads the buffer only during the call, so the staging
+                # memory is handed over without a copy
+                view = (C.c_uint8 * n).from_address(C.addressof(buf.contents))
+                got = self.writers[shard].write(memoryview(view).cast("B"))
                 return n if got is None else int(got)
             except Exception as e:
                 self.exc[("write", shard)] = e
