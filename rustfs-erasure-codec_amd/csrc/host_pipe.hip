@@ -239,23 +239,27 @@ int run_chunks(HostPipe& P, ChunkPlan& cp) {
     pending[s] = false;
     return RSGPU_OK;
   };
-  int rc = RSGPU_OK;
-  for (size_t c = 0; c < chunks && !rc; ++c) {
+  // one chunk; an error returns from here only, so both slots are still finished below
+  auto step = [&](size_t c) -> int {
     const int s = (int)(c & 1);
     const size_t off = c * cp.chunk, w = std::min(cp.chunk, cp.len - off);
-    if ((rc = drain(s))) break;
+    int r = drain(s);
+    if (r) return r;
     HIP_TRY(hipEventSynchronize(P.ev[s]));  // the slot's previous transfers are done
-    par_copy(cp.in_rows.size(), w, [&](size_t r) { return cp.host_in(cp.in_rows[r]) + off; },
-             [&](size_t r) { return P.pinned[s] + cp.in_rows[r] * P.pitch; });
-    if ((rc = copy_rows(HipDir::kIn, P, s, cp.in_rows, w))) break;
-    if ((rc = cp.op(c, s, off, w))) break;
+    par_copy(cp.in_rows.size(), w, [&](size_t i) { return cp.host_in(cp.in_rows[i]) + off; },
+             [&](size_t i) { return P.pinned[s] + cp.in_rows[i] * P.pitch; });
+    if ((r = copy_rows(HipDir::kIn, P, s, cp.in_rows, w))) return r;
+    if ((r = cp.op(c, s, off, w))) return r;
     if (c == 0 && cp.after_first) cp.after_first();
-    if ((rc = copy_rows(HipDir::kOut, P, s, cp.out_rows, w))) break;
+    if ((r = copy_rows(HipDir::kOut, P, s, cp.out_rows, w))) return r;
     HIP_TRY(hipEventRecord(P.ev[s], P.stream[s]));
     pending[s] = !cp.out_rows.empty();
     pend_off[s] = off;
     pend_w[s] = w;
-  }
+    return RSGPU_OK;
+  };
+  int rc = RSGPU_OK;
+  for (size_t c = 0; c < chunks && !rc; ++c) rc = step(c);
   // finish both slots even after an error, so no transfer into staging is left
   for (int s = 0; s < 2; ++s) {
     (void)hipStreamSynchronize(P.stream[s]);
