@@ -247,7 +247,7 @@ static uint32_t ceil_pow2(uint32_t v) {
 
 struct LeoDevPlan {
   DeviceBuffer layers, blocks, tables;
-  uint32_t n_layers = 0;
+  uint32_t n_layers = 0, n_blocks = 0, n_table_words = 0;
 };
 
 static int upload_plan(const PlanBuilder& b, LeoDevPlan& p) {
@@ -265,6 +265,8 @@ static int upload_plan(const PlanBuilder& b, LeoDevPlan& p) {
       hipMemcpy(p.tables.p, b.tables.data(), b.tables.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return RSGPU_ERR_HIP;
   p.n_layers = (uint32_t)b.layers.size();
+  p.n_blocks = (uint32_t)b.blocks.size();
+  p.n_table_words = (uint32_t)b.tables.size();
   return RSGPU_OK;
 }
 
@@ -313,6 +315,9 @@ struct LeoArgs {
   int32_t* flag;               // verify
   uint32_t mode;               // 0 encode, 1 verify
   uint32_t vec;                // every shard pointer 16-byte aligned
+  // A small plan is copied into LDS at dword plan_off when the kernel starts
+  // (plan_lds = 1): its layers then read blocks and tables from LDS.
+  uint32_t plan_lds, plan_off, plan_layers, plan_blocks, plan_tab_words;
   // Up to kLeoInline shards: pointers and flags travel in the kernel arguments
   // (inline_args = 1) instead of through the staged copy above, which saves a
   // host-to-device copy per call.
@@ -477,6 +482,29 @@ __device__ __forceinline__ void inv2(Word<G16>& x, Word<G16>& y, const Tab<G16>&
 
 typedef const __attribute__((address_space(4))) uint32_t* cword_t;
 
+// The transform plan as the layers read it: the arrays in global memory, or
+// their copy in LDS.
+struct PlanPtrs {
+  const LeoLayer* layers;
+  const LeoBlock* blocks;
+  const uint32_t* tables;
+};
+
+// Copies the plan (layers, blocks, tables: whole 16-byte vectors each) into LDS
+// at dst.  The first barrier of the kernel body orders the copy before any use.
+__device__ __forceinline__ PlanPtrs stage_plan(const LeoArgs& a, uint32_t* dst) {
+  const uint32_t nl = a.plan_layers * 8, nb = a.plan_blocks * 8, nt = a.plan_tab_words;
+  auto copy = [&](uint32_t* to, const uint32_t* from, uint32_t dwords) {
+    for (uint32_t i = threadIdx.x; i < dwords / 4; i += blockDim.x)
+      reinterpret_cast<uint4*>(to)[i] = reinterpret_cast<const uint4*>(from)[i];
+  };
+  copy(dst, reinterpret_cast<const uint32_t*>(a.layers), nl);
+  copy(dst + nl, reinterpret_cast<const uint32_t*>(a.blocks), nb);
+  copy(dst + nl + nb, a.tables, nt);
+  return PlanPtrs{reinterpret_cast<const LeoLayer*>(dst), reinterpret_cast<const LeoBlock*>(dst + nl),
+                  dst + nl + nb};
+}
+
 template <bool G16>
 __device__ __forceinline__ Tab<G16> load_tab_c(cword_t p) {
   Tab<G16> t;
@@ -494,9 +522,11 @@ __device__ __forceinline__ Tab<G16> load_tab_c(cword_t p) {
 
 // The items of one layer.  UNI: every wave's 64 items lie in one block (the
 // layer's cnt * ng is a multiple of 64 and waves start at multiples of 64), so
-// the block and its twiddle tables are read once per wave with scalar loads.
-template <bool G16, int G, bool UNI>
-__device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, const LeoArgs& a) {
+// the block and its twiddle tables are read once per wave -- with scalar loads
+// from global memory, or as broadcast reads when the plan is in LDS (PL).
+template <bool G16, int G, bool UNI, bool PL>
+__device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, const LeoArgs& a,
+                                            const PlanPtrs& P) {
   constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
   const uint32_t C = a.C, ng = a.C / G;
   const uint32_t items = L.n_blocks * L.cnt * ng;
@@ -505,17 +535,17 @@ __device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, co
     uint32_t blk = bf / L.cnt;
     if constexpr (UNI) blk = __builtin_amdgcn_readfirstlane(blk);
     LeoBlock B;
-    if constexpr (UNI) {
-      const cword_t q = (cword_t)(a.blocks + L.block0 + blk);
+    if constexpr (UNI && !PL) {
+      const cword_t q = (cword_t)(P.blocks + L.block0 + blk);
       B.r = q[0], B.cnt = q[1], B.t01 = q[2], B.t23 = q[3], B.t02 = q[4];
     } else {
-      B = a.blocks[L.block0 + blk];
+      B = P.blocks[L.block0 + blk];
     }
     const uint32_t i = B.r + (bf - blk * L.cnt);
     const uint32_t d = L.dist;
     auto tab = [&](uint32_t t) {
-      if constexpr (UNI) return load_tab_c<G16>((cword_t)a.tables + (size_t)t * TW);
-      else return load_tab<G16>(a.tables + (size_t)t * TW);
+      if constexpr (UNI && !PL) return load_tab_c<G16>((cword_t)P.tables + (size_t)t * TW);
+      else return load_tab<G16>(P.tables + (size_t)t * TW);
     };
     const Tab<G16> t01 = tab(B.t01);
     if (L.radix == 2) {
@@ -561,13 +591,14 @@ __device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, co
 
 // Runs layers [l0, l0 + nl) over the LDS lanes.  An item is one butterfly on G
 // consecutive words, so its twiddle tables are loaded once for G words.
-template <bool G16, int G>
-__device__ __forceinline__ void run_layers(uint32_t* lds, uint32_t l0, uint32_t nl, const LeoArgs& a) {
+template <bool G16, int G, bool PL>
+__device__ __forceinline__ void run_layers(uint32_t* lds, uint32_t l0, uint32_t nl, const LeoArgs& a,
+                                           const PlanPtrs& P) {
   const uint32_t ng = a.C / G;
   for (uint32_t l = l0; l < l0 + nl; ++l) {
-    const LeoLayer L = a.layers[l];
-    if ((L.cnt * ng) % 64 == 0) layer_items<G16, G, true>(lds, L, a);
-    else layer_items<G16, G, false>(lds, L, a);
+    const LeoLayer L = P.layers[l];
+    if ((L.cnt * ng) % 64 == 0) layer_items<G16, G, true, PL>(lds, L, a, P);
+    else layer_items<G16, G, false, PL>(lds, L, a, P);
     __syncthreads();
   }
 }
@@ -658,8 +689,8 @@ __device__ __forceinline__ bool wequal(Word<G16> a, Word<G16> b) {
 // encode_with_tables (leopard_gf8/encode.rs:194-372; leopard_gf16/encode.rs:109-253):
 // groups of m data lanes are IFFT'd (skew offset m-1+group*m) and XORed into the
 // first m lanes, which are then FFT'd; lanes 0..parity are the parity.
-template <bool G16, int G>
-__device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds) {
+template <bool G16, int G, bool PL>
+__device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
   const uint32_t C = a.C, m = a.m, ng = C / G;
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   uint32_t* A = lds;
@@ -679,7 +710,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
       lds_putv<G16, G>(dst, lane * C + wq, v);
     }
     __syncthreads();
-    run_layers<G16, G>(dst, Gp.layer0, Gp.n_layers, a);
+    run_layers<G16, G, PL>(dst, Gp.layer0, Gp.n_layers, a, P);
     if (g > 0) {
       for (uint32_t it = threadIdx.x; it < m * ng; it += blockDim.x) {
         Word<G16> x[G], y[G];
@@ -692,7 +723,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
       __syncthreads();
     }
   }
-  run_layers<G16, G>(A, a.fft_layer0, a.fft_layers, a);
+  run_layers<G16, G, PL>(A, a.fft_layer0, a.fft_layers, a, P);
   bool mismatch = false;
   for (uint32_t it = threadIdx.x; it < a.parity * ng; it += blockDim.x) {
     const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
@@ -716,14 +747,22 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds)
 template <bool G16, int NT, int MINW = NT / 256>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT / 256))) void leo_encode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
-  if (a.C % 4 == 0) leo_encode_body<G16, 4>(a, lds);
-  else if (a.C % 2 == 0) leo_encode_body<G16, 2>(a, lds);
-  else leo_encode_body<G16, 1>(a, lds);
+  if (a.plan_lds) {
+    const PlanPtrs P = stage_plan(a, lds + a.plan_off);
+    if (a.C % 4 == 0) leo_encode_body<G16, 4, true>(a, lds, P);
+    else if (a.C % 2 == 0) leo_encode_body<G16, 2, true>(a, lds, P);
+    else leo_encode_body<G16, 1, true>(a, lds, P);
+  } else {
+    const PlanPtrs P{a.layers, a.blocks, a.tables};
+    if (a.C % 4 == 0) leo_encode_body<G16, 4, false>(a, lds, P);
+    else if (a.C % 2 == 0) leo_encode_body<G16, 2, false>(a, lds, P);
+    else leo_encode_body<G16, 1, false>(a, lds, P);
+  }
 }
 
 // reconstruct_with_resources (leopard_gf8/decode.rs:319-429; leopard_gf16/decode.rs:816-905).
-template <bool G16, int G>
-__device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds) {
+template <bool G16, int G, bool PL>
+__device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
   constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
   const uint32_t C = a.C, n = a.n, m = a.m, ng = C / G;
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
@@ -746,7 +785,7 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds)
     lds_putv<G16, G>(lds, lane * C + wq, v);
   }
   __syncthreads();
-  run_layers<G16, G>(lds, a.ifft_layer0, a.ifft_layers, a);
+  run_layers<G16, G, PL>(lds, a.ifft_layer0, a.ifft_layers, a, P);
   // formal derivative (decode.rs:537-551): in closed form every lane j takes
   // lane j ^ XOR_{b : bit b of j clear, j + 2^b < n} lane (j + 2^b), all of
   // them before the update.
@@ -770,7 +809,7 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds)
     if (live) lds_putv<G16, G>(lds, j * C + wq, acc);
     __syncthreads();
   }
-  run_layers<G16, G>(lds, a.fft_layer0, a.fft_layers, a);
+  run_layers<G16, G, PL>(lds, a.fft_layer0, a.fft_layers, a, P);
   for (uint32_t it = threadIdx.x; it < (a.k + a.parity) * ng; it += blockDim.x) {
     const uint32_t s = it / ng, wq = (it - s * ng) * G;
     if ((shard_flags(a, s) & 3) != 2 || w0 + wq >= a.words) continue;  // missing and required
@@ -787,9 +826,17 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds)
 template <bool G16, int NT, int MINW = NT / 256>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT / 256))) void leo_decode_kernel(const LeoArgs a) {
   extern __shared__ uint32_t lds[];
-  if (a.C % 4 == 0) leo_decode_body<G16, 4>(a, lds);
-  else if (a.C % 2 == 0) leo_decode_body<G16, 2>(a, lds);
-  else leo_decode_body<G16, 1>(a, lds);
+  if (a.plan_lds) {
+    const PlanPtrs P = stage_plan(a, lds + a.plan_off);
+    if (a.C % 4 == 0) leo_decode_body<G16, 4, true>(a, lds, P);
+    else if (a.C % 2 == 0) leo_decode_body<G16, 2, true>(a, lds, P);
+    else leo_decode_body<G16, 1, true>(a, lds, P);
+  } else {
+    const PlanPtrs P{a.layers, a.blocks, a.tables};
+    if (a.C % 4 == 0) leo_decode_body<G16, 4, false>(a, lds, P);
+    else if (a.C % 2 == 0) leo_decode_body<G16, 2, false>(a, lds, P);
+    else leo_decode_body<G16, 1, false>(a, lds, P);
+  }
 }
 
 // Error locator in closed form: err[i] = sum_{j erased} log[i ^ j] mod (2^n - 1)
@@ -876,15 +923,18 @@ static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopar
 // max(64, 4096 / rows) -- a layer then has rows * C / 8 >= 512 four-word items
 // for the threads even when the codec has few work lanes -- or the smaller
 // `want` a dispatched variant asks for.
-static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0) {
+static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0,
+                           size_t reserve = 0) {
   const uint32_t wb = s.gf16 ? 8 : 4;
-  const uint64_t usable = s.lds_limit - 1024;  // room for the GF16 layout's block padding
+  // room for the GF16 layout's block padding and for a plan copied into LDS
+  const uint64_t usable = s.lds_limit - 1024 - reserve;
   uint64_t C = usable / ((uint64_t)rows * wb);
   const uint64_t cap = std::max<uint64_t>(64, 4096 / rows);
   if (C > cap) C = cap;
   // Default: two workgroups per CU when that still leaves 8 words a lane -- a
   // lone workgroup (one wave per SIMD) leaves its table loads' latency exposed.
-  const uint64_t half = usable / 2 / ((uint64_t)rows * wb);
+  const uint64_t half_room = (s.lds_limit - 1024) / 2;
+  const uint64_t half = half_room > reserve ? (half_room - reserve) / ((uint64_t)rows * wb) : 0;
   if (!want && C > half && half >= 8) C = half;
   if (want && C > want) C = want;
   if (C < 1) C = 1;
@@ -908,6 +958,30 @@ static int leo_threads(const LeoState& s, size_t lds, bool decode) {
   const bool one_fills = 2 * lds > s.lds_limit;
   if (decode) return one_fills ? 1024 : 512;
   return one_fills ? 512 : 256;
+}
+
+// Bytes of LDS a plan takes when copied there, or 0 for a plan kept in global
+// memory.  Up to 16 KiB (small codecs, whose tiles are small) the copy replaces
+// each layer's chain of dependent global loads (layer, block, tables) with LDS
+// reads.
+constexpr size_t kPlanLdsMax = 16 << 10;
+static size_t plan_lds_bytes(const LeoState& s, const LeoDevPlan& p, uint32_t rows) {
+  const size_t b = ((size_t)p.n_layers + p.n_blocks) * 32 + (size_t)p.n_table_words * 4;
+  if (b > kPlanLdsMax) return 0;
+  // the tile keeps at least 8 words a lane beside it
+  const size_t wb = s.gf16 ? 8 : 4;
+  return s.lds_limit > 1024 + b && (s.lds_limit - 1024 - b) / (rows * wb) >= 8 ? b : 0;
+}
+
+// Puts the plan behind `data_bytes` of LDS when it is small; returns the total.
+static size_t place_plan(const LeoDevPlan& p, size_t plan_bytes, size_t data_bytes, LeoArgs& a) {
+  if (!plan_bytes) return data_bytes;
+  a.plan_lds = 1;
+  a.plan_off = (uint32_t)((data_bytes + 15) / 16 * 4);
+  a.plan_layers = p.n_layers;
+  a.plan_blocks = p.n_blocks;
+  a.plan_tab_words = p.n_table_words;
+  return (size_t)a.plan_off * 4 + plan_bytes;
 }
 
 static bool all_aligned16(const std::vector<const uint8_t*>& ptrs) {
@@ -1123,7 +1197,8 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
                          int32_t* flag, hipStream_t s, int variant) {
   LeoState& st = state(c);
   const uint64_t words = len / (st.gf16 ? 8 : 4);
-  const uint32_t C = tile_words(st, 2 * st.m, words, variant_tile(variant));
+  const size_t plan_bytes = plan_lds_bytes(st, st.enc, 2 * st.m);
+  const uint32_t C = tile_words(st, 2 * st.m, words, variant_tile(variant), plan_bytes);
   LeoArgs a{};
   size_t off = 0, bytes = 0;
   int rc = pass_args(a, ptrs, nullptr, s, &off, &bytes);
@@ -1143,8 +1218,10 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   a.flag = flag;
   a.mode = (uint32_t)mode;
   a.vec = all_aligned16(ptrs) ? 1 : 0;
-  const size_t lds = st.gf16 && C % 4 == 0 ? (size_t)2 * gf16_lds_dwords(st.m * C) * 4
-                                            : (size_t)2 * st.m * C * (st.gf16 ? 8 : 4);
+  const size_t lds = place_plan(st.enc, plan_bytes,
+                                st.gf16 && C % 4 == 0 ? (size_t)2 * gf16_lds_dwords(st.m * C) * 4
+                                                      : (size_t)2 * st.m * C * (st.gf16 ? 8 : 4),
+                                a);
   const unsigned grid = (unsigned)((words + C - 1) / C);
   const int nt = leo_threads(st, lds, false);
   if (!st.gf16) hipLaunchKernelGGL((leo_encode_kernel<false, 512>), dim3(grid), dim3(nt), lds, s, a);
@@ -1270,7 +1347,8 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   int rc = decode_entry(c, pres, s, e);
   if (rc) return rc;
   const uint64_t words = len / (st.gf16 ? 8 : 4);
-  const uint32_t C = tile_words(st, st.n, words, variant_tile(variant));
+  const size_t plan_bytes = plan_lds_bytes(st, e->plan, st.n);
+  const uint32_t C = tile_words(st, st.n, words, variant_tile(variant), plan_bytes);
   std::vector<const uint8_t*> ptrs(total);
   for (size_t i = 0; i < total; ++i) ptrs[i] = shards[i].ptr;
   LeoArgs a{};
@@ -1293,8 +1371,10 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   a.fft_layer0 = e->n_ifft;
   a.fft_layers = e->n_fft;
   a.vec = all_aligned16(ptrs) ? 1 : 0;
-  const size_t lds = st.gf16 && C % 4 == 0 ? (size_t)gf16_lds_dwords(st.n * C) * 4
-                                            : (size_t)st.n * C * (st.gf16 ? 8 : 4);
+  const size_t lds = place_plan(e->plan, plan_bytes,
+                                st.gf16 && C % 4 == 0 ? (size_t)gf16_lds_dwords(st.n * C) * 4
+                                                      : (size_t)st.n * C * (st.gf16 ? 8 : 4),
+                                a);
   const unsigned grid = (unsigned)((words + C - 1) / C);
   const int nt = leo_threads(st, lds, true);
   if (!st.gf16) hipLaunchKernelGGL((leo_decode_kernel<false, 512>), dim3(grid), dim3(nt), lds, s, a);
