@@ -287,6 +287,7 @@ struct LeoState {
   LeoDevPlan enc;
   std::vector<LeoGroup> groups;
   uint32_t fft_layer0 = 0, fft_layers = 0;
+  uint32_t fused_layer0 = 0, fused_layers = 0;  // every group's IFFT in one layer list
   DeviceBuffer d_groups;
   // device copies of the field log/exp tables (for the error locator and tables)
   DeviceBuffer d_log, d_exp;
@@ -318,6 +319,9 @@ struct LeoArgs {
   // A small plan is copied into LDS at dword plan_off when the kernel starts
   // (plan_lds = 1): its layers then read blocks and tables from LDS.
   uint32_t plan_lds, plan_off, plan_layers, plan_blocks, plan_tab_words;
+  // Encode with every data group resident in LDS (group g on lanes g*m ..):
+  // one load phase and the fused IFFT layers instead of one pass per group.
+  uint32_t resident, fused_layer0, fused_layers;
   // Up to kLeoInline shards: pointers and flags travel in the kernel arguments
   // (inline_args = 1) instead of through the staged copy above, which saves a
   // host-to-device copy per call.
@@ -690,12 +694,45 @@ __device__ __forceinline__ bool wequal(Word<G16> a, Word<G16> b) {
 // groups of m data lanes are IFFT'd (skew offset m-1+group*m) and XORed into the
 // first m lanes, which are then FFT'd; lanes 0..parity are the parity.
 template <bool G16, int G, bool PL>
+__device__ __forceinline__ void leo_encode_resident(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
+  const uint32_t C = a.C, m = a.m, ng = C / G;
+  const uint64_t w0 = (uint64_t)blockIdx.x * C;
+  // group g holds data shards g*m .. g*m + min(m, k - g*m) - 1; the rest are zero
+  const uint32_t lanes = a.n_groups * m;
+  for (uint32_t it = threadIdx.x; it < lanes * ng; it += blockDim.x) {
+    const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+    Word<G16> v[G];
+    if (lane < a.k) {
+      gloadv<G16, G>(shard_ptr(a, lane), w0 + wq, a.words, a.vec != 0, v);
+    } else {
+#pragma unroll
+      for (int q = 0; q < G; ++q) v[q] = Word<G16>{};
+    }
+    lds_putv<G16, G>(lds, lane * C + wq, v);
+  }
+  __syncthreads();
+  run_layers<G16, G, PL>(lds, a.fused_layer0, a.fused_layers, a, P);
+  for (uint32_t it = threadIdx.x; it < m * ng; it += blockDim.x) {
+    Word<G16> x[G], y[G];
+    lds_getv<G16, G>(lds, it * G, x);
+    for (uint32_t g = 1; g < a.n_groups; ++g) {
+      lds_getv<G16, G>(lds, g * m * C + it * G, y);
+#pragma unroll
+      for (int q = 0; q < G; ++q) wxor(x[q], y[q]);
+    }
+    lds_putv<G16, G>(lds, it * G, x);
+  }
+  __syncthreads();
+}
+
+template <bool G16, int G, bool PL>
 __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
   const uint32_t C = a.C, m = a.m, ng = C / G;
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   uint32_t* A = lds;
   uint32_t* Bv = lds + (G16 && G == 4 ? gf16_lds_dwords(m * C) : m * C * (G16 ? 2u : 1u));
-  for (uint32_t g = 0; g < a.n_groups; ++g) {
+  if (a.resident) leo_encode_resident<G16, G, PL>(a, lds, P);
+  for (uint32_t g = 0; g < (a.resident ? 0u : a.n_groups); ++g) {
     const LeoGroup Gp = a.groups[g];
     uint32_t* dst = g == 0 ? A : Bv;
     for (uint32_t it = threadIdx.x; it < m * ng; it += blockDim.x) {
@@ -920,16 +957,16 @@ std::shared_ptr<const FieldTables> leopard8_field_tables() {
 static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopard.get()); }
 
 // Column tile of a launch: as many words as fit LDS for `rows` lanes, at most
-// max(64, 4096 / rows) -- a layer then has rows * C / 8 >= 512 four-word items
-// for the threads even when the codec has few work lanes -- or the smaller
+// max(64, 4096 / rows) (rows = cap_rows when given) -- so that a layer keeps
+// the threads busy even when the codec has few work lanes -- or the smaller
 // `want` a dispatched variant asks for.
 static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0,
-                           size_t reserve = 0) {
+                           size_t reserve = 0, uint32_t cap_rows = 0) {
   const uint32_t wb = s.gf16 ? 8 : 4;
   // room for the GF16 layout's block padding and for a plan copied into LDS
   const uint64_t usable = s.lds_limit - 1024 - reserve;
   uint64_t C = usable / ((uint64_t)rows * wb);
-  const uint64_t cap = std::max<uint64_t>(64, 4096 / rows);
+  const uint64_t cap = std::max<uint64_t>(64, 4096 / (cap_rows ? cap_rows : rows));
   if (C > cap) C = cap;
   // Default: two workgroups per CU when that still leaves 8 words a lane -- a
   // lone workgroup (one wave per SIMD) leaves its table loads' latency exposed.
@@ -1175,6 +1212,32 @@ int init_leopard(Codec& c) {
       st->groups.push_back(g);
     }
   }
+  // The groups' IFFTs as one layer list (layer j of every group, group g's
+  // blocks on lanes g*m ..), for the resident encode.  All groups have the same
+  // layer shapes (they depend on m only).
+  bool same = st->groups.size() > 1;
+  for (const LeoGroup& g : st->groups)
+    same = same && g.n_layers == st->groups[0].n_layers;
+  if (same) {
+    st->fused_layer0 = (uint32_t)pb.layers.size();
+    for (uint32_t j = 0; j < st->groups[0].n_layers; ++j) {
+      const LeoLayer L0 = pb.layers[st->groups[0].layer0 + j];
+      LeoLayer L{};
+      L.dist = L0.dist, L.cnt = L0.cnt, L.radix = L0.radix, L.inverse = L0.inverse;
+      L.block0 = (uint32_t)pb.blocks.size();
+      for (size_t g = 0; g < st->groups.size(); ++g) {
+        const LeoLayer Lg = pb.layers[st->groups[g].layer0 + j];
+        for (uint32_t b = 0; b < Lg.n_blocks; ++b) {
+          LeoBlock B = pb.blocks[Lg.block0 + b];
+          B.r += (uint32_t)g * m;
+          pb.blocks.push_back(B);
+          ++L.n_blocks;
+        }
+      }
+      pb.layers.push_back(L);
+    }
+    st->fused_layers = st->groups[0].n_layers;
+  }
   st->fft_layer0 = (uint32_t)pb.layers.size();
   st->fft_layers = pb.fft(st->parity, m, skew);
   int rc = upload_plan(pb, st->enc);
@@ -1199,6 +1262,11 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   const uint64_t words = len / (st.gf16 ? 8 : 4);
   const size_t plan_bytes = plan_lds_bytes(st, st.enc, 2 * st.m);
   const uint32_t C = tile_words(st, 2 * st.m, words, variant_tile(variant), plan_bytes);
+  // Every group resident in LDS when that keeps the same tile (small codecs,
+  // whose tile is capped rather than LDS-bound).
+  const uint32_t rrows = (uint32_t)st.groups.size() * st.m;
+  const bool resident = st.fused_layers && plan_lds_bytes(st, st.enc, rrows) == plan_bytes &&
+                        tile_words(st, rrows, words, variant_tile(variant), plan_bytes, 2 * st.m) == C;
   LeoArgs a{};
   size_t off = 0, bytes = 0;
   int rc = pass_args(a, ptrs, nullptr, s, &off, &bytes);
@@ -1218,10 +1286,14 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   a.flag = flag;
   a.mode = (uint32_t)mode;
   a.vec = all_aligned16(ptrs) ? 1 : 0;
-  const size_t lds = place_plan(st.enc, plan_bytes,
-                                st.gf16 && C % 4 == 0 ? (size_t)2 * gf16_lds_dwords(st.m * C) * 4
-                                                      : (size_t)2 * st.m * C * (st.gf16 ? 8 : 4),
-                                a);
+  a.resident = resident ? 1 : 0;
+  a.fused_layer0 = st.fused_layer0;
+  a.fused_layers = st.fused_layers;
+  const size_t data_lds =
+      resident ? (st.gf16 && C % 4 == 0 ? (size_t)gf16_lds_dwords(rrows * C) * 4 : (size_t)rrows * C * (st.gf16 ? 8 : 4))
+               : (st.gf16 && C % 4 == 0 ? (size_t)2 * gf16_lds_dwords(st.m * C) * 4
+                                        : (size_t)2 * st.m * C * (st.gf16 ? 8 : 4));
+  const size_t lds = place_plan(st.enc, plan_bytes, data_lds, a);
   const unsigned grid = (unsigned)((words + C - 1) / C);
   const int nt = leo_threads(st, lds, false);
   if (!st.gf16) hipLaunchKernelGGL((leo_encode_kernel<false, 512>), dim3(grid), dim3(nt), lds, s, a);

@@ -58,6 +58,25 @@ def test_encode_matches_oracle(bits, k, m, length):
     assert not rs.verify(sh)
 
 
+@pytest.mark.parametrize("bits,k,m,length", [(8, 10, 4, (1 << 20) + 192), (8, 32, 16, (1 << 20) + 64),
+                                             (8, 192, 64, (64 << 10) + 64), (16, 64, 16, (256 << 10) + 64),
+                                             (16, 1024, 256, 8192)])
+def test_encode_long_shards_match_oracle(bits, k, m, length):
+    """Multi-workgroup launches with 16-byte accesses and a ragged last tile: the
+    encode that holds every data group in LDS (small codecs) and the one that
+    streams the groups through two buffers (LDS-bound codecs)."""
+    announce(f"leopard gf{bits} long encode {k}+{m} len={length}")
+    rng = np.random.default_rng(bits + k + m + length)
+    data = [rng.integers(0, 256, length, dtype=np.uint8) for _ in range(k)]
+    rs = rsgpu.ReedSolomon(k, m, codec_family=FAMILY[bits])
+    sh = _encode(rs, data, m)
+    rc, exp = O.leo_encode(bits, k, m, data)
+    assert rc == 0
+    for j in range(m):
+        assert (to_host(sh[k + j]) == exp[j]).all(), j
+    assert rs.verify(sh)
+
+
 @pytest.mark.parametrize("bits,k,m", [(8, 10, 4), (8, 12, 7), (8, 128, 64), (16, 10, 4),
                                       (16, 33, 17), (16, 300, 45)])
 def test_reconstruct_random_patterns_match_oracle(bits, k, m):
