@@ -381,7 +381,9 @@ int rsgpu_mlp_train(rsgpu_mlp* mlp, const float* x, const float* t, const float*
  * [4][6]: k_min, k_max, m_min, m_max, shard_len_min, shard_len_max of the trained
  * shapes per kind (classic GF8, classic GF16, Leopard GF8, Leopard GF16; k_max 0 =
  * none); calls outside it keep the default tiling.  The network must have 14
- * inputs (the features of rsgpu/dispatch.py) and rsgpu_variant_count() outputs. */
+ * inputs (the features of rsgpu/dispatch.py) and rsgpu_variant_count() outputs,
+ * or one more: that last output predicts the default tiling, which is installed
+ * (-1) where it is strictly below every valid variant's prediction. */
 int rsgpu_codec_install_dispatch(rsgpu_codec* codec, rsgpu_mlp* net, const int64_t* envelope,
                                  void* stream);
 

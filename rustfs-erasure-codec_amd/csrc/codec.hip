@@ -1461,7 +1461,9 @@ static void dispatch_features(int kind, size_t k, size_t m, size_t len, int op, 
 
 // Fills the codec's variant table from the dispatcher network: one forward pass
 // over every (op, erasures, size bucket) and the argmin over the variants valid
-// for that call.  `envelope` ([4][6] int64: k_min, k_max, m_min, m_max, len_min,
+// for that call.  A network with one output more than there are variants
+// predicts the default tiling there too; where that is the argmin the entry
+// stays -1 (default tiling).  `envelope` ([4][6] int64: k_min, k_max, m_min, m_max, len_min,
 // len_max per kind classic-GF8, classic-GF16, Leopard GF8, Leopard GF16; a row
 // with k_max == 0 is unknown) limits it to trained shapes; other entries keep the
 // default tiling.
@@ -1472,7 +1474,7 @@ int rsgpu_codec_install_dispatch(rsgpu_codec* h, rsgpu_mlp* net, const int64_t* 
   size_t n_params = 0;
   int rc = rsgpu_mlp_dims(net, &n_in, &n_hidden, &n_out, &n_params);
   if (rc) return rc;
-  if (n_in != 14 || n_out != kNumVariants) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (n_in != 14 || (n_out != kNumVariants && n_out != kNumVariants + 1)) return RSGPU_ERR_INVALID_ARGUMENT;
   const int kind = c.family == RSGPU_FAMILY_LEOPARD_GF8    ? 2
                    : c.family == RSGPU_FAMILY_LEOPARD_GF16 ? 3
                    : c.field == RSGPU_FIELD_GF16           ? 1
@@ -1520,6 +1522,7 @@ int rsgpu_codec_install_dispatch(rsgpu_codec* h, rsgpu_mlp* net, const int64_t* 
       const float t = y[i * n_out + v];
       if (best < 0 || t < bv) best = v, bv = t;
     }
+    if (n_out == kNumVariants + 1 && (best < 0 || y[i * n_out + kNumVariants] < bv)) best = -1;
     tab[((size_t)r.op * ne + r.e) * kNumBuckets + r.b] = (int16_t)best;
   }
   // encode / verify rows are keyed by e = 0; copy them so every erasure index resolves

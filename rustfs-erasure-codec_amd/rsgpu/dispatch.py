@@ -12,10 +12,11 @@ src/core/parallel.rs:65-127).  Here the choice is learned:
   synthetic calls -- field/family, k, m, shard size, operation, erasure count --
   with HIP events around each call, so host gaps do not count;
 * ``Dispatcher.fit`` trains the fp32 network (``rsgpu.Mlp``: features -> 64 ->
-  64 -> one predicted log-time per variant) on the device with the library's
-  own forward/backward/Adam kernels;
+  64 -> one predicted log-time per variant, plus one for the default tiling) on
+  the device with the library's own forward/backward/Adam kernels;
 * ``Dispatcher.install`` evaluates the network for every (op, erasures, size
-  bucket) of a codec and installs the argmin as the codec's variant table;
+  bucket) of a codec and installs the argmin -- a variant, or -1 where the
+  default tiling is predicted fastest -- as the codec's variant table;
   shapes outside the trained envelope fall back to ``autotune`` (exhaustive
   timing), which is also the judge of the network's regret.
 
@@ -225,6 +226,11 @@ def load(path):
         return {k: z[k] for k in z.files}
 
 
+def with_default(T, D):
+    """Timings with the default tiling as one more column (always valid)."""
+    return np.concatenate([T, np.asarray(D, T.dtype)[:, None]], axis=1)
+
+
 def targets(T):
     """Per-row centred log-times and their weights (0 for invalid variants)."""
     valid = np.isfinite(T)
@@ -242,10 +248,11 @@ def split(n, frac=0.2, seed=7):
 
 
 def regret(T, D, picks):
-    """Per-row t(pick)/t(best) - 1 and summary statistics; D = default-tiling times."""
-    best = np.nanmin(T, axis=1)
+    """Per-row t(pick)/t(best) - 1 and summary statistics; D = default-tiling times,
+    picks of -1 choose the default tiling, which the best is also taken over."""
+    best = np.minimum(np.nanmin(T, axis=1), D)
     rows = np.arange(T.shape[0])
-    tp = T[rows, picks]
+    tp = np.where(picks < 0, D, T[rows, np.maximum(picks, 0)])
     r = tp / best - 1.0
     rd = D / best - 1.0
     return {
@@ -262,14 +269,16 @@ class Dispatcher:
     """The trained network plus the envelope of shapes it was trained on."""
 
     def __init__(self, seed=0):
-        self.mlp = Mlp(N_FEATURES, variant_count(), seed=seed)
+        # one output per variant, then one for the default tiling
+        self.mlp = Mlp(N_FEATURES, variant_count() + 1, seed=seed)
         self.envelope = {}  # kind -> (k_min, k_max, m_min, m_max, len_min, len_max)
 
     # -- training
-    def fit(self, X, T, meta=None, steps=3000, lr=3e-3, stream=None):
+    def fit(self, X, T, D, meta=None, steps=3000, lr=3e-3, stream=None):
+        """T: [n][variants] timings (NaN: invalid), D: [n] default-tiling timings."""
         import torch
 
-        t, w = targets(T)
+        t, w = targets(with_default(T, D))
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()  # noqa: E731
         x_d, t_d, w_d = dev(X), dev(t), dev(w)
         loss = self.mlp.train(x_d, t_d, w_d, steps, lr, stream)
@@ -302,9 +311,12 @@ class Dispatcher:
         return y.cpu().numpy()
 
     def pick(self, X, valid):
+        """Argmin over the valid variants and the default tiling (returned as -1)."""
         y = self.predict(X)
+        valid = np.concatenate([valid, np.ones((valid.shape[0], 1), bool)], axis=1)
         y = np.where(valid, y, np.inf)
-        return np.argmin(y, axis=1)
+        p = np.argmin(y, axis=1)
+        return np.where(p == valid.shape[1] - 1, -1, p)
 
     # -- persistence
     def save(self, path):
@@ -397,8 +409,9 @@ def autotune(torch, rs, kind, shard_len, op, erasures, seed=0):
 
 def evaluate_live(torch, disp, specs, seed=900):
     """Judges the network live: for each call shape, the variant its installed
-    table picks vs exhaustive timing of every valid variant (regret) and vs the
-    default tiling (gain).  Returns per-kind and overall summaries."""
+    table picks vs exhaustive timing of every valid variant and the default
+    tiling (regret) and vs the default tiling alone (gain).  Returns per-kind and
+    overall summaries."""
     rows = []
     for i, (kind, k, m, size, op, e) in enumerate(specs):
         rs = make_codec(kind, k, m)
@@ -410,7 +423,7 @@ def evaluate_live(torch, disp, specs, seed=900):
         if v != -1 and v not in vs:
             raise AssertionError(f"table picked invalid variant {v} for {(kind, k, m, size, op, e)}")
         t = case.time(vs + [-1])
-        best = min(t[x] for x in vs)
+        best = min(t[x] for x in vs + [-1])
         rows.append((kind, t[v] if v >= 0 else t[-1], t[-1], best))
         del case
     tp = np.array([r[1] for r in rows])
@@ -452,7 +465,7 @@ def _main(argv=None):
         data = load(a.data)
         tr, te = split(len(data["X"]))
         d = Dispatcher()
-        loss = d.fit(data["X"][tr], data["T"][tr], data["meta"][tr], steps=a.steps, lr=a.lr)
+        loss = d.fit(data["X"][tr], data["T"][tr], data["D"][tr], data["meta"][tr], steps=a.steps, lr=a.lr)
         valid = np.isfinite(data["T"][te])
         picks = d.pick(data["X"][te], valid)
         r = regret(data["T"][te], data["D"][te], picks)

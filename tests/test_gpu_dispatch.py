@@ -107,11 +107,12 @@ def test_dispatcher_trained_on_device_regret_and_gain():
     assert int(np.isfinite(data["T"]).sum()) > 40000
     tr, te = D.split(len(data["X"]))
     disp = D.Dispatcher(seed=0)
-    loss = disp.fit(data["X"][tr], data["T"][tr], data["meta"][tr], steps=3000, lr=3e-3)
+    loss = disp.fit(data["X"][tr], data["T"][tr], data["D"][tr], data["meta"][tr], steps=3000, lr=3e-3)
     assert loss < 0.05
     valid = np.isfinite(data["T"][te])
     picks = disp.pick(data["X"][te], valid)
-    assert valid[np.arange(len(picks)), picks].all()  # only exact variants are picked
+    # only exact variants are picked, or the default tiling (-1)
+    assert ((picks < 0) | valid[np.arange(len(picks)), np.maximum(picks, 0)]).all()
     r = D.regret(data["T"][te], data["D"][te], picks)
     print(f"held-out regret {r}", flush=True)
     assert r["mean"] <= 0.10, r
