@@ -354,10 +354,10 @@ static bool variant_valid(const Codec& c, int op, size_t erasures, int v) {
   if (v < 0 || v >= kNumVariants) return false;
   if (c.family == RSGPU_FAMILY_CLASSIC) return v < kNumApplyVariants;
   if (v >= kNumApplyVariants) return true;  // Leopard FFT column tiles
-  if (c.family != RSGPU_FAMILY_LEOPARD_GF8) return false;
-  // The matrix path rebuilds the FFT decoder's bytes only when exactly k shards
-  // are present (every input is then a codeword).
-  return op != kOpReconstruct || erasures == c.m;
+  // Leopard GF(2^8): matrix variants for every operation; reconstruct applies
+  // the generator-matrix solve (exactly k shards present) or the FFT decoder's
+  // own matrix (more), both bit-identical to the FFT decoder.
+  return c.family == RSGPU_FAMILY_LEOPARD_GF8;
 }
 
 static int size_bucket(size_t len) {
@@ -842,6 +842,77 @@ int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int
   int rc = leo_matrix_codec(c, s, mc);
   if (rc) return rc;
   return reconstruct_impl(*mc, shards, n, goal, required, n_required, s, variant);
+}
+
+// The FFT decoder is linear over the field (constant multiplies, XORs and the
+// transforms), so for one (present, target) pattern its output is M * (present
+// shards).  M is read off one decode of unit columns: present shard p holds the
+// field's identity at byte p, so target t comes back with M[t][p] at byte p.
+// Applying M then gives the decoder's bytes for any input, codewords or not.
+static int build_decoder_matrix(const Codec& c, const uint8_t* flags, hipStream_t s,
+                                std::shared_ptr<const DecoderMatrix>& out) {
+  const size_t total = c.k + c.m;
+  auto dm = std::make_shared<DecoderMatrix>();
+  for (size_t i = 0; i < total; ++i) {
+    if (flags[i] & 1) dm->inputs.push_back((int)i);
+    else if (flags[i] & 2) dm->targets.push_back((int)i);
+  }
+  const size_t np = dm->inputs.size(), nt = dm->targets.size();
+  const size_t L = std::max<size_t>(64, (np + 63) / 64 * 64);
+  std::shared_ptr<const FieldTables> ft = leopard8_field_tables();
+  DeviceBuffer buf;
+  HIP_TRY(buf.alloc(total * L));
+  std::vector<uint8_t> host(total * L, 0);
+  for (size_t p = 0; p < np; ++p) host[(size_t)dm->inputs[p] * L + p] = ft->one;
+  HIP_TRY(hipMemcpyAsync(buf.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
+  std::vector<rsgpu_slot> sl(total);
+  std::vector<uint8_t> req(total);
+  for (size_t i = 0; i < total; ++i) {
+    sl[i] = {(uint8_t*)buf.p + i * L, L, (int32_t)(flags[i] & 1)};
+    req[i] = (flags[i] & 2) ? 1 : 0;
+  }
+  int rc = leopard_reconstruct(c, sl.data(), total, 2, req.data(), total, s, -1);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(host.data(), buf.p, host.size(), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  std::vector<uint16_t> rows(nt * np);
+  for (size_t t = 0; t < nt; ++t)
+    for (size_t p = 0; p < np; ++p) rows[t * np + p] = host[(size_t)dm->targets[t] * L + p];
+  std::vector<uint32_t> ht;
+  build_tables_host(rows.data(), (int)nt, (int)np, false, ht, ft.get());
+  HIP_TRY(dm->tables.alloc(ht.size() * 4));
+  HIP_TRY(hipMemcpy(dm->tables.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice));
+  out = dm;
+  return RSGPU_OK;
+}
+
+int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, const uint8_t* flags,
+                                       size_t len, hipStream_t s, int variant) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  const std::string key((const char*)flags, total);
+  std::shared_ptr<const DecoderMatrix> dm;
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.dec_matrix.find(key);
+    if (it != c.dec_matrix.end()) dm = it->second;
+  }
+  if (!dm) {
+    int rc = build_decoder_matrix(c, flags, s, dm);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(c.mu);
+    if (c.dec_matrix.size() >= 256) c.dec_matrix.clear();
+    c.dec_matrix.emplace(key, dm);
+  }
+  std::vector<const uint8_t*> in;
+  std::vector<uint8_t*> out;
+  for (int i : dm->inputs) in.push_back(shards[i].ptr);
+  for (int t : dm->targets) out.push_back(shards[t].ptr);
+  int rc = run_apply((const uint32_t*)dm->tables.p, (int)in.size(), (int)out.size(), false, in, out, len,
+                     kWrite, nullptr, s, variant);
+  if (rc) return rc;
+  for (int t : dm->targets) shards[t].present = 1;
+  return RSGPU_OK;
 }
 
 // ------------------------------------------------------------ batch stripes

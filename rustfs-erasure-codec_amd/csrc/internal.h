@@ -190,6 +190,13 @@ struct DecodeEntry {
   ~DecodeEntry();
 };
 
+// A Leopard GF(2^8) decode as a matrix over the present shards (codec.hip,
+// leopard_decoder_matrix_reconstruct): coefficient tables for every target.
+struct DecoderMatrix {
+  DeviceBuffer tables;  // [targets][inputs][kTabWords]
+  std::vector<int> inputs, targets;
+};
+
 struct VariantTable {
   int n_erasures = 0;      // erasure counts 0 .. n_erasures-1 (larger ones clamp)
   std::vector<int16_t> v;  // [kNumOps][n_erasures][kNumBuckets]
@@ -226,6 +233,8 @@ struct Codec {
   // Leopard GF(2^8): classic codec over the generator matrix [I; G], built on the
   // first call that dispatches to a matrix-apply variant (guarded by mu).
   mutable std::shared_ptr<Codec> leo_matrix;
+  // Leopard GF(2^8): decoder matrices by (present, target) pattern (guarded by mu).
+  mutable std::unordered_map<std::string, std::shared_ptr<const DecoderMatrix>> dec_matrix;
   // Byte field of the matrix when it is not the 0x11d field (the generator-matrix
   // codec of a Leopard GF(2^8) codec), with a device copy of log + exp.
   std::shared_ptr<const FieldTables> ftab;

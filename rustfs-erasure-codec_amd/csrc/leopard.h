@@ -13,7 +13,8 @@ int leopard_encode(const Codec& c, const rsgpu_cslice* data, const rsgpu_slice* 
 int leopard_verify(const Codec& c, const rsgpu_cslice* shards, size_t len, int32_t* d_flag,
                    hipStream_t s, int variant);
 // goal: 0 all, 1 data only, 2 required flags.  A matrix-apply variant on a GF(2^8)
-// codec with exactly k shards present goes to leopard_matrix_reconstruct.
+// codec goes to leopard_matrix_reconstruct (exactly k shards present) or to
+// leopard_decoder_matrix_reconstruct (more).
 int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
                         const uint8_t* required, size_t n_required, hipStream_t s, int variant);
 
@@ -25,5 +26,11 @@ std::shared_ptr<const FieldTables> leopard8_field_tables();
 // the classic matrix-apply kernels over [I; G], where G is the codec's parity map.
 int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
                                const uint8_t* required, size_t n_required, hipStream_t s, int variant);
+
+// Reconstruct with more than k shards present as one matrix apply over them,
+// bit-identical to the FFT decoder (codec.hip).  flags[i]: bit 0 present, bit 1
+// a target; every shard is len bytes.
+int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, const uint8_t* flags,
+                                       size_t len, hipStream_t s, int variant);
 
 }  // namespace rsgpu

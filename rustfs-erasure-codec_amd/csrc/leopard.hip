@@ -1409,10 +1409,15 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   if (!need) return RSGPU_OK;
   if (total - present > c.m) return RSGPU_TOO_FEW_SHARDS_PRESENT;  // decode.rs:744-747
   LeoState& st = state(c);
-  // With exactly k shards present every input is a codeword, so the generator-
-  // matrix solve over those k shards rebuilds the same bytes as the FFT decoder.
-  if (variant >= 0 && variant < kNumApplyVariants && !st.gf16 && present == c.k)
-    return leopard_matrix_reconstruct(c, shards, n, goal, required, n_required, s, variant);
+  // Matrix variants on GF(2^8).  With exactly k shards present every input is a
+  // codeword, so the generator-matrix solve over those k shards rebuilds the same
+  // bytes as the FFT decoder; with more, the decoder's own matrix over all of
+  // them does.
+  if (variant >= 0 && variant < kNumApplyVariants && !st.gf16) {
+    if (present == c.k)
+      return leopard_matrix_reconstruct(c, shards, n, goal, required, n_required, s, variant);
+    return leopard_decoder_matrix_reconstruct(c, shards, n, flags.data(), len, s, variant);
+  }
   std::vector<uint8_t> pres(total);
   for (size_t i = 0; i < total; ++i) pres[i] = shards[i].present ? 1 : 0;
   std::shared_ptr<LeoDecodeEntry> e;

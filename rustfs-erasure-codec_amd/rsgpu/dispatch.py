@@ -144,7 +144,7 @@ class OpCase:
 def sample_specs(n, seed=0x5EED_D15C):
     """Seeded call shapes: (kind, k, m, shard_len, op, erasures)."""
     rng = np.random.default_rng(seed)
-    leo8 = [(10, 4), (16, 8), (32, 8), (64, 32), (100, 28), (128, 64), (192, 64)]
+    leo8 = [(10, 4), (16, 8), (32, 8), (32, 16), (64, 32), (100, 28), (128, 64), (192, 64)]
     leo16 = [(10, 4), (64, 16), (300, 45), (1024, 256), (4096, 1024)]
 
     def loguni(lo, hi, align):
@@ -167,7 +167,7 @@ def sample_specs(n, seed=0x5EED_D15C):
         elif u < 0.90:
             kind = "leopard_gf8"
             k, m = leo8[int(rng.integers(len(leo8)))]
-            size = loguni(4 << 10, 1 << 20, 64)
+            size = loguni(4 << 10, 2 << 20, 64)
         else:
             kind = "leopard_gf16"
             k, m = leo16[int(rng.integers(len(leo16)))]

@@ -153,6 +153,36 @@ def test_inconsistent_input_matches_oracle():
                 assert (to_host(work[i]) == exp[i]).all(), (bits, missing, i)
 
 
+@pytest.mark.parametrize("k,m,n", [(6, 4, 192), (10, 4, (64 << 10) + 64), (12, 7, 4096), (32, 16, 640)])
+def test_decoder_matrix_reconstruct_matches_oracle(k, m, n):
+    """Matrix variants with more than k shards present apply the FFT decoder's own
+    matrix over all of them: on non-codeword inputs the bytes equal the oracle's
+    decoder, for every erasure count below m, for reconstruct_data, and again
+    from the cached matrix."""
+    announce(f"leopard decoder matrix {k}+{m} len={n}")
+    rng = np.random.default_rng(k * 100 + m)
+    shards = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
+    rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf8")
+    rs.set_variant(5)  # a matrix-apply variant
+    for e in range(1, m):
+        missing = sorted(rng.choice(k + m, e, replace=False).tolist())
+        present = [0 if i in missing else 1 for i in range(k + m)]
+        rc, exp = O.leo_reconstruct(8, k, m, [s.copy() for s in shards], present)
+        assert rc == 0
+        for rep in range(2):  # the second call uses the cached matrix
+            work = [None if i in missing else to_dev(shards[i]) for i in range(k + m)]
+            rs.reconstruct(work)
+            for i in missing:
+                assert (to_host(work[i]) == exp[i]).all(), (e, missing, i, rep)
+        work = [None if i in missing else to_dev(shards[i]) for i in range(k + m)]
+        rs.reconstruct_data(work)
+        for i in missing:
+            if i < k:
+                assert (to_host(work[i]) == exp[i]).all(), (e, missing, i)
+            else:
+                assert work[i] is None
+
+
 def test_reconstruct_data_some_and_errors():
     announce("leopard reconstruct_data / errors")
     k, m, n = 8, 4, 128
