@@ -274,7 +274,9 @@ int rsgpu_batch_stats(const rsgpu_batch* batch, uint64_t* n_ops, uint64_t* bytes
  *   kind 0: matrix apply, `lanes` per workgroup, `vectors_per_lane` 16-byte
  *           vectors per lane, `rows_per_group` outputs per workgroup (0 = all);
  *           on a Leopard GF(2^8) codec this applies its generator matrix
- *           (encode, verify, and reconstruct with exactly k shards present);
+ *           (encode, verify, reconstruct with exactly k shards present) or, for
+ *           reconstruct with more present, the FFT decoder's own matrix over
+ *           every present shard (cached per erasure pattern);
  *   kind 1: Leopard FFT with a column tile of `leopard_tile_words` words.
  * A codec uses its forced variant if set, else its variant table, else the
  * default tiling; an entry that is not valid for a call falls back to the
