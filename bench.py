@@ -484,7 +484,10 @@ def main():
     torch.cuda.synchronize()
     assert torch.equal(st.buf[:, k:], ref_parity), "batched encode differs from plain encode"
     t_enc_batch = timed(torch, b_enc.graph_launch, args.steps, args.warmup)
-    d_batch = event_durations(torch, lambda i: b_enc.graph_launch(), 3)
+    # roofline: the kernel's own duration, HIP events around 10 direct launches of the
+    # same tuned batch (an event pair around a graph launch also holds the graph's
+    # start-up on the device, which a kernel trace leaves out)
+    d_batch = event_durations(torch, lambda i: b_enc.run(), 10)
     bytes_per_launch = N * (k + m) * S
     achieved = bytes_per_launch / (float(d_batch.mean()) * 1e-3) / 1e9
 
@@ -634,6 +637,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(bytes_per_launch),
                      "kernel": "batch_kernel (512 x RS 10+4 x 1 MiB encode, one launch)",
+                     "launches_timed": int(len(d_batch)),
                      "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": round(float(d_batch.mean()), 4)},
         "cpu_baseline": cpu,
