@@ -72,3 +72,51 @@ def test_slots_recycle_after_eviction():
     st = rs.cache_stats()
     assert st["entries"] <= 4
     assert st["misses"] >= 3 * len(patterns) - 4
+
+
+def test_host_threads_share_codecs():
+    """Four host threads call one classic and one Leopard GF(2^8) codec at once, each
+    on its own stream: decode-cache misses, hits and evictions (capacity 16), slot
+    recycling, and Leopard decoder matrices and plans built concurrently.  Every
+    rebuilt shard equals the original."""
+    import threading
+
+    announce("decode caches under concurrent host threads")
+    k, m, n = 10, 4, 1 << 16
+    full = _stripe(k, m, n, 7)
+    rs = rsgpu.ReedSolomon(k, m, inversion_cache_capacity=16)
+    rng0 = np.random.default_rng(8)
+    ldata = [rng0.integers(0, 256, n, dtype=np.uint8) for _ in range(k)]
+    rc, lpar = O.leo_encode(8, k, m, ldata)
+    assert rc == 0
+    lfull = np.stack(ldata + list(lpar))
+    leo = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf8")
+    leo.set_variant(5)  # matrix variant: decoder matrices for fewer than m erasures
+    dev, ldev = torch.from_numpy(full).cuda(), torch.from_numpy(lfull).cuda()
+    torch.cuda.synchronize()
+    patterns = [p for e in (1, 2) for p in itertools.combinations(range(k + m), e)]
+    errors = []
+
+    def worker(tid):
+        try:
+            s = torch.cuda.Stream()
+            rng = np.random.default_rng(100 + tid)
+            for it in range(40):
+                missing = patterns[int(rng.integers(len(patterns)))]
+                codec, src, ref = (rs, dev, full) if (it + tid) % 2 == 0 else (leo, ldev, lfull)
+                work = [None if i in missing else src[i] for i in range(k + m)]
+                codec.reconstruct(work, stream=s)
+                s.synchronize()
+                for i in missing:
+                    if not (work[i].cpu().numpy() == ref[i]).all():
+                        errors.append((tid, it, missing, i))
+        except Exception as e:  # reported below
+            errors.append((tid, repr(e)))
+
+    ts = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors[:5]
+    assert rs.cache_stats()["entries"] <= 16
