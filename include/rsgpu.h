@@ -184,6 +184,25 @@ int rsgpu_reconstruct_data(const rsgpu_codec* codec, rsgpu_slot* shards, size_t 
 int rsgpu_reconstruct_some(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
                            const uint8_t* required, size_t n_required, void* stream);
 
+/* ---- Leopard plug points: leopard_gf8_encode / leopard_gf16_encode
+ * (src/core/leopard.rs:277-296) and reconstruct_with_tables
+ * (leopard_gf8/mod.rs:546-562, leopard.rs:299-317): the FFT engines by field
+ * (bits 8 or 16), data_shards and parity_shards, without a codec handle (one is
+ * kept per shape and device for the process).  Shards are device pointers.
+ * encode: n_data / n_parity other than data_shards / parity_shards give
+ * RSGPU_TOO_FEW_SHARDS (encode.rs:201-203); the length is a non-zero multiple of
+ * 64 shared by every shard.
+ * reconstruct: present, inputs and outputs have data_shards + parity_shards
+ * entries.  A present shard's input is read; a missing shard is rebuilt into its
+ * output unless that output has length 0 (not requested).  Validation follows
+ * validate_leopard_decode_buffers (src/core/reconstruct.rs:140-185). ---- */
+int rsgpu_leopard_encode(int32_t bits, size_t data_shards, size_t parity_shards,
+                         const rsgpu_cslice* data, size_t n_data, const rsgpu_slice* parity,
+                         size_t n_parity, void* stream);
+int rsgpu_leopard_reconstruct(int32_t bits, size_t data_shards, size_t parity_shards,
+                              const uint8_t* present, const rsgpu_cslice* inputs,
+                              const rsgpu_slice* outputs, void* stream);
+
 /* ---- block streams: src/core/stream.rs:677-1031 ----
  * Shard streams are read and written through callbacks, one block at a time
  * (block_size clamped to [1 KiB, 16 MiB]; the reference's default is 4 MiB).

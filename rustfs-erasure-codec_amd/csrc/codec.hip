@@ -5,6 +5,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 
 #include "batch.h"
 #include "internal.h"
@@ -1655,6 +1657,90 @@ int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k, con
   int rc = run_apply((const uint32_t*)dt.p, (int)k, (int)r, tower, vin, vout, len, kWrite, nullptr, s);
   HIP_TRY(hipStreamSynchronize(s));  // the table buffer is freed on return
   return rc;
+}
+
+// ---- Leopard plug points (leopard_gf8_encode / leopard_gf16_encode /
+// reconstruct_with_tables, src/core/leopard.rs:277-317): the FFT engines by
+// (field, k, m), without a codec handle.  One Leopard codec per (device, field,
+// k, m) is built on first use and kept for the process, as the reference keeps
+// its tables and per-shape plans.
+static int leopard_plug_codec(int32_t bits, size_t k, size_t m, const rsgpu_codec** out) {
+  if (bits != 8 && bits != 16) return RSGPU_ERR_INVALID_ARGUMENT;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, size_t, size_t>, rsgpu_codec*> codecs;
+  std::lock_guard<std::mutex> g(mu);
+  const auto key = std::make_tuple(dev, (int)bits, k, m);
+  auto it = codecs.find(key);
+  if (it == codecs.end()) {
+    rsgpu_options o;
+    rsgpu_options_default(&o);
+    o.codec_family = bits == 8 ? RSGPU_FAMILY_LEOPARD_GF8 : RSGPU_FAMILY_LEOPARD_GF16;
+    rsgpu_codec* c = nullptr;
+    int rc = rsgpu_codec_new(k, m, &o, &c);
+    if (rc) return rc;
+    it = codecs.emplace(key, c).first;
+  }
+  *out = it->second;
+  return RSGPU_OK;
+}
+
+int rsgpu_leopard_encode(int32_t bits, size_t data_shards, size_t parity_shards, const rsgpu_cslice* data,
+                         size_t n_data, const rsgpu_slice* parity, size_t n_parity, void* stream) {
+  if ((!data && n_data) || (!parity && n_parity)) return RSGPU_ERR_INVALID_ARGUMENT;
+  // encode_with_tables (leopard_gf8/encode.rs:201-210), then the driver's length rule
+  if (n_data != data_shards || n_parity != parity_shards || n_data == 0) return RSGPU_TOO_FEW_SHARDS;
+  const size_t len = data[0].len;
+  if (len == 0 || len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
+  for (size_t i = 0; i < n_data; ++i)
+    if (data[i].len != len || !data[i].ptr) return RSGPU_INCORRECT_SHARD_SIZE;
+  for (size_t j = 0; j < n_parity; ++j)
+    if (parity[j].len != len || !parity[j].ptr) return RSGPU_INCORRECT_SHARD_SIZE;
+  const rsgpu_codec* h = nullptr;
+  int rc = leopard_plug_codec(bits, data_shards, parity_shards, &h);
+  if (rc) return rc;
+  return rsgpu_encode_sep(h, data, n_data, parity, n_parity, stream);
+}
+
+int rsgpu_leopard_reconstruct(int32_t bits, size_t data_shards, size_t parity_shards, const uint8_t* present,
+                              const rsgpu_cslice* inputs, const rsgpu_slice* outputs, void* stream) {
+  if (!present || !inputs || !outputs) return RSGPU_ERR_INVALID_ARGUMENT;
+  // validate_leopard_decode_buffers (src/core/reconstruct.rs:140-185)
+  if (data_shards == 0) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (parity_shards == 0) return RSGPU_TOO_FEW_PARITY_SHARDS;
+  const size_t total = data_shards + parity_shards;
+  size_t len = 0, n_present = 0;
+  for (size_t i = 0; i < total; ++i) {
+    if (!present[i]) continue;
+    if (!inputs[i].ptr) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+    if (inputs[i].len == 0) return RSGPU_EMPTY_SHARD;
+    if (inputs[i].len % 64 || (len && inputs[i].len != len)) return RSGPU_INCORRECT_SHARD_SIZE;
+    len = inputs[i].len;
+    ++n_present;
+  }
+  if (n_present < data_shards || !len) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  bool any_output = false;
+  for (size_t i = 0; i < total; ++i) {
+    if (outputs[i].len != 0 && outputs[i].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+    any_output |= outputs[i].len != 0;
+  }
+  // decode.rs:305-309: nothing missing or nothing requested
+  if (n_present == total || !any_output) return RSGPU_OK;
+  const rsgpu_codec* h = nullptr;
+  int rc = leopard_plug_codec(bits, data_shards, parity_shards, &h);
+  if (rc) return rc;
+  std::vector<rsgpu_slot> slots(total);
+  std::vector<uint8_t> required(total, 0);
+  for (size_t i = 0; i < total; ++i) {
+    if (present[i]) {
+      slots[i] = {const_cast<uint8_t*>(inputs[i].ptr), len, 1};
+    } else {
+      slots[i] = {outputs[i].ptr, len, 0};
+      required[i] = outputs[i].len != 0 ? 1 : 0;
+    }
+  }
+  return rsgpu_reconstruct_some(h, slots.data(), total, required.data(), total, stream);
 }
 
 // ---- host-pointer variants

@@ -113,6 +113,8 @@ _lib.rsgpu_encode_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz]
 _lib.rsgpu_verify_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz, _i32p]
 _lib.rsgpu_reconstruct_host.argtypes = [_vp, C.POINTER(_vp), _u8p, _sz, _sz, C.c_int32]
 _lib.rsgpu_apply_rows.argtypes = [C.c_int32, _u8p, _sz, _sz, C.POINTER(CSlice), C.POINTER(Slice), _vp]
+_lib.rsgpu_leopard_encode.argtypes = [C.c_int32, _sz, _sz, C.POINTER(CSlice), _sz, C.POINTER(Slice), _sz, _vp]
+_lib.rsgpu_leopard_reconstruct.argtypes = [C.c_int32, _sz, _sz, _u8p, C.POINTER(CSlice), C.POINTER(Slice), _vp]
 _lib.rsgpu_cache_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
 _lib.rsgpu_codec_format_id.argtypes = [_vp, _u8p, _sz, C.POINTER(_sz)]
 _lib.rsgpu_split_len.argtypes = [_vp, _sz, C.POINTER(_sz)]
@@ -776,6 +778,27 @@ def apply_rows(rows, inputs, outputs, field="gf8", stream=None):
     ins = (CSlice * k)(*[CSlice(*_ptr_len(t)) for t in inputs])
     outs = (Slice * r)(*[Slice(*_ptr_len(t)) for t in outputs])
     _check(_lib.rsgpu_apply_rows(f, rows_b.ctypes.data_as(_u8p), r, k, ins, outs, _stream(stream)))
+
+
+def leopard_encode(bits, data_shards, parity_shards, data, parity, stream=None):
+    """leopard_gf8_encode / leopard_gf16_encode (src/core/leopard.rs:277-296): the FFT
+    encoder by shape, without a codec object.  bits: 8 or 16."""
+    ins = (CSlice * max(1, len(data)))(*[CSlice(*_ptr_len(t)) for t in data])
+    outs = (Slice * max(1, len(parity)))(*[Slice(*_ptr_len(t)) for t in parity])
+    _check(_lib.rsgpu_leopard_encode(bits, data_shards, parity_shards, ins, len(data), outs, len(parity),
+                                     _stream(stream)))
+
+
+def leopard_reconstruct(bits, present, outputs, inputs, data_shards, parity_shards, stream=None):
+    """reconstruct_with_tables (leopard_gf8/mod.rs:546-562, leopard.rs:299-317): present[i]
+    marks shard i present (its input is read); outputs[i] receives missing shard i, None or
+    an empty tensor = not requested; inputs[i] is the present shard (None when missing)."""
+    total = len(present)
+    pres = np.ascontiguousarray([1 if p else 0 for p in present], np.uint8)
+    ins = (CSlice * total)(*[CSlice(*_ptr_len(t)) for t in inputs])
+    outs = (Slice * total)(*[Slice(*_ptr_len(t)) for t in outputs])
+    _check(_lib.rsgpu_leopard_reconstruct(bits, data_shards, parity_shards, pres.ctypes.data_as(_u8p),
+                                          ins, outs, _stream(stream)))
 
 
 class ShardByShard:

@@ -211,3 +211,51 @@ def test_prefer_leopard_mode_resolves_family():
     assert rsgpu.ReedSolomon(10, 4, leopard_mode="prefer_gf16").codec_family() == "leopard_gf16"
     assert rsgpu.ReedSolomon(10, 4, leopard_mode="as_needed").codec_family() == "classic"
     assert rsgpu.ReedSolomon(300, 100, leopard_mode="as_needed").codec_family() == "leopard_gf16"
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_plug_points_match_oracle(bits):
+    """leopard_gf8_encode / leopard_gf16_encode and reconstruct_with_tables by shape,
+    without a codec object (src/core/leopard.rs:277-317): parity equals the oracle,
+    requested outputs are rebuilt, an output of length 0 is not requested, and
+    validation follows validate_leopard_decode_buffers (reconstruct.rs:140-185)."""
+    announce(f"leopard gf{bits} plug points")
+    k, m, n = 20, 6, 4096
+    rng = np.random.default_rng(70 + bits)
+    data = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k)]
+    rc, par = O.leo_encode(bits, k, m, data)
+    assert rc == 0
+    dd = [to_dev(d) for d in data]
+    pp = [torch.zeros(n, dtype=torch.uint8, device="cuda") for _ in range(m)]
+    rsgpu.leopard_encode(bits, k, m, dd, pp)
+    for j in range(m):
+        assert (to_host(pp[j]) == par[j]).all(), j
+    full = data + list(par)
+    missing = [0, 3, k + 1, k + 4]
+    present = [i not in missing for i in range(k + m)]
+    inputs = [to_dev(full[i]) if present[i] else None for i in range(k + m)]
+    outputs = [None] * (k + m)
+    for i in missing[:3]:
+        outputs[i] = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    rsgpu.leopard_reconstruct(bits, present, outputs, inputs, k, m)
+    for i in missing[:3]:
+        assert (to_host(outputs[i]) == full[i]).all(), i
+    assert outputs[missing[3]] is None
+    # errors, in the reference's order
+    with pytest.raises(rsgpu.RsError) as ei:
+        rsgpu.leopard_encode(bits, k, m, dd[:-1], pp)
+    assert ei.value.name == "TooFewShards"
+    short = [d[:100] for d in dd]
+    with pytest.raises(rsgpu.RsError) as ei:
+        rsgpu.leopard_encode(bits, k, m, short, [p[:100] for p in pp])
+    assert ei.value.name == "IncorrectShardSize"
+    too_few = [i not in range(m + 1) for i in range(k + m)]
+    with pytest.raises(rsgpu.RsError) as ei:
+        rsgpu.leopard_reconstruct(bits, too_few, outputs, [inputs[i] if too_few[i] else None
+                                                            for i in range(k + m)], k, m)
+    assert ei.value.name == "TooFewShardsPresent"
+    bad = list(outputs)
+    bad[missing[0]] = torch.zeros(n - 64, dtype=torch.uint8, device="cuda")
+    with pytest.raises(rsgpu.RsError) as ei:
+        rsgpu.leopard_reconstruct(bits, present, bad, inputs, k, m)
+    assert ei.value.name == "IncorrectShardSize"
