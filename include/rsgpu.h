@@ -246,7 +246,9 @@ int rsgpu_reconstruct_host(const rsgpu_codec* codec, uint8_t* const* shards, uin
 /* ---- the matrix-apply primitive (SelectedEncodeKernel::execute,
  * src/galois_8/codegen.rs:46-79; code_some_slices encode.rs:327-602):
  * out_j = XOR_i rows[j*k + i] * in_i over GF(2^8) (field 0) or the tower
- * field (field 1, rows as [x, c] byte pairs). ---- */
+ * field (field 1, rows as [x, c] byte pairs).  Asynchronous on `stream`: the
+ * coefficient tables of up to 64 row sets stay on the device, so a repeated row
+ * set only enqueues the kernels; a new one is built and uploaded first. ---- */
 int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k,
                      const rsgpu_cslice* in, const rsgpu_slice* out, void* stream);
 

@@ -1629,6 +1629,44 @@ int rsgpu_reconstruct_some(const rsgpu_codec* h, rsgpu_slot* shards, size_t n,
   return reconstruct_impl(h->c, shards, n, 2, required, n_required, (hipStream_t)stream);
 }
 
+// Coefficient tables of rsgpu_apply_rows, kept on the device by (device, field,
+// shape, row bytes): callers pass the same parity rows call after call, which
+// then only enqueue the kernels (no allocation, upload or synchronisation).  At
+// most 64 row sets are kept; an evicted set is freed by hipFree, which waits for
+// the device, so no launch still reads it.
+static int apply_rows_tables(int32_t field, const uint8_t* rows, size_t r, size_t k,
+                             std::shared_ptr<const DeviceBuffer>& out) {
+  const bool tower = field == RSGPU_FIELD_GF16;
+  const size_t nbytes = r * k * (tower ? 2 : 1);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::string key(reinterpret_cast<const char*>(&dev), sizeof dev);
+  const uint64_t shape[3] = {(uint64_t)field, (uint64_t)r, (uint64_t)k};
+  key.append(reinterpret_cast<const char*>(shape), sizeof shape);
+  key.append(reinterpret_cast<const char*>(rows), nbytes);
+  static std::mutex mu;
+  static std::unordered_map<std::string, std::shared_ptr<const DeviceBuffer>> cache;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      out = it->second;
+      return RSGPU_OK;
+    }
+  }
+  std::vector<uint16_t> rr(r * k);
+  for (size_t i = 0; i < r * k; ++i) rr[i] = tower ? (uint16_t)((rows[2 * i] << 8) | rows[2 * i + 1]) : rows[i];
+  std::vector<uint32_t> ht;
+  build_tables_host(rr.data(), (int)r, (int)k, tower, ht);
+  auto dt = std::make_shared<DeviceBuffer>();
+  HIP_TRY(dt->alloc(ht.size() * 4));
+  HIP_TRY(hipMemcpy(dt->p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> g(mu);
+  if (cache.size() >= 64) cache.clear();
+  out = cache.emplace(key, std::move(dt)).first->second;
+  return RSGPU_OK;
+}
+
 int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k, const rsgpu_cslice* in,
                      const rsgpu_slice* out, void* stream) {
   if (!rows || !in || !out || r == 0 || k == 0) return RSGPU_ERR_INVALID_ARGUMENT;
@@ -1642,21 +1680,15 @@ int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k, con
     if (out[j].len != len || !out[j].ptr) return RSGPU_INCORRECT_SHARD_SIZE;
   const bool tower = field == RSGPU_FIELD_GF16;
   if (tower && (len & 1)) return RSGPU_INCORRECT_SHARD_SIZE;
-  std::vector<uint16_t> rr(r * k);
-  for (size_t i = 0; i < r * k; ++i) rr[i] = tower ? (uint16_t)((rows[2 * i] << 8) | rows[2 * i + 1]) : rows[i];
-  std::vector<uint32_t> ht;
-  build_tables_host(rr.data(), (int)r, (int)k, tower, ht);
   hipStream_t s = (hipStream_t)stream;
-  DeviceBuffer dt;
-  HIP_TRY(dt.alloc(ht.size() * 4));
-  HIP_TRY(hipMemcpyAsync(dt.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, s));
+  std::shared_ptr<const DeviceBuffer> dt;
+  int rc = apply_rows_tables(field, rows, r, k, dt);
+  if (rc) return rc;
   std::vector<const uint8_t*> vin(k);
   std::vector<uint8_t*> vout(r);
   for (size_t i = 0; i < k; ++i) vin[i] = in[i].ptr;
   for (size_t j = 0; j < r; ++j) vout[j] = out[j].ptr;
-  int rc = run_apply((const uint32_t*)dt.p, (int)k, (int)r, tower, vin, vout, len, kWrite, nullptr, s);
-  HIP_TRY(hipStreamSynchronize(s));  // the table buffer is freed on return
-  return rc;
+  return run_apply((const uint32_t*)dt->p, (int)k, (int)r, tower, vin, vout, len, kWrite, nullptr, s);
 }
 
 // ---- Leopard plug points (leopard_gf8_encode / leopard_gf16_encode /

@@ -293,3 +293,25 @@ def test_apply_rows_primitive_and_host_paths():
     damaged = [h if p else np.zeros_like(h) for h, p in zip(host, present)]
     arrs, pres = rs.reconstruct_host(damaged, present)
     assert pres.all() and all((a == h).all() for a, h in zip(arrs, host))
+
+
+def test_apply_rows_table_cache_back_to_back():
+    """rsgpu_apply_rows keeps coefficient tables per row set: 70 row sets (more than
+    the 64 kept, so the cache is cleared on the way) and a repeated set, launched back
+    to back on one stream without synchronisation, each equal the oracle."""
+    announce("apply_rows table cache")
+    rng = np.random.default_rng(21)
+    r, k, n = 2, 6, 4096 + 48
+    ins = rand_shards(rng, k, n)
+    dins = [to_dev(x) for x in ins]
+    sets = [rng.integers(0, 256, (r, k)) for _ in range(70)]
+    sets.append(sets[3])
+    outs = []
+    for rows in sets:
+        o = [torch.zeros(n, dtype=torch.uint8, device="cuda") for _ in range(r)]
+        rsgpu.apply_rows(rows, dins, o)
+        outs.append(o)
+    torch.cuda.synchronize()
+    for i, (rows, o) in enumerate(zip(sets, outs)):
+        exp = O.apply_rows(O.GF8, rows.astype(np.uint16), ins)
+        assert all((to_host(a) == e).all() for a, e in zip(o, exp)), i
