@@ -1644,8 +1644,9 @@ static int apply_rows_tables(int32_t field, const uint8_t* rows, size_t r, size_
   const uint64_t shape[3] = {(uint64_t)field, (uint64_t)r, (uint64_t)k};
   key.append(reinterpret_cast<const char*>(shape), sizeof shape);
   key.append(reinterpret_cast<const char*>(rows), nbytes);
-  static std::mutex mu;
-  static std::unordered_map<std::string, std::shared_ptr<const DeviceBuffer>> cache;
+  // never destroyed: no hipFree may run in static destruction at process exit
+  static std::mutex& mu = *new std::mutex;
+  static auto& cache = *new std::unordered_map<std::string, std::shared_ptr<const DeviceBuffer>>;
   {
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find(key);
