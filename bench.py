@@ -413,7 +413,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(stripes shard across GPUs with no exchange; see DESIGN.md)"}))
+                                   "(stripes are independent: more GPUs would each take their own "
+                                   "stripes, with no exchange between them)"}))
         sys.exit(2)
     if args.roofline_only:
         roofline_only(args)
