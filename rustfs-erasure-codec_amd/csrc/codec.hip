@@ -1701,8 +1701,9 @@ static int leopard_plug_codec(int32_t bits, size_t k, size_t m, const rsgpu_code
   if (bits != 8 && bits != 16) return RSGPU_ERR_INVALID_ARGUMENT;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
-  static std::mutex mu;
-  static std::map<std::tuple<int, int, size_t, size_t>, rsgpu_codec*> codecs;
+  // never destroyed: a host thread may still call in while the process exits
+  static std::mutex& mu = *new std::mutex;
+  static auto& codecs = *new std::map<std::tuple<int, int, size_t, size_t>, rsgpu_codec*>;
   std::lock_guard<std::mutex> g(mu);
   const auto key = std::make_tuple(dev, (int)bits, k, m);
   auto it = codecs.find(key);
