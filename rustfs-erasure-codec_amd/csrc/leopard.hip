@@ -472,16 +472,35 @@ __device__ __forceinline__ void wxor(Word<G16>& a, Word<G16> b) {
     a.v ^= b.v;
   }
 }
+// One half of a GF(2^16) product folded into acc: acc ^ (the six byte lookups
+// of the low map tl on sl and the high map th on sh), three xor3s in all.
+__device__ __forceinline__ uint32_t fold6(uint32_t acc, const uint32_t (&tl)[5], const Sel& sl,
+                                          const uint32_t (&th)[5], const Sel& sh) {
+  acc = xor3(acc, __builtin_amdgcn_perm(tl[1], tl[0], sl.s0), __builtin_amdgcn_perm(tl[3], tl[2], sl.s1));
+  acc = xor3(acc, __builtin_amdgcn_perm(tl[4], tl[4], sl.s2), __builtin_amdgcn_perm(th[1], th[0], sh.s0));
+  return xor3(acc, __builtin_amdgcn_perm(th[3], th[2], sh.s1), __builtin_amdgcn_perm(th[4], th[4], sh.s2));
+}
+// x ^= y * g^log for the twiddle's tables
+template <bool G16>
+__device__ __forceinline__ void wmul_acc(const Tab<G16>& t, Word<G16> y, Word<G16>& x) {
+  if constexpr (G16) {
+    const Sel sl = selectors(y.lo), sh = selectors(y.hi);
+    x.lo = fold6(x.lo, t.a[0], sl, t.a[2], sh);
+    x.hi = fold6(x.hi, t.a[1], sl, t.a[3], sh);
+  } else {
+    wxor(x, wmul<G16>(t, y));
+  }
+}
 // fft_dit2: x ^= y * t; y ^= x      ifft_dit2: y ^= x; x ^= y * t
 template <bool G16>
 __device__ __forceinline__ void fwd2(Word<G16>& x, Word<G16>& y, const Tab<G16>& t) {
-  wxor(x, wmul<G16>(t, y));
+  wmul_acc<G16>(t, y, x);
   wxor(y, x);
 }
 template <bool G16>
 __device__ __forceinline__ void inv2(Word<G16>& x, Word<G16>& y, const Tab<G16>& t) {
   wxor(y, x);
-  wxor(x, wmul<G16>(t, y));
+  wmul_acc<G16>(t, y, x);
 }
 
 typedef const __attribute__((address_space(4))) uint32_t* cword_t;
