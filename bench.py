@@ -520,11 +520,22 @@ def main():
             st.reconstruct(arr, sp)
 
     ref = st.buf.clone()
+
+    def poison_erased():
+        # the erased shards are overwritten with a pattern first, so a reconstruct
+        # that leaves its targets alone fails the comparison below
+        for s in range(N):
+            for i in er[s]:
+                st.buf[s, i].fill_(0xA5)
+
+    poison_erased()
     t_rec_plain = timed(torch, rec_plain, max(1, args.steps // 2), 1)
+    assert torch.equal(ref, st.buf), "plain reconstruct: rebuilt shards differ"
     b_rec = st.batch("reconstruct", sp, erasures=er)
     b_rec.autotune(reps=2)
+    poison_erased()
     t_rec_batch = timed(torch, b_rec.graph_launch, args.steps, 1)
-    assert torch.equal(ref, st.buf), "reconstruct changed stripe bytes"
+    assert torch.equal(ref, st.buf), "batched reconstruct: rebuilt shards differ"
     del ref
 
     # ---- the same encode batch with shards at an exact 1 MiB pitch
@@ -556,10 +567,17 @@ def main():
         rg = workload.RaggedSet(rsgpu, torch, spec)
         rg.make_consistent(sp)
         torch.cuda.synchronize()
+        # every output region (encode parity, erased shards) is poisoned before each
+        # timed run and compared with the consistent bytes after it
+        rg_golden = rg.output_copies()
+        rg.poison_outputs()
         t_rg_plain = timed(torch, lambda: rg.run_plain(sp), 1, 1)
+        assert rg.outputs_equal(rg_golden), "ragged plain: outputs differ"
         for (rk, rm), rcodec in rg.codecs.items():
             disp.install(rcodec, "gf8")
+        rg.poison_outputs()
         t_rg_plain_d = timed(torch, lambda: rg.run_plain(sp), 1, 1)
+        assert rg.outputs_equal(rg_golden), "ragged plain (dispatched): outputs differ"
         for rcodec in rg.codecs.values():
             rcodec.set_variant_table(None)
         b_rg = rg.make_batch(sp)
@@ -569,8 +587,11 @@ def main():
             rg.flags.zero_()
             b_rg.graph_launch()
 
+        rg.poison_outputs()
         t_rg_batch = timed(torch, rg_batch, args.steps, 1)
         assert int(rg.flags.sum()) == 0, "ragged verify reported a mismatch"
+        assert rg.outputs_equal(rg_golden), "ragged batched: outputs differ"
+        del rg_golden
         ragged = {"stripes": len(spec), "data_bytes": rg.data_bytes, "moved_bytes": rg.moved_bytes,
                   "plain_gbs": gbs(rg.data_bytes, t_rg_plain),
                   "batched_gbs": gbs(rg.data_bytes, t_rg_batch),

@@ -369,7 +369,10 @@ int rsgpu_server_outputs(const rsgpu_server* server, uint64_t tag, uint8_t** out
 int rsgpu_server_release(rsgpu_server* server, uint64_t tag);
 /* Open-loop replay: submits each request at its arrival offset (requests sorted
  * by arrival), runs the loop until all complete; out[i] is request i's record
- * (times relative to the replay start).  keep_outputs = 0 releases on completion. */
+ * (times relative to the replay start).  keep_outputs = 0 releases on completion.
+ * A request that cannot be admitted while nothing is in flight (kept outputs,
+ * the replay's or earlier ones, hold the budget or the arena) completes with
+ * status RSGPU_ERR_OUT_OF_MEMORY. */
 int rsgpu_server_replay(rsgpu_server* server, const rsgpu_request* requests, size_t n,
                         int32_t keep_outputs, rsgpu_completion* out);
 int rsgpu_server_stats(const rsgpu_server* server, rsgpu_server_counters* stats);

@@ -272,6 +272,9 @@ struct rsgpu_batch {
   size_t h_stage_bytes = 0;
   hipEvent_t staged = nullptr;
   hipStream_t stage_stream = nullptr;
+  // Recorded behind every launch that reads d_stripes / d_ptrs: a later upload
+  // on another stream waits for it before overwriting them in place.
+  hipEvent_t ran = nullptr;
   bool dirty = true;
   int device = 0, n_cu = 256;
   hipGraph_t graph = nullptr;
@@ -297,6 +300,10 @@ struct rsgpu_batch {
     if (staged) {
       (void)hipEventSynchronize(staged);
       (void)hipEventDestroy(staged);
+    }
+    if (ran) {
+      (void)hipEventSynchronize(ran);
+      (void)hipEventDestroy(ran);
     }
     if (h_stage) (void)hipHostFree(h_stage);
   }
@@ -376,6 +383,9 @@ static int upload(rsgpu_batch* b, hipStream_t s) {
   if (!b->staged && hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess)
     return RSGPU_ERR_HIP;
   if (hipEventSynchronize(b->staged) != hipSuccess) return RSGPU_ERR_HIP;  // last copies are done
+  // the copies below overwrite the tables in place: order them after the
+  // launches that still read them (on any stream)
+  if (b->ran && hipStreamWaitEvent(s, b->ran, 0) != hipSuccess) return RSGPU_ERR_HIP;
   const size_t stage = sb + pb;
   if (b->h_stage_bytes < stage) {
     const size_t want = std::max(stage, 2 * b->h_stage_bytes);
@@ -414,6 +424,13 @@ static int run_deferred(rsgpu_batch* b, hipStream_t s) {
     }
     if (rc) return rc;
   }
+  return RSGPU_OK;
+}
+
+// Records b->ran behind the launches just enqueued on s.
+static int mark_ran(rsgpu_batch* b, hipStream_t s) {
+  if (!b->ran && hipEventCreateWithFlags(&b->ran, hipEventDisableTiming) != hipSuccess) return RSGPU_ERR_HIP;
+  if (hipEventRecord(b->ran, s) != hipSuccess) return RSGPU_ERR_HIP;
   return RSGPU_OK;
 }
 
@@ -529,7 +546,7 @@ int rsgpu_batch_run(rsgpu_batch* b, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   int rc = upload(b, s);
   if (rc) return rc;
-  if ((rc = launch_main(b, s))) return rc;
+  if ((rc = launch_main(b, s)) || (rc = mark_ran(b, s))) return rc;
   return run_deferred(b, s);
 }
 
@@ -573,7 +590,7 @@ int rsgpu_batch_graph_launch(rsgpu_batch* b, void* stream) {
     set_hip_error(e, "hipGraphLaunch");
     return RSGPU_ERR_HIP;
   }
-  return RSGPU_OK;
+  return mark_ran(b, s);
 }
 
 int rsgpu_batch_stats(const rsgpu_batch* b, uint64_t* n_ops, uint64_t* bytes_moved,
@@ -612,6 +629,7 @@ int rsgpu_batch_autotune(rsgpu_batch* b, void* stream, int32_t reps, int32_t* be
       (void)hipEventRecord(e0, s);
       for (int r = 0; r < reps && !rc; ++r) rc = launch_main(b, s);
       (void)hipEventRecord(e1, s);
+      if (!rc) rc = mark_ran(b, s);
       (void)hipEventSynchronize(e1);
       float ms = 0;
       (void)hipEventElapsedTime(&ms, e0, e1);

@@ -84,7 +84,7 @@ struct Req {
   uint64_t arrival = 0, admit = 0, launch = 0, done = 0;
   int32_t status = RSGPU_OK, verify_ok = 0;
   int slot = -1, flag_idx = -1;
-  bool has_arena = false, completed = false;
+  bool has_arena = false, completed = false, admitted = false;
 };
 
 struct BatchSlot {
@@ -161,7 +161,8 @@ void complete(rsgpu_server* S, Req& q, uint64_t t) {
 void release_req(rsgpu_server* S, Req& q) {
   if (q.has_arena) S->arena.release(q.arena_off, q.out_bytes);
   q.has_arena = false;
-  S->used -= q.footprint;
+  if (q.admitted) S->used -= q.footprint;
+  q.admitted = false;
 }
 
 // Admits arrived requests in FIFO order while footprint and arena allow.
@@ -173,6 +174,7 @@ void admit(rsgpu_server* S, uint64_t t) {
     if (q.out_bytes && !S->arena.alloc(q.out_bytes, &off)) break;
     q.has_arena = q.out_bytes != 0;
     q.arena_off = off;
+    q.admitted = true;
     S->used += q.footprint;
     S->st.peak_footprint = std::max<uint64_t>(S->st.peak_footprint, S->used);
     q.admit = t;
@@ -302,6 +304,24 @@ int pump(rsgpu_server* S, uint64_t t) {
   return launch(S, t);
 }
 
+// With no batch in flight and nothing admitted, a request still at the head of
+// the queue waits on memory that only a caller's release can free: outputs kept
+// past completion hold the budget and the arena.  A replay never releases kept
+// outputs, so it completes that request with RSGPU_ERR_OUT_OF_MEMORY instead of
+// waiting forever.  Returns true when it did.
+bool fail_blocked_head(rsgpu_server* S, uint64_t t) {
+  // completions not yet polled may still release their outputs (keep_outputs = 0)
+  if (S->queue.empty() || !S->pending.empty() || !S->finished.empty()) return false;
+  for (auto& B : S->slots)
+    if (B->busy) return false;
+  Req& q = S->reqs.at(S->queue.front());
+  S->queue.pop_front();
+  q.status = RSGPU_ERR_OUT_OF_MEMORY;
+  q.admit = q.launch = t;
+  complete(S, q, t);
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -405,6 +425,7 @@ int rsgpu_server_replay(rsgpu_server* S, const rsgpu_request* reqs, size_t n, in
     }
     int rc = pump(S, t);
     if (rc) return rc;
+    progress |= fail_blocked_head(S, t);
     size_t got = 0;
     rsgpu_server_poll(S, buf.data(), buf.size(), &got);
     for (size_t i = 0; i < got; ++i) {

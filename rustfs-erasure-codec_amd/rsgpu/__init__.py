@@ -146,13 +146,23 @@ def _stream(stream):
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _torch_uint8():
+    import torch
+
+    return torch.uint8
+
+
 def _ptr_len(t):
     """(device pointer, byte length) of a uint8 device tensor view."""
     if t is None:
         return 0, 0
+    if not t.is_cuda:
+        raise ValueError("shards must be device (cuda) tensors; host pointers would reach a kernel")
+    if t.dtype != _torch_uint8():
+        raise ValueError(f"shards must be uint8 tensors, not {t.dtype}")
     if not t.is_contiguous():
         raise ValueError("shard views must be contiguous")
-    return t.data_ptr(), t.numel() * t.element_size()
+    return t.data_ptr(), t.numel()
 
 
 class ShardSlot:
@@ -793,7 +803,11 @@ def leopard_reconstruct(bits, present, outputs, inputs, data_shards, parity_shar
     """reconstruct_with_tables (leopard_gf8/mod.rs:546-562, leopard.rs:299-317): present[i]
     marks shard i present (its input is read); outputs[i] receives missing shard i, None or
     an empty tensor = not requested; inputs[i] is the present shard (None when missing)."""
-    total = len(present)
+    total = data_shards + parity_shards
+    # the C side reads data_shards + parity_shards entries of each list
+    # (IncorrectShardSize as validate_leopard_decode_buffers, reconstruct.rs:155-157)
+    if not (len(present) == len(inputs) == len(outputs) == total):
+        raise RsError(9)
     pres = np.ascontiguousarray([1 if p else 0 for p in present], np.uint8)
     ins = (CSlice * total)(*[CSlice(*_ptr_len(t)) for t in inputs])
     outs = (Slice * total)(*[Slice(*_ptr_len(t)) for t in outputs])

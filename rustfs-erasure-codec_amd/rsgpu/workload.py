@@ -86,6 +86,26 @@ class RaggedSet:
             self._ok(self.lib.rsgpu_batch_add_encode(b._h, self.codecs[(k, m)]._h, arr, k + m))
         self._ok(self.lib.rsgpu_batch_run(b._h, stream))
 
+    def output_regions(self):
+        """(offset, length) of every byte range an op writes: encode parity shards and
+        the erased shards of reconstruct ops."""
+        out = []
+        for (k, m, size, op, er), off in zip(self.spec, self.offs):
+            targets = range(k, k + m) if op == "encode" else (er if op == "reconstruct" else [])
+            out += [(off + i * (size + 256), size) for i in targets]
+        return out
+
+    def output_copies(self):
+        return [self.buf[o:o + n].clone() for o, n in self.output_regions()]
+
+    def poison_outputs(self, value=0xA5):
+        for o, n in self.output_regions():
+            self.buf[o:o + n].fill_(value)
+
+    def outputs_equal(self, copies):
+        return all(self.torch.equal(self.buf[o:o + n], c)
+                   for (o, n), c in zip(self.output_regions(), copies))
+
     def reset_slots(self):
         for (k, m, size, op, erased), a in zip(self.spec, self.args):
             if op == "reconstruct":

@@ -118,3 +118,26 @@ def test_validation_error_stays_with_its_request():
     good = [i for i in range(12) if i != bad]
     _check_results(srv, store, [comps[i] for i in good], [meta[i] for i in good])
     srv.close()
+
+
+def test_replay_returns_when_kept_outputs_fill_the_arena():
+    """Kept outputs are never released during a replay: once they fill the arena, a
+    request that cannot be admitted while nothing is in flight completes with
+    OutOfMemory instead of blocking the replay forever.  Admitted requests keep
+    their exact results."""
+    announce("serve kept outputs exceed the arena")
+    store = SV.Store(torch, per_shape=1, max_shard=64 << 10)
+    reqs, keep, meta = SV.poisson_requests(store, 16, rate=1e6, seed=4, lo=512 << 10, hi=640 << 10,
+                                           mix=(1.0, 0.0, 0.0))
+    srv = SV.Server(hbm_budget=1 << 30, arena_bytes=1 << 20, max_inflight_batches=2, max_batch_ops=4)
+    comps = srv.replay(reqs, keep_outputs=True)
+    names = [rsgpu.ERROR_NAMES[c.status] if c.status < 100 else c.status for c in comps]
+    print("statuses", names, flush=True)
+    good = [i for i, c in enumerate(comps) if c.status == 0]
+    oom = [i for i, c in enumerate(comps) if c.status == 104]  # RSGPU_ERR_OUT_OF_MEMORY
+    assert len(good) + len(oom) == 16 and oom and good
+    _check_results(srv, store, [comps[i] for i in good], [meta[i] for i in good])
+    for c in comps:
+        srv.release(c.tag)
+    assert srv.stats()["footprint"] == 0
+    srv.close()

@@ -596,6 +596,58 @@ def test_leopard_error_locator_equals_log_convolution(bits):
             assert int(el[i]) % mod == expect
 
 
+@pytest.mark.parametrize("bits,k,m,erased", [(8, 6, 4, 1), (8, 6, 4, 3), (8, 10, 4, 2), (8, 5, 3, 2),
+                                             (8, 20, 11, 5), (8, 12, 7, 7), (16, 6, 4, 1), (16, 10, 4, 3),
+                                             (16, 9, 7, 4), (16, 3, 2, 1)])
+def test_leopard_decoder_on_non_codewords_is_forney_form(bits, k, m, erased):
+    """Closed form of the Leopard decoder for ANY input, codeword or not
+    (leopard_gf8/decode.rs:290-551, leopard_gf16/decode.rs:507-1100).  Parity j
+    sits at the point w_j, data i at w_{M+i} (M = next power of two >= m); the
+    erased set R holds the missing shards' points and the padding points
+    w_m..w_{M-1}.  With E(x) = prod_{j in R} (x + w_j), the decoder multiplies
+    present values by E, interpolates over all n points (IFFT), takes the formal
+    derivative and evaluates (FFT).  Because the subspace polynomial of the n
+    points has a constant derivative, the derivative of the interpolant at an
+    erased point w_r is sum_i v_i E(w_i) / (w_r + w_i), so
+        out_r = sum_{i present} v_i * E(w_i) / (E'(w_r) * (w_r + w_i)),
+        E'(w_r) = prod_{j in R, j != r} (w_r + w_j).
+    On non-codewords this fixes every byte of the decoder's output, which round
+    trips (codewords only) leave open."""
+    mul, div = _leo_field(bits)
+    M = 1 << (m - 1).bit_length()
+    rng = np.random.default_rng(bits * 100 + k * 7 + m + erased)
+    n = 64
+    shards = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]  # not a codeword
+    missing = sorted(rng.choice(k + m, erased, replace=False).tolist())
+    present = np.ones(k + m, np.uint8)
+    present[missing] = 0
+    rc, out = O.leo_reconstruct(bits, k, m, [s.copy() for s in shards], present)
+    assert rc == 0
+
+    def point(idx):
+        return idx - k if idx >= k else M + idx
+
+    R = [point(i) for i in missing] + list(range(m, M))
+    pres = [i for i in range(k + m) if present[i]]
+    elems = {i: _leo_elements(bits, shards[i]) for i in pres}
+    for r in missing:
+        wr = point(r)
+        dr = 1
+        for j in R:
+            if j != wr:
+                dr = mul(dr, wr ^ j)
+        want = [0] * len(elems[pres[0]])
+        for i in pres:
+            wi = point(i)
+            ei = 1
+            for j in R:
+                ei = mul(ei, wi ^ j)
+            coef = div(ei, mul(dr, wr ^ wi))
+            for e, v in enumerate(elems[i]):
+                want[e] ^= mul(coef, v)
+        assert _leo_elements(bits, out[r]) == want, (r, missing)
+
+
 def test_smallrng_seeded_payload_matches_golden_checksum_path():
     # seeded_data(61, 0xA5A5_1024) feeds test_golden_split_encode; check the stream is stable
     a = O.smallrng_bytes(0xA5A51024, 61)
