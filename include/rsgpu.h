@@ -183,6 +183,21 @@ int rsgpu_reconstruct_data(const rsgpu_codec* codec, rsgpu_slot* shards, size_t 
 /* ReedSolomon::reconstruct_some (required flags of length k or k+m). */
 int rsgpu_reconstruct_some(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
                            const uint8_t* required, size_t n_required, void* stream);
+/* rsgpu_reconstruct (data_only = 0) or rsgpu_reconstruct_data (data_only = 1)
+ * without any allocation: no hipMalloc, hipFree or allocating library call, no
+ * decode-cache lookup.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  The
+ * decode solve goes into a library-owned device scratch array of 2 MiB per GPU
+ * (held until the process ends): 256 B + the coefficient tables (32 B per
+ * coefficient, 128 B for GF(2^16)) + the rows, each rounded up to 256 B, for
+ * (missing targets) x k coefficients.  Classic codecs of at most 256 shards;
+ * larger shapes return RSGPU_ERR_UNSUPPORTED, Leopard codecs
+ * RSGPU_UNSUPPORTED_CODEC_FAMILY.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams or threads, before the
+ * previous call's kernels finished) race and must be serialised by the caller.
+ * Same bytes and validation as the allocating entry points. */
+int rsgpu_reconstruct_noalloc(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
+                              int32_t data_only, void* stream);
 
 /* ---- Leopard plug points: leopard_gf8_encode / leopard_gf16_encode
  * (src/core/leopard.rs:277-296) and reconstruct_with_tables

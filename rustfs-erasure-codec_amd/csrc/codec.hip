@@ -98,7 +98,7 @@ struct Field {
 
 // Gauss-Jordan elimination with the first-nonzero-below pivot rule
 // (src/matrix.rs:373-422).  Returns false for a singular matrix.
-static bool gauss_jordan(const Field& F, std::vector<uint16_t>& w, int rows, int cols) {
+static bool gauss_jordan(const Field& F, uint16_t* w, int rows, int cols) {
   for (int r = 0; r < rows; ++r) {
     if (w[r * cols + r] == 0)
       for (int rb = r + 1; rb < rows; ++rb)
@@ -127,7 +127,7 @@ static bool invert(const Field& F, const std::vector<uint16_t>& a, int n, std::v
     for (int c = 0; c < n; ++c) w[(size_t)r * 2 * n + c] = a[(size_t)r * n + c];
     w[(size_t)r * 2 * n + n + r] = 1;
   }
-  if (!gauss_jordan(F, w, n, 2 * n)) return false;
+  if (!gauss_jordan(F, w.data(), n, 2 * n)) return false;
   out.assign((size_t)n * n, 0);
   for (int r = 0; r < n; ++r)
     for (int c = 0; c < n; ++c) out[(size_t)r * n + c] = w[(size_t)r * 2 * n + n + c];
@@ -264,15 +264,16 @@ static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 // `table_cols` (default k_total) is the column count of the table rows and
 // `col0` the table column of in[0], so a subset of a wider table can be applied.
 // mode kAccumulate XORs into the outputs from the first input on.
+// Takes plain arrays of k_total inputs and r_total outputs and allocates nothing.
 static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
-                     const std::vector<const uint8_t*>& in, const std::vector<uint8_t*>& out,
+                     const uint8_t* const* in, uint8_t* const* out,
                      size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1,
                      int table_cols = -1, int col0 = 0) {
   if (table_cols < 0) table_cols = k_total;
   const int W = words_per_coef(tower);
   bool vec_ok = true;
-  for (auto p : in) vec_ok &= aligned16(p);
-  for (auto p : out) vec_ok &= aligned16(p);
+  for (int i = 0; i < k_total; ++i) vec_ok &= aligned16(in[i]);
+  for (int j = 0; j < r_total; ++j) vec_ok &= aligned16(out[j]);
   if (mode == kVerify && k_total > kMaxIn) return RSGPU_ERR_UNSUPPORTED;  // handled by caller
   const uint64_t n16 = vec_ok ? len / 16 : 0;
   const size_t tail_begin = (size_t)n16 * 16;
@@ -318,6 +319,14 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
     }
   }
   return RSGPU_OK;
+}
+
+static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
+                     const std::vector<const uint8_t*>& in, const std::vector<uint8_t*>& out,
+                     size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1,
+                     int table_cols = -1, int col0 = 0) {
+  return run_apply(d_tables, k_total, r_total, tower, in.data(), out.data(), len, mode, flag, s, variant,
+                   table_cols, col0);
 }
 
 // ---------------------------------------------------------------- validation
@@ -524,13 +533,30 @@ struct Plan {
 // with_classic_reconstruct_plan / fill_classic_reconstruct_shape_from_suffix
 // (recovery_plan.rs:262-387) and the target selection of
 // fill_classic_reconstruct_plan (recovery_plan.rs:425-517).
-static int make_plan(const Codec& c, const rsgpu_slot* shards, size_t n, int goal,
-                     const uint8_t* required, size_t n_required, Plan& plan) {
+// Index list of fixed capacity for the planner of the allocation-free entry point.
+struct FixedIdx {
+  static constexpr size_t kCap = 256;
+  int v[kCap];
+  size_t n = 0;
+  void push_back(int x) { v[n++] = x; }
+  size_t size() const { return n; }
+  int operator[](size_t i) const { return v[i]; }
+  const int* begin() const { return v; }
+  const int* end() const { return v + n; }
+};
+
+// The planner over any index list (std::vector<int>, or FixedIdx for codecs of at
+// most FixedIdx::kCap shards).
+template <class V>
+static int plan_shards(const Codec& c, const rsgpu_slot* shards, size_t n, int goal,
+                       const uint8_t* required, size_t n_required, V& valid, V& targets,
+                       size_t& shard_len, bool& all_present) {
   const size_t total = c.k + c.m;
   if (goal == 2 && n_required != c.k && n_required != total) return RSGPU_INVALID_SHARD_FLAGS;
   if (n < total) return RSGPU_TOO_FEW_SHARDS;
   if (n > total) return RSGPU_TOO_MANY_SHARDS;
-  size_t shard_len = 0, present = 0;
+  size_t present = 0;
+  shard_len = 0;
   for (size_t i = 0; i < n; ++i) {
     if (!shards[i].present) continue;
     if (shards[i].len == 0) return RSGPU_EMPTY_SHARD;
@@ -538,30 +564,35 @@ static int make_plan(const Codec& c, const rsgpu_slot* shards, size_t n, int goa
     shard_len = shards[i].len;
     ++present;
   }
-  plan.shard_len = shard_len;
   if (present == total) {
-    plan.all_present = true;
+    all_present = true;
     return RSGPU_OK;
   }
   if (present < c.k) return RSGPU_TOO_FEW_SHARDS_PRESENT;
   int rc = check_elem_len(c, shard_len);
   if (rc) return rc;
-  for (size_t i = 0; i < n && plan.valid.size() < c.k; ++i)
-    if (shards[i].present) plan.valid.push_back((int)i);
+  for (size_t i = 0; i < n && valid.size() < c.k; ++i)
+    if (shards[i].present) valid.push_back((int)i);
   for (size_t i = 0; i < n; ++i) {
     if (shards[i].present) continue;
     bool want;
     if (goal == 2) want = i < n_required && required[i];
     else want = (i < c.k) || goal == 0;
-    if (want) plan.targets.push_back((int)i);
+    if (want) targets.push_back((int)i);
   }
-  for (int t : plan.targets) {
+  for (int t : targets) {
     if (!shards[t].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
     if (shards[t].len != shard_len) return RSGPU_INCORRECT_SHARD_SIZE;
   }
-  for (int v : plan.valid)
+  for (int v : valid)
     if (!shards[v].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
   return RSGPU_OK;
+}
+
+static int make_plan(const Codec& c, const rsgpu_slot* shards, size_t n, int goal,
+                     const uint8_t* required, size_t n_required, Plan& plan) {
+  return plan_shards(c, shards, n, goal, required, n_required, plan.valid, plan.targets, plan.shard_len,
+                     plan.all_present);
 }
 
 static std::string plan_key(const Codec& c, const rsgpu_slot* shards, const Plan& plan) {
@@ -575,20 +606,43 @@ static std::string plan_key(const Codec& c, const rsgpu_slot* shards, const Plan
 // custom matrices: the library's own constructions are MDS, so every survivor
 // set is solvable, but a user matrix may not be, and the reference reports that
 // as InvalidCustomMatrix before any output is written (reconstruct.rs:1130-1160).
-static bool custom_solvable(const Codec& c, const Plan& plan) {
+template <class V>
+static bool custom_solvable_valid(const Codec& c, const V& valid) {
   const Field F{c.field == RSGPU_FIELD_GF16};
   int p = 0;
-  while (p < (int)c.k && plan.valid[p] < (int)c.k) ++p;
+  while (p < (int)c.k && valid[p] < (int)c.k) ++p;
   int e = (int)c.k - p;
   if (e == 0) return true;
   std::vector<int> missing;
   for (int d = 0, kn = 0; d < (int)c.k; ++d) {
-    if (kn < p && plan.valid[kn] == d) ++kn;
+    if (kn < p && valid[kn] == d) ++kn;
     else missing.push_back(d);
   }
   std::vector<uint16_t> w((size_t)e * e);
   for (int r = 0; r < e; ++r)
-    for (int col = 0; col < e; ++col) w[(size_t)r * e + col] = c.matrix[(size_t)plan.valid[p + r] * c.k + missing[col]];
+    for (int col = 0; col < e; ++col) w[(size_t)r * e + col] = c.matrix[(size_t)valid[p + r] * c.k + missing[col]];
+  return gauss_jordan(F, w.data(), e, e);
+}
+
+static bool custom_solvable(const Codec& c, const Plan& plan) { return custom_solvable_valid(c, plan.valid); }
+
+// The same check on the stack (at most kMaxSolve missing data shards), for the
+// allocation-free entry point.
+static bool custom_solvable_fixed(const Codec& c, const FixedIdx& valid) {
+  const Field F{c.field == RSGPU_FIELD_GF16};
+  int p = 0;
+  while (p < (int)c.k && valid[p] < (int)c.k) ++p;
+  const int e = (int)c.k - p;
+  if (e == 0) return true;
+  if (e > kMaxSolve) return false;
+  int missing[kMaxSolve];
+  for (int d = 0, kn = 0, j = 0; d < (int)c.k; ++d) {
+    if (kn < p && valid[kn] == d) ++kn;
+    else missing[j++] = d;
+  }
+  uint16_t w[kMaxSolve * kMaxSolve];
+  for (int r = 0; r < e; ++r)
+    for (int col = 0; col < e; ++col) w[r * e + col] = c.matrix[(size_t)valid[p + r] * c.k + missing[col]];
   return gauss_jordan(F, w, e, e);
 }
 
@@ -613,15 +667,20 @@ static void cache_insert(const Codec& c, const std::string& key, const std::shar
 }
 
 // The one-workgroup solve's job for `plan` (k <= 256, at most 256 targets).
-static void fill_decode_job(const Codec& c, const Plan& plan, DecodeJob& job) {
+template <class V>
+static void fill_decode_job_lists(const Codec& c, const V& valid, const V& targets, DecodeJob& job) {
   job.k = (int)c.k;
   job.m = (int)c.m;
-  job.n_targets = (int)plan.targets.size();
+  job.n_targets = (int)targets.size();
   job.tower = c.field == RSGPU_FIELD_GF16 ? 1 : 0;
   job.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
   job.one = c.ftab ? c.ftab->one : 1;
-  for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)plan.valid[i];
-  for (size_t i = 0; i < plan.targets.size(); ++i) job.targets[i] = (uint16_t)plan.targets[i];
+  for (size_t i = 0; i < c.k; ++i) job.valid[i] = (uint16_t)valid[i];
+  for (size_t i = 0; i < targets.size(); ++i) job.targets[i] = (uint16_t)targets[i];
+}
+
+static void fill_decode_job(const Codec& c, const Plan& plan, DecodeJob& job) {
+  fill_decode_job_lists(c, plan.valid, plan.targets, job);
 }
 
 // A decode entry for `plan` on a pool slot (with c.mu held); its solve is
@@ -1021,6 +1080,91 @@ int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int g
     return rc;
   plan.keep.push_back(e);
   for (int t : p.targets) shards[t].present = 1;
+  return RSGPU_OK;
+}
+
+// ---------------------------------------------- allocation-free reconstruct
+// Device scratch of rsgpu_reconstruct_noalloc: one decode solve's status word,
+// coefficient tables and rows.  Each device holds its own copy (for the life of
+// the process); host code reaches it only through the address that
+// hipGetSymbolAddress returns for that device, kept in g_scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = (size_t)2 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+static std::atomic<uint8_t*> g_scratch_ptr[kMaxScratchDevices];
+static std::mutex g_scratch_mu;
+
+// The device address of `device`'s scratch, looked up once per device with that
+// device current.
+static int scratch_for_device(int device, uint8_t** out) {
+  if (device < 0 || device >= kMaxScratchDevices) return RSGPU_ERR_UNSUPPORTED;
+  uint8_t* p = g_scratch_ptr[device].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    p = g_scratch_ptr[device].load(std::memory_order_relaxed);
+    if (!p) {
+      int cur = 0;
+      HIP_TRY(hipGetDevice(&cur));
+      if (cur != device) HIP_TRY(hipSetDevice(device));
+      void* addr = nullptr;
+      hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+      if (cur != device) (void)hipSetDevice(cur);
+      HIP_TRY(e);
+      p = (uint8_t*)addr;
+      g_scratch_ptr[device].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return RSGPU_OK;
+}
+
+static size_t round256(size_t n) { return (n + 255) / 256 * 256; }
+
+// reconstruct / reconstruct_data with the decode solve written into the device
+// scratch instead of the decode cache: validation, planning and launches use
+// stack storage only.
+static int reconstruct_noalloc_impl(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s) {
+  if (c.family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  if (c.k + c.m > FixedIdx::kCap) {
+    set_hip_error(hipErrorInvalidValue, "reconstruct_noalloc: more than 256 shards");
+    return RSGPU_ERR_UNSUPPORTED;
+  }
+  FixedIdx valid, targets;
+  size_t len = 0;
+  bool all_present = false;
+  int rc = plan_shards(c, shards, n, goal, nullptr, 0, valid, targets, len, all_present);
+  if (rc || all_present || targets.size() == 0) return rc;
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  const size_t nt = targets.size();
+  const size_t tab_bytes = round256(nt * c.k * (size_t)words_per_coef(tower) * 4);
+  const size_t row_bytes = round256(nt * c.k * (tower ? 2 : 1));
+  if (256 + tab_bytes + row_bytes > kNoAllocScratchBytes) {
+    set_hip_error(hipErrorInvalidValue, "reconstruct_noalloc: decode tables exceed the 2 MiB scratch");
+    return RSGPU_ERR_UNSUPPORTED;
+  }
+  if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable_fixed(c, valid)) return RSGPU_INVALID_CUSTOM_MATRIX;
+  uint8_t* base = nullptr;
+  if ((rc = scratch_for_device(c.device, &base))) return rc;
+  int32_t* status = (int32_t*)base;
+  uint32_t* tables = (uint32_t*)(base + 256);
+  uint8_t* rows = base + 256 + tab_bytes;
+  DecodeJob job{};
+  fill_decode_job_lists(c, valid, targets, job);
+  hipError_t he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, rows, tables, status, s);
+  if (he != hipSuccess) {
+    set_hip_error(he, "reconstruct_noalloc decode_rows");
+    return he == hipErrorInvalidValue ? RSGPU_ERR_UNSUPPORTED : RSGPU_ERR_HIP;
+  }
+  const uint8_t* in[FixedIdx::kCap];
+  uint8_t* out[FixedIdx::kCap];
+  for (size_t i = 0; i < c.k; ++i) in[i] = shards[valid[i]].ptr;
+  for (size_t t = 0; t < nt; ++t) out[t] = shards[targets[t]].ptr;
+  size_t erased = 0;
+  for (size_t i = 0; i < n; ++i) erased += shards[i].present ? 0 : 1;
+  rc = run_apply(tables, (int)c.k, (int)nt, tower, in, out, len, kWrite, nullptr, s,
+                 pick_variant(c, kOpReconstruct, erased, len));
+  if (rc) return rc;
+  for (int t : targets) shards[t].present = 1;
   return RSGPU_OK;
 }
 
@@ -1629,6 +1773,12 @@ int rsgpu_reconstruct_some(const rsgpu_codec* h, rsgpu_slot* shards, size_t n,
   return reconstruct_impl(h->c, shards, n, 2, required, n_required, (hipStream_t)stream);
 }
 
+int rsgpu_reconstruct_noalloc(const rsgpu_codec* h, rsgpu_slot* shards, size_t n, int32_t data_only,
+                              void* stream) {
+  if (!h || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
+  return reconstruct_noalloc_impl(h->c, shards, n, data_only ? 1 : 0, (hipStream_t)stream);
+}
+
 // Coefficient tables of rsgpu_apply_rows, kept on the device by (device, field,
 // shape, row bytes): callers pass the same parity rows call after call, which
 // then only enqueue the kernels (no allocation, upload or synchronisation).  At
@@ -1824,7 +1974,8 @@ int rsgpu_init(int device) {
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return RSGPU_ERR_NO_DEVICE;
   if (device < 0 || device >= n) return RSGPU_ERR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(device));
-  return RSGPU_OK;
+  uint8_t* scratch_ptr = nullptr;  // the allocation-free entry point's scratch address
+  return device < kMaxScratchDevices ? scratch_for_device(device, &scratch_ptr) : RSGPU_OK;
 }
 
 const char* rsgpu_last_hip_error(void) { return t_hip_error.c_str(); }

@@ -109,6 +109,7 @@ _lib.rsgpu_verify_with_buffer.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER
 _lib.rsgpu_reconstruct.argtypes = [_vp, C.POINTER(Slot), _sz, _vp]
 _lib.rsgpu_reconstruct_data.argtypes = [_vp, C.POINTER(Slot), _sz, _vp]
 _lib.rsgpu_reconstruct_some.argtypes = [_vp, C.POINTER(Slot), _sz, _u8p, _sz, _vp]
+_lib.rsgpu_reconstruct_noalloc.argtypes = [_vp, C.POINTER(Slot), _sz, C.c_int32, _vp]
 _lib.rsgpu_encode_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz]
 _lib.rsgpu_verify_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz, _i32p]
 _lib.rsgpu_reconstruct_host.argtypes = [_vp, C.POINTER(_vp), _u8p, _sz, _sz, C.c_int32]
@@ -356,6 +357,17 @@ class ReedSolomon:
         req = np.ascontiguousarray(required, np.uint8)
         _check(_lib.rsgpu_reconstruct_some(self._h, slots, len(shards), req.ctypes.data_as(_u8p),
                                            req.size, _stream(stream)))
+        self._finish(shards, slots, alloc)
+
+    def reconstruct_noalloc(self, shards, data_only=False, stream=None):
+        """reconstruct / reconstruct_data through the entry point that allocates
+        nothing (decode solve in the library's per-device scratch).  Pass ShardSlots
+        or tensors: a None entry would need an allocation here, so it is refused."""
+        if any(s is None for s in shards):
+            raise ValueError("reconstruct_noalloc needs a buffer for every shard (use ShardSlot)")
+        slots, alloc = self._slots(shards)
+        _check(_lib.rsgpu_reconstruct_noalloc(self._h, slots, len(shards), int(bool(data_only)),
+                                              _stream(stream)))
         self._finish(shards, slots, alloc)
 
     # ---- host buffers (PCIe-inclusive path)
