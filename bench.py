@@ -531,6 +531,27 @@ def main():
     poison_erased()
     t_rec_plain = timed(torch, rec_plain, max(1, args.steps // 2), 1)
     assert torch.equal(ref, st.buf), "plain reconstruct: rebuilt shards differ"
+    # plain, through prepared workspaces (prepare_reconstruct_workspace once per
+    # stripe's pattern; each timed call only validates and enqueues)
+    workspaces = []
+    for s in range(N):
+        for i in range(k + m):  # the erasure layout (the plain run marked targets present)
+            slot_arrays[s][i].present = 0 if i in er[s] else 1
+        h = C.c_void_p()
+        st._ok(st.lib.rsgpu_prepare_reconstruct_workspace(st.rs._h, slot_arrays[s], k + m, sp, C.byref(h)))
+        workspaces.append(rsgpu.ReconstructWorkspace(h))
+
+    def rec_plain_ws():
+        for s in range(N):
+            arr = slot_arrays[s]
+            for i in range(k + m):
+                arr[i].present = 0 if i in er[s] else 1
+            st._ok(st.lib.rsgpu_reconstruct_with_workspace(st.rs._h, arr, k + m, workspaces[s]._h, sp))
+
+    poison_erased()
+    t_rec_plain_ws = timed(torch, rec_plain_ws, max(1, args.steps // 2), 1)
+    assert torch.equal(ref, st.buf), "plain reconstruct with workspaces: rebuilt shards differ"
+    del workspaces
     b_rec = st.batch("reconstruct", sp, erasures=er)
     b_rec.autotune(reps=2)
     poison_erased()
@@ -643,6 +664,7 @@ def main():
         "verify_plain_gbs": gbs(data_bytes, t_ver_plain),
         "reconstruct_batched_gbs": gbs(data_bytes, t_rec_batch),
         "reconstruct_plain_gbs": gbs(data_bytes, t_rec_plain),
+        "reconstruct_plain_workspace_gbs": gbs(data_bytes, t_rec_plain_ws),
         "reconstruct_batched_bytes_moved_gbs": gbs(rec_moved, t_rec_batch),
         "encode_8x3_batched_gbs": gbs(N * 8 * S, t83_batch),
         "encode_8x3_plain_gbs": gbs(N * 8 * S, t83_plain),

@@ -183,6 +183,58 @@ int rsgpu_reconstruct_data(const rsgpu_codec* codec, rsgpu_slot* shards, size_t 
 /* ReedSolomon::reconstruct_some (required flags of length k or k+m). */
 int rsgpu_reconstruct_some(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
                            const uint8_t* required, size_t n_required, void* stream);
+/* ---- prepared reconstruction: ReconstructWorkspace (src/core/reconstruct.rs:539-700,
+ * src/core/workspace.rs:85-140).  A workspace owns the plan of one erasure layout
+ * (present flags and shard length of `shards`; missing slots need no buffer here)
+ * and its device decode rows, which stay valid when the codec's cache evicts or
+ * is disabled.  Leopard codecs return RSGPU_UNSUPPORTED_CODEC_FAMILY. ---- */
+typedef struct rsgpu_reconstruct_workspace rsgpu_reconstruct_workspace;
+/* prepare_reconstruct_workspace: every missing shard. */
+int rsgpu_prepare_reconstruct_workspace(const rsgpu_codec* codec, const rsgpu_slot* shards,
+                                        size_t n_shards, void* stream,
+                                        rsgpu_reconstruct_workspace** out);
+/* prepare_reconstruct_data_workspace: missing data shards only. */
+int rsgpu_prepare_reconstruct_data_workspace(const rsgpu_codec* codec, const rsgpu_slot* shards,
+                                             size_t n_shards, void* stream,
+                                             rsgpu_reconstruct_workspace** out);
+/* prepare_reconstruct_some_workspace: the missing shards flagged in `required`
+ * (k or k+m flags, else RSGPU_INVALID_SHARD_FLAGS). */
+int rsgpu_prepare_reconstruct_some_workspace(const rsgpu_codec* codec, const rsgpu_slot* shards,
+                                             size_t n_shards, const uint8_t* required,
+                                             size_t n_required, void* stream,
+                                             rsgpu_reconstruct_workspace** out);
+/* reconstruct_with_workspace / reconstruct_with_workspace_par: validates counts,
+ * then the codec's format against the workspace's (data/parity counts:
+ * TooFew/TooMany{Data,Parity}Shards; other differences: RSGPU_INVALID_CUSTOM_MATRIX),
+ * then the layout (another missing pattern: RSGPU_INVALID_SHARD_FLAGS; another
+ * length: RSGPU_INCORRECT_SHARD_SIZE), then only enqueues the prepared launches on
+ * `stream`.  An equivalent codec (same format, same device) may use the workspace. */
+int rsgpu_reconstruct_with_workspace(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards,
+                                     const rsgpu_reconstruct_workspace* workspace, void* stream);
+/* data_targets() then parity_targets() (workspace.rs:113-122): the first
+ * min(cap, n_data + n_parity) target indices into `targets`. */
+int rsgpu_reconstruct_workspace_targets(const rsgpu_reconstruct_workspace* workspace, size_t* targets,
+                                        size_t cap, size_t* n_data, size_t* n_parity);
+void rsgpu_reconstruct_workspace_free(rsgpu_reconstruct_workspace* workspace);
+
+/* ReedSolomon::decode_idx (src/galois_8/policy.rs:543-704): progressive recovery.
+ * dst, input: k+m entries each, ptr == NULL = None (a Some of length 0 needs a
+ * non-NULL ptr).  With expect_input (n_expect == k+m flags) the k expected
+ * shards are the decode's survivors; each call may supply any subset of them as
+ * inputs, and every non-NULL destination accumulates (XOR) its row of the decode
+ * (data rows of the inverse, parity rows composed through it) over that subset,
+ * so calls over a partition of the survivors sum to the reconstruction.
+ * Destinations start zeroed by the caller.  With expect_input == NULL, partial
+ * results merge: dst[i] ^= input[i] for every non-NULL input (lengths may differ
+ * between indices).  Validation follows the reference (Leopard codecs:
+ * RSGPU_UNSUPPORTED_CODEC_FAMILY; expected count other than k:
+ * RSGPU_TOO_FEW_SHARDS_PRESENT), and every pair is validated before any XOR, so
+ * a rejected call changes nothing.  Classic GF(2^16) codecs are accepted too
+ * (the reference has the method for galois_8 only); codecs with k > 256 or more
+ * than 256 destinations return RSGPU_ERR_UNSUPPORTED.  Asynchronous on `stream`. */
+int rsgpu_decode_idx(const rsgpu_codec* codec, const rsgpu_slice* dst, size_t n_dst,
+                     const uint8_t* expect_input, size_t n_expect, const rsgpu_cslice* input,
+                     size_t n_input, void* stream);
 /* rsgpu_reconstruct (data_only = 0) or rsgpu_reconstruct_data (data_only = 1)
  * without any allocation: no hipMalloc, hipFree or allocating library call, no
  * decode-cache lookup.  No counterpart in the reference; it is for callers that

@@ -236,15 +236,18 @@ __device__ __forceinline__ void decode_rows_body(const uint16_t* __restrict__ ma
     if (tid == 0) *status = RSGPU_INVALID_CUSTOM_MATRIX;
     return;
   }
-  // rows for each target
-  for (int idx = tid; idx < job.n_targets * k; idx += blockDim.x) {
-    int t = idx / k, col = idx % k;
+  // rows for each target, over the selected survivor columns
+  const int ncols = job.n_cols > 0 ? job.n_cols : k;
+  for (int idx = tid; idx < job.n_targets * ncols; idx += blockDim.x) {
+    int t = idx / ncols, q = idx % ncols;
+    int col = job.n_cols > 0 ? job.cols[q] : q;
     int target = job.targets[t];
     uint16_t v = 0;
     if (target < k) {
       int j = 0;
       while (j < e && missing[j] != target) ++j;
-      v = wm[j * cols + e + col];  // [C^-1 B | C^-1] row j, column col
+      if (j < e) v = wm[j * cols + e + col];  // [C^-1 B | C^-1] row j, column col
+      else v = job.valid[col] == target ? (uint16_t)job.one : (uint16_t)0;  // a surviving data shard
     } else {
       const uint16_t* prow = matrix + (size_t)target * k;
       if (col < p) v = prow[job.valid[col]];
@@ -254,7 +257,7 @@ __device__ __forceinline__ void decode_rows_body(const uint16_t* __restrict__ ma
   }
   __syncthreads();
   const int W = job.tower ? 4 * kTabWords : kTabWords;
-  for (int idx = tid; idx < job.n_targets * k; idx += blockDim.x) {
+  for (int idx = tid; idx < job.n_targets * ncols; idx += blockDim.x) {
     uint16_t v = rows[idx];
     uint32_t* dst = tables_out + (size_t)idx * W;
     if (job.tower) {
@@ -294,7 +297,8 @@ size_t decode_rows_lds(const DecodeJob& job) {
   while (p < job.k && job.valid[p] < job.k) ++p;
   const size_t e = (size_t)(job.k - p);
   if (e > (size_t)kMaxSolve) return 0;
-  return (e * (e + p + e) + (size_t)job.n_targets * job.k) * sizeof(uint16_t);
+  const size_t ncols = job.n_cols > 0 ? (size_t)job.n_cols : (size_t)job.k;
+  return (e * (e + p + e) + (size_t)job.n_targets * ncols) * sizeof(uint16_t);
 }
 
 hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
@@ -439,7 +443,8 @@ __global__ __launch_bounds__(256) void decode_rows_big_kernel(const uint16_t* __
     int d = col;  // the col-th present data index
     for (int j = 0; j < e; ++j)
       if (job.missing[j] <= d) ++d;
-    v = target < k ? 0 : prow[d];
+    // a data target's own column: the unit row of a surviving data shard
+    v = target < k ? (d == target ? (uint16_t)job.one : (uint16_t)0) : prow[d];
     for (int i = 0; i < e; ++i) v ^= F.mul(w[i], matrix[(size_t)job.par_surv[i] * k + d]);
   } else {
     v = w[col - p];
@@ -485,6 +490,38 @@ hipError_t launch_compare_rows(const CompareArgs& c, hipStream_t s) {
   if (c.rows <= 0 || c.len == 0) return hipSuccess;
   const uint64_t blocks = std::min<uint64_t>((c.len + kBlock - 1) / kBlock, 1024);
   hipLaunchKernelGGL(compare_rows_kernel, dim3((unsigned)blocks, (unsigned)c.rows), dim3(kBlock), 0, s, c);
+  return hipGetLastError();
+}
+
+// Pair blockIdx.y: 16-byte vectors when both ends are aligned, then the byte tail.
+__global__ __launch_bounds__(kBlock) void xor_pairs_kernel(const XorPairs p) {
+  const int y = blockIdx.y;
+  const uint8_t* src = p.src[y];
+  uint8_t* dst = p.dst[y];
+  const uint64_t len = p.len[y];
+  const bool vec = (((uintptr_t)src | (uintptr_t)dst) & 15) == 0;
+  const uint64_t n16 = vec ? len / 16 : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (uint64_t i = t; i < n16; i += stride) {
+    uint4 a = reinterpret_cast<const uint4*>(src)[i];
+    uint4 b = reinterpret_cast<uint4*>(dst)[i];
+    b.x ^= a.x;
+    b.y ^= a.y;
+    b.z ^= a.z;
+    b.w ^= a.w;
+    reinterpret_cast<uint4*>(dst)[i] = b;
+  }
+  for (uint64_t i = n16 * 16 + t; i < len; i += stride) dst[i] ^= src[i];
+}
+
+hipError_t launch_xor_pairs(const XorPairs& p, hipStream_t s) {
+  if (p.n <= 0 || p.n > kXorPairs) return p.n == 0 ? hipSuccess : hipErrorInvalidValue;
+  uint64_t longest = 0;
+  for (int i = 0; i < p.n; ++i) longest = std::max<uint64_t>(longest, p.len[i]);
+  if (longest == 0) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((longest / 16 + kBlock) / kBlock, 2048);
+  hipLaunchKernelGGL(xor_pairs_kernel, dim3((unsigned)blocks, (unsigned)p.n), dim3(kBlock), 0, s, p);
   return hipGetLastError();
 }
 

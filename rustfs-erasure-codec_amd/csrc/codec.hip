@@ -72,7 +72,12 @@ void DecodePool::release(const Slot& s) {
 }
 
 DecodeEntry::~DecodeEntry() {
-  if (pool && slot.mem) pool->release(slot);
+  if (pool && slot.mem) {
+    pool->release(slot);
+  } else if (own && slot.ready) {
+    (void)hipEventSynchronize(slot.ready);  // the buffer is freed below
+    (void)hipEventDestroy(slot.ready);
+  }
 }
 
 hipError_t DeviceBuffer::alloc(size_t n) {
@@ -550,7 +555,7 @@ struct FixedIdx {
 template <class V>
 static int plan_shards(const Codec& c, const rsgpu_slot* shards, size_t n, int goal,
                        const uint8_t* required, size_t n_required, V& valid, V& targets,
-                       size_t& shard_len, bool& all_present) {
+                       size_t& shard_len, bool& all_present, bool check_buffers = true) {
   const size_t total = c.k + c.m;
   if (goal == 2 && n_required != c.k && n_required != total) return RSGPU_INVALID_SHARD_FLAGS;
   if (n < total) return RSGPU_TOO_FEW_SHARDS;
@@ -580,6 +585,7 @@ static int plan_shards(const Codec& c, const rsgpu_slot* shards, size_t n, int g
     else want = (i < c.k) || goal == 0;
     if (want) targets.push_back((int)i);
   }
+  if (!check_buffers) return RSGPU_OK;
   for (int t : targets) {
     if (!shards[t].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
     if (shards[t].len != shard_len) return RSGPU_INCORRECT_SHARD_SIZE;
@@ -685,23 +691,37 @@ static void fill_decode_job(const Codec& c, const Plan& plan, DecodeJob& job) {
 
 // A decode entry for `plan` on a pool slot (with c.mu held); its solve is
 // enqueued on s by the caller, who records slot.ready behind it.
+// `n_coefs` (default: targets x k) is the number of row coefficients the solve
+// writes; an entry with more than the pool's m x k gets a buffer of its own.
 static int new_decode_entry(const Codec& c, const Plan& plan, const std::string& key, hipStream_t s,
-                            std::shared_ptr<DecodeEntry>& out) {
+                            std::shared_ptr<DecodeEntry>& out, size_t n_coefs = 0) {
   const bool tower = c.field == RSGPU_FIELD_GF16;
-  const size_t max_tab = c.m * c.k * words_per_coef(tower) * 4;
+  const size_t W4 = (size_t)words_per_coef(tower) * 4;
+  const size_t max_tab = c.m * c.k * W4;
   const size_t max_e = std::min<size_t>(std::min(c.k, c.m), kMaxSolve);
   if (!c.decode_pool) {  // one slot size per codec: tables and rows of m targets, C^-1
     c.decode_pool = std::make_shared<DecodePool>();
     c.decode_pool->slot_bytes = (max_tab + c.m * c.k * 2 + max_e * max_e * 2 + 255) / 256 * 256;
   }
+  if (!n_coefs) n_coefs = plan.targets.size() * c.k;
   auto e = std::make_shared<DecodeEntry>();
   e->valid = plan.valid;
   e->targets = plan.targets;
-  HIP_TRY(c.decode_pool->acquire(e->slot));
-  e->pool = c.decode_pool;
-  e->tables = (uint32_t*)e->slot.mem;
-  e->rows = (uint8_t*)e->slot.mem + max_tab;
-  e->bytes = c.decode_pool->slot_bytes + key.size();
+  if (n_coefs <= c.m * c.k) {
+    HIP_TRY(c.decode_pool->acquire(e->slot));
+    e->pool = c.decode_pool;
+    e->tables = (uint32_t*)e->slot.mem;
+    e->rows = (uint8_t*)e->slot.mem + max_tab;
+    e->bytes = c.decode_pool->slot_bytes + key.size();
+  } else {
+    const size_t tab = (n_coefs * W4 + 255) / 256 * 256;
+    e->own = std::make_unique<DeviceBuffer>();
+    HIP_TRY(e->own->alloc(tab + n_coefs * 2));
+    HIP_TRY(hipEventCreateWithFlags(&e->slot.ready, hipEventDisableTiming));
+    e->tables = (uint32_t*)e->own->p;
+    e->rows = (uint8_t*)e->own->p + tab;
+    e->bytes = e->own->bytes + key.size();
+  }
   e->solve_stream = s;
   out = std::move(e);
   return RSGPU_OK;
@@ -895,6 +915,232 @@ static int reconstruct_impl(const Codec& c, rsgpu_slot* shards, size_t n, int go
   // slot is reused only after a device-wide synchronisation.
   for (int t : plan.targets) shards[t].present = 1;
   return RSGPU_OK;
+}
+
+}  // namespace rsgpu
+
+// ------------------------------------------------- prepared reconstruction
+// ReconstructWorkspace (src/core/workspace.rs:85-140): the effective format it
+// was prepared for, the validated erasure layout, the targets and the device
+// decode entry, owned by the workspace so a cache eviction does not drop it.
+struct rsgpu_reconstruct_workspace {
+  size_t k = 0, m = 0;
+  int field = 0, family = 0;
+  std::vector<uint16_t> matrix;
+  std::vector<uint8_t> present;  // prepared layout: 1 = present
+  size_t shard_len = 0;
+  bool all_present = false;
+  std::vector<int> valid, targets;  // targets: data ascending, then parity ascending
+  std::shared_ptr<rsgpu::DecodeEntry> entry;
+};
+
+namespace rsgpu {
+
+// prepare_reconstruct_workspace_with_goal (reconstruct.rs:598-611): counts,
+// family, the shape of the present lengths, then the plan and its decode solve.
+static int prepare_workspace(const Codec& c, const rsgpu_slot* shards, size_t n, int goal,
+                             const uint8_t* required, size_t n_required, hipStream_t s,
+                             rsgpu_reconstruct_workspace** out) {
+  const size_t total = c.k + c.m;
+  if (goal == 2 && n_required != c.k && n_required != total) return RSGPU_INVALID_SHARD_FLAGS;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  if (c.family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  Plan plan;
+  int rc = plan_shards(c, shards, n, goal, required, n_required, plan.valid, plan.targets, plan.shard_len,
+                       plan.all_present, false);
+  if (rc) return rc;
+  auto w = std::make_unique<rsgpu_reconstruct_workspace>();
+  w->k = c.k;
+  w->m = c.m;
+  w->field = c.field;
+  w->family = c.family;
+  w->matrix = c.matrix;
+  w->present.resize(total);
+  for (size_t i = 0; i < total; ++i) w->present[i] = shards[i].present ? 1 : 0;
+  w->shard_len = plan.shard_len;
+  w->all_present = plan.all_present;
+  w->valid = plan.valid;
+  w->targets = plan.targets;
+  // an empty request is a validated no-op without a solve (recovery_plan.rs:488-495)
+  if (!plan.all_present && !plan.targets.empty())
+    if ((rc = get_decode_entry(c, shards, plan, s, w->entry))) return rc;
+  *out = w.release();
+  return RSGPU_OK;
+}
+
+// validate_reconstruct_workspace (reconstruct.rs:669-700) then the prepared
+// launches: no planning, cache lookup or solve here.
+static int reconstruct_with_workspace_impl(const Codec& c, rsgpu_slot* shards, size_t n,
+                                           const rsgpu_reconstruct_workspace& w, hipStream_t s) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  if (c.family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  if (w.k < c.k) return RSGPU_TOO_FEW_DATA_SHARDS;
+  if (w.k > c.k) return RSGPU_TOO_MANY_DATA_SHARDS;
+  if (w.m < c.m) return RSGPU_TOO_FEW_PARITY_SHARDS;
+  if (w.m > c.m) return RSGPU_TOO_MANY_PARITY_SHARDS;
+  if (w.field != c.field || w.family != c.family || w.matrix != c.matrix) return RSGPU_INVALID_CUSTOM_MATRIX;
+  // validate_classic_plan_layout (recovery_plan.rs:390-410)
+  size_t len = 0;
+  bool same = true;
+  for (size_t i = 0; i < total; ++i) {
+    const bool present = shards[i].present != 0;
+    if (present) {
+      if (shards[i].len == 0) return RSGPU_EMPTY_SHARD;
+      if (len != 0 && shards[i].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+      len = shards[i].len;
+    }
+    same &= present == (w.present[i] != 0);
+  }
+  if (!same) return RSGPU_INVALID_SHARD_FLAGS;
+  if (len != w.shard_len) return RSGPU_INCORRECT_SHARD_SIZE;
+  if (w.all_present || w.targets.empty()) return RSGPU_OK;
+  for (int t : w.targets) {
+    if (!shards[t].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
+    if (shards[t].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+  }
+  const size_t nt = w.targets.size();
+  const uint8_t* in_buf[kMaxIn];
+  uint8_t* out_buf[kMaxOut];
+  std::vector<const uint8_t*> vin;
+  std::vector<uint8_t*> vout;
+  const uint8_t** pin = in_buf;
+  uint8_t** pout = out_buf;
+  if (c.k > (size_t)kMaxIn || nt > (size_t)kMaxOut) {  // wide codecs: heap lists
+    vin.resize(c.k);
+    vout.resize(nt);
+    pin = vin.data();
+    pout = vout.data();
+  }
+  for (size_t i = 0; i < c.k; ++i) {
+    pin[i] = shards[w.valid[i]].ptr;
+    if (!pin[i]) return RSGPU_ERR_INVALID_ARGUMENT;
+  }
+  for (size_t t = 0; t < nt; ++t) pout[t] = shards[w.targets[t]].ptr;
+  int rc = wait_ready(*w.entry, s);
+  if (rc) return rc;
+  rc = run_apply(w.entry->tables, (int)c.k, (int)nt, c.field == RSGPU_FIELD_GF16, pin, pout, len, kWrite,
+                 nullptr, s, pick_variant(c, kOpReconstruct, total - c.k, len));
+  if (rc) return rc;
+  for (int t : w.targets) shards[t].present = 1;
+  return RSGPU_OK;
+}
+
+// ------------------------------------------------------- progressive recovery
+// decode_idx (src/galois_8/policy.rs:543-704).  With expect != NULL the k
+// expected survivors define the decode; this call's inputs are a subset of them
+// and each requested destination accumulates sum_col row[col] * input[col] over
+// that subset, so calls over a partition of the survivors add up to the
+// reconstruction.  With expect == NULL partial results merge by XOR.  NULL ptr =
+// None.  Every pair or length is validated before any device work.
+static int decode_idx_impl(const Codec& c, const rsgpu_slice* dst, size_t n_dst, const uint8_t* expect,
+                           size_t n_expect, const rsgpu_cslice* input, size_t n_input, hipStream_t s) {
+  if (c.family != RSGPU_FAMILY_CLASSIC) return RSGPU_UNSUPPORTED_CODEC_FAMILY;
+  const size_t total = c.k + c.m;
+  for (size_t count : {n_dst, n_input}) {  // validate_opt_shard_count (policy.rs:40-48)
+    if (count < total) return RSGPU_TOO_FEW_SHARDS;
+    if (count > total) return RSGPU_TOO_MANY_SHARDS;
+  }
+  if (!expect) {  // merge mode (policy.rs:675-700): validate every pair, then XOR
+    for (size_t i = 0; i < total; ++i) {
+      if (!input[i].ptr) continue;
+      if (!dst[i].ptr) return RSGPU_TOO_FEW_SHARDS;
+      if (dst[i].len != input[i].len) return RSGPU_INCORRECT_SHARD_SIZE;
+    }
+    XorPairs xp{};
+    for (size_t i = 0; i <= total; ++i) {
+      if (i < total && input[i].ptr && input[i].len) {
+        xp.src[xp.n] = input[i].ptr;
+        xp.dst[xp.n] = dst[i].ptr;
+        xp.len[xp.n] = input[i].len;
+        ++xp.n;
+      }
+      if (xp.n == kXorPairs || (i == total && xp.n)) {
+        HIP_TRY(launch_xor_pairs(xp, s));
+        xp.n = 0;
+      }
+    }
+    return RSGPU_OK;
+  }
+  if (n_expect != total) return RSGPU_INVALID_SHARD_FLAGS;
+  std::vector<int> valid;
+  for (size_t i = 0; i < total; ++i)
+    if (expect[i]) valid.push_back((int)i);
+  if (valid.size() < c.k) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  // the shard length: the first input, else the first destination (policy.rs:579-583)
+  size_t len = 0;
+  bool found = false;
+  for (size_t i = 0; i < total && !found; ++i)
+    if (input[i].ptr) len = input[i].len, found = true;
+  for (size_t i = 0; i < total && !found; ++i)
+    if (dst[i].ptr) len = dst[i].len, found = true;
+  if (!found) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  for (size_t i = 0; i < total; ++i)
+    if (input[i].ptr && input[i].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+  for (size_t i = 0; i < total; ++i)
+    if (dst[i].ptr && dst[i].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+  std::vector<int> outputs, cols;
+  for (size_t i = 0; i < total; ++i)
+    if (dst[i].ptr) outputs.push_back((int)i);
+  if (outputs.empty()) return RSGPU_OK;
+  for (size_t col = 0; col < c.k; ++col)
+    if (input[valid[col]].ptr) cols.push_back((int)col);
+  if (cols.empty()) return RSGPU_OK;
+  // get_data_decode_matrix takes exactly k survivors (reconstruct.rs:1181-1183)
+  if (valid.size() != c.k) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable_valid(c, valid)) return RSGPU_INVALID_CUSTOM_MATRIX;
+  if (len == 0) return RSGPU_OK;  // decode_idx_accumulate_reduced_outputs: nothing to add
+  int rc = check_elem_len(c, len);
+  if (rc) return rc;
+  if (c.k > 256 || outputs.size() > 256) return RSGPU_ERR_UNSUPPORTED;  // one-workgroup solve
+  // cached rows: key = expected set, destinations and the columns of this call
+  std::string key = "x";
+  for (size_t i = 0; i < total; ++i) key.push_back((char)('a' + (expect[i] ? 1 : 0) + (dst[i].ptr ? 2 : 0)));
+  key.push_back('|');
+  for (size_t col = 0, q = 0; col < c.k; ++col) {
+    const bool on = q < cols.size() && cols[q] == (int)col;
+    q += on ? 1 : 0;
+    key.push_back(on ? '1' : '0');
+  }
+  std::shared_ptr<DecodeEntry> e;
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    ++c.requests;
+    auto it = c.index.find(key);
+    if (it != c.index.end()) {
+      ++c.hits;
+      c.lru.splice(c.lru.begin(), c.lru, it->second);
+      e = it->second->second;
+      if ((rc = wait_ready(*e, s))) return rc;
+    } else {
+      ++c.misses;
+      Plan plan;
+      plan.valid = valid;
+      plan.targets = outputs;
+      if ((rc = new_decode_entry(c, plan, key, s, e, outputs.size() * cols.size()))) return rc;
+      int32_t* st = thread_flag(c.device);
+      if (!st) return RSGPU_ERR_OUT_OF_MEMORY;
+      DecodeJob job{};
+      fill_decode_job(c, plan, job);
+      job.n_cols = (int32_t)cols.size();
+      for (size_t q = 0; q < cols.size(); ++q) job.cols[q] = (uint16_t)cols[q];
+      hipError_t he = launch_decode_rows((const uint16_t*)c.d_matrix.p, job, e->rows, e->tables, st + 1, s);
+      if (he != hipSuccess) {
+        set_hip_error(he, "decode_idx rows");
+        return he == hipErrorInvalidValue ? RSGPU_ERR_UNSUPPORTED : RSGPU_ERR_HIP;
+      }
+      HIP_TRY(hipEventRecord(e->slot.ready, s));
+      cache_insert(c, key, e);
+    }
+  }
+  std::vector<const uint8_t*> in(cols.size());
+  std::vector<uint8_t*> out(outputs.size());
+  for (size_t q = 0; q < cols.size(); ++q) in[q] = input[valid[cols[q]]].ptr;
+  for (size_t t = 0; t < outputs.size(); ++t) out[t] = dst[outputs[t]].ptr;
+  return run_apply(e->tables, (int)cols.size(), (int)outputs.size(), c.field == RSGPU_FIELD_GF16, in, out, len,
+                   kAccumulate, nullptr, s, pick_variant(c, kOpReconstruct, c.m, len));
 }
 
 int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
@@ -1771,6 +2017,53 @@ int rsgpu_reconstruct_some(const rsgpu_codec* h, rsgpu_slot* shards, size_t n,
                            const uint8_t* required, size_t n_required, void* stream) {
   if (!h || !shards || (!required && n_required)) return RSGPU_ERR_INVALID_ARGUMENT;
   return reconstruct_impl(h->c, shards, n, 2, required, n_required, (hipStream_t)stream);
+}
+
+int rsgpu_prepare_reconstruct_workspace(const rsgpu_codec* h, const rsgpu_slot* shards, size_t n, void* stream,
+                                        rsgpu_reconstruct_workspace** out) {
+  if (!h || !shards || !out) return RSGPU_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  return prepare_workspace(h->c, shards, n, 0, nullptr, 0, (hipStream_t)stream, out);
+}
+
+int rsgpu_prepare_reconstruct_data_workspace(const rsgpu_codec* h, const rsgpu_slot* shards, size_t n,
+                                             void* stream, rsgpu_reconstruct_workspace** out) {
+  if (!h || !shards || !out) return RSGPU_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  return prepare_workspace(h->c, shards, n, 1, nullptr, 0, (hipStream_t)stream, out);
+}
+
+int rsgpu_prepare_reconstruct_some_workspace(const rsgpu_codec* h, const rsgpu_slot* shards, size_t n,
+                                             const uint8_t* required, size_t n_required, void* stream,
+                                             rsgpu_reconstruct_workspace** out) {
+  if (!h || !shards || !out || (!required && n_required)) return RSGPU_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  return prepare_workspace(h->c, shards, n, 2, required, n_required, (hipStream_t)stream, out);
+}
+
+int rsgpu_reconstruct_with_workspace(const rsgpu_codec* h, rsgpu_slot* shards, size_t n,
+                                     const rsgpu_reconstruct_workspace* w, void* stream) {
+  if (!h || !shards || !w) return RSGPU_ERR_INVALID_ARGUMENT;
+  return reconstruct_with_workspace_impl(h->c, shards, n, *w, (hipStream_t)stream);
+}
+
+int rsgpu_reconstruct_workspace_targets(const rsgpu_reconstruct_workspace* w, size_t* targets, size_t cap,
+                                        size_t* n_data, size_t* n_parity) {
+  if (!w || (!targets && cap) || !n_data || !n_parity) return RSGPU_ERR_INVALID_ARGUMENT;
+  size_t nd = 0;
+  for (int t : w->targets) nd += (size_t)t < w->k ? 1 : 0;
+  *n_data = nd;
+  *n_parity = w->targets.size() - nd;
+  for (size_t i = 0; i < w->targets.size() && i < cap; ++i) targets[i] = (size_t)w->targets[i];
+  return RSGPU_OK;
+}
+
+void rsgpu_reconstruct_workspace_free(rsgpu_reconstruct_workspace* w) { delete w; }
+
+int rsgpu_decode_idx(const rsgpu_codec* h, const rsgpu_slice* dst, size_t n_dst, const uint8_t* expect_input,
+                     size_t n_expect, const rsgpu_cslice* input, size_t n_input, void* stream) {
+  if (!h || (!dst && n_dst) || (!input && n_input)) return RSGPU_ERR_INVALID_ARGUMENT;
+  return decode_idx_impl(h->c, dst, n_dst, expect_input, n_expect, input, n_input, (hipStream_t)stream);
 }
 
 int rsgpu_reconstruct_noalloc(const rsgpu_codec* h, rsgpu_slot* shards, size_t n, int32_t data_only,

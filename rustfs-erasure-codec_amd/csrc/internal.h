@@ -89,10 +89,13 @@ hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint64_t n, hipStream_t s);
 // rows, then writes their byte-permute tables.
 struct DecodeJob {
   uint16_t valid[256];    // survivor shard indices (first k present), k entries
-  uint16_t targets[256];  // requested outputs: missing data first, then parity
+  uint16_t targets[256];  // requested outputs: missing data first, then parity; a target
+                          // that is itself a survivor gets its unit row
+  uint16_t cols[256];     // survivor columns kept in the rows (n_cols of them, ascending)
   int32_t k, m, n_targets, tower;
   const uint8_t* field;   // device FieldTables log[256] + exp[512] (nullptr: 0x11d field)
   int32_t one;            // the field's multiplicative identity
+  int32_t n_cols;         // 0: every column 0..k-1
 };
 // Sets *flag when row r of a differs from row r of b anywhere in [0, len).
 struct CompareArgs {
@@ -103,6 +106,16 @@ struct CompareArgs {
   int32_t rows;
 };
 hipError_t launch_compare_rows(const CompareArgs& c, hipStream_t s);
+
+// dst[i] ^= src[i] over len[i] bytes for up to kXorPairs pairs per launch.
+constexpr int kXorPairs = 16;
+struct XorPairs {
+  const uint8_t* src[kXorPairs];
+  uint8_t* dst[kXorPairs];
+  uint64_t len[kXorPairs];
+  int32_t n;
+};
+hipError_t launch_xor_pairs(const XorPairs& p, hipStream_t s);
 
 hipError_t launch_decode_rows(const uint16_t* d_matrix, const DecodeJob& job, uint8_t* d_rows,
                               uint32_t* d_tables, int32_t* d_status, hipStream_t s);
@@ -178,9 +191,12 @@ struct DecodePool {
 // `slot` holds the tables [n_targets][k][words] followed by the rows
 // [n_targets][k]; `slot.ready` is recorded on `solve_stream` behind the solve,
 // and a call on another stream waits for it before reading the entry.
+// An entry wider than the pool's slots (decode_idx rows for more targets than
+// the codec has parity shards) owns its buffer and event instead.
 struct DecodeEntry {
   std::shared_ptr<DecodePool> pool;
   DecodePool::Slot slot{nullptr, nullptr};
+  std::unique_ptr<DeviceBuffer> own;  // set when the entry does not use a pool slot
   uint32_t* tables = nullptr;
   uint8_t* rows = nullptr;
   hipStream_t solve_stream = nullptr;

@@ -109,6 +109,14 @@ _lib.rsgpu_verify_with_buffer.argtypes = [_vp, C.POINTER(CSlice), _sz, C.POINTER
 _lib.rsgpu_reconstruct.argtypes = [_vp, C.POINTER(Slot), _sz, _vp]
 _lib.rsgpu_reconstruct_data.argtypes = [_vp, C.POINTER(Slot), _sz, _vp]
 _lib.rsgpu_reconstruct_some.argtypes = [_vp, C.POINTER(Slot), _sz, _u8p, _sz, _vp]
+_lib.rsgpu_prepare_reconstruct_workspace.argtypes = [_vp, C.POINTER(Slot), _sz, _vp, C.POINTER(_vp)]
+_lib.rsgpu_prepare_reconstruct_data_workspace.argtypes = [_vp, C.POINTER(Slot), _sz, _vp, C.POINTER(_vp)]
+_lib.rsgpu_prepare_reconstruct_some_workspace.argtypes = [_vp, C.POINTER(Slot), _sz, _u8p, _sz, _vp,
+                                                          C.POINTER(_vp)]
+_lib.rsgpu_reconstruct_with_workspace.argtypes = [_vp, C.POINTER(Slot), _sz, _vp, _vp]
+_lib.rsgpu_reconstruct_workspace_targets.argtypes = [_vp, C.POINTER(_sz), _sz, C.POINTER(_sz), C.POINTER(_sz)]
+_lib.rsgpu_reconstruct_workspace_free.argtypes = [_vp]
+_lib.rsgpu_decode_idx.argtypes = [_vp, C.POINTER(Slice), _sz, _u8p, _sz, C.POINTER(CSlice), _sz, _vp]
 _lib.rsgpu_reconstruct_noalloc.argtypes = [_vp, C.POINTER(Slot), _sz, C.c_int32, _vp]
 _lib.rsgpu_encode_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz]
 _lib.rsgpu_verify_host.argtypes = [_vp, C.POINTER(_vp), _sz, _sz, _i32p]
@@ -189,6 +197,31 @@ class ShardSlot:
 
     def mark_present(self):
         self.present = True
+
+
+class ReconstructWorkspace:
+    """ReconstructWorkspace (src/core/workspace.rs:85-140): a prepared Classic plan for
+    one erasure layout, with its device decode rows."""
+
+    def __init__(self, handle):
+        self._h = handle
+        cap = 65536
+        buf = (_sz * cap)()
+        nd, np_ = _sz(), _sz()
+        _check(_lib.rsgpu_reconstruct_workspace_targets(handle, buf, cap, C.byref(nd), C.byref(np_)))
+        self._data = [buf[i] for i in range(nd.value)]
+        self._parity = [buf[nd.value + i] for i in range(np_.value)]
+
+    def data_targets(self):
+        return list(self._data)
+
+    def parity_targets(self):
+        return list(self._parity)
+
+    def __del__(self, _free=_lib.rsgpu_reconstruct_workspace_free):  # bound now: globals go at exit
+        if getattr(self, "_h", None):
+            _free(self._h)
+            self._h = None
 
 
 class ReedSolomon:
@@ -358,6 +391,65 @@ class ReedSolomon:
         _check(_lib.rsgpu_reconstruct_some(self._h, slots, len(shards), req.ctypes.data_as(_u8p),
                                            req.size, _stream(stream)))
         self._finish(shards, slots, alloc)
+
+    # ---- prepared reconstruction (src/core/reconstruct.rs:539-700)
+    @staticmethod
+    def _layout(shards):
+        """Slots describing the erasure layout only (missing entries need no buffer)."""
+        slots = (Slot * len(shards))()
+        for i, s in enumerate(shards):
+            if isinstance(s, ShardSlot):
+                p, n = _ptr_len(s.data)
+                slots[i] = Slot(p, n, int(s.present))
+            elif s is not None:
+                p, n = _ptr_len(s)
+                slots[i] = Slot(p, n, 1)
+            else:
+                slots[i] = Slot(None, 0, 0)
+        return slots
+
+    def _prepare(self, fn, shards, *extra, stream=None):
+        h = _vp()
+        _check(fn(self._h, self._layout(shards), len(shards), *extra, _stream(stream), C.byref(h)))
+        return ReconstructWorkspace(h)
+
+    def prepare_reconstruct_workspace(self, shards, stream=None):
+        return self._prepare(_lib.rsgpu_prepare_reconstruct_workspace, shards, stream=stream)
+
+    def prepare_reconstruct_data_workspace(self, shards, stream=None):
+        return self._prepare(_lib.rsgpu_prepare_reconstruct_data_workspace, shards, stream=stream)
+
+    def prepare_reconstruct_some_workspace(self, shards, required, stream=None):
+        req = np.ascontiguousarray([1 if r else 0 for r in required], np.uint8)
+        return self._prepare(_lib.rsgpu_prepare_reconstruct_some_workspace, shards,
+                             req.ctypes.data_as(_u8p), req.size, stream=stream)
+
+    def reconstruct_with_workspace(self, shards, workspace, stream=None):
+        slots, alloc = self._slots(shards)
+        _check(_lib.rsgpu_reconstruct_with_workspace(self._h, slots, len(shards), workspace._h,
+                                                     _stream(stream)))
+        self._finish(shards, slots, alloc)
+
+    reconstruct_with_workspace_par = reconstruct_with_workspace
+
+    def decode_idx(self, dst, expect_input, input, stream=None):
+        """Progressive recovery (src/galois_8/policy.rs:543-704).  dst / input: lists of
+        k+m Optional device tensors; expect_input: list of k+m bools, or None to merge
+        partial results by XOR.  Destinations accumulate in place."""
+        d = (Slice * max(1, len(dst)))(*[Slice(*_ptr_len(t)) if t is not None else Slice(None, 0)
+                                         for t in dst])
+        i = (CSlice * max(1, len(input)))(*[CSlice(*_ptr_len(t)) if t is not None else CSlice(None, 0)
+                                            for t in input])
+        for arr, lst in ((d, dst), (i, input)):  # a Some of length 0 still needs a non-NULL ptr
+            for j, t in enumerate(lst):
+                if t is not None and t.numel() == 0:
+                    arr[j].ptr = 1
+        if expect_input is None:
+            ep, ne = None, 0
+        else:
+            ex = np.ascontiguousarray([1 if x else 0 for x in expect_input], np.uint8)
+            ep, ne = ex.ctypes.data_as(_u8p), ex.size
+        _check(_lib.rsgpu_decode_idx(self._h, d, len(dst), ep, ne, i, len(input), _stream(stream)))
 
     def reconstruct_noalloc(self, shards, data_only=False, stream=None):
         """reconstruct / reconstruct_data through the entry point that allocates
