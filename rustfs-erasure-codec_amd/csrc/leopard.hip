@@ -150,50 +150,51 @@ static void leo_table(const LeoField& F, uint32_t log_m, bool twiddle, uint32_t*
 }
 
 // ------------------------------------------------------------ transform plans
+// A butterfly block: lanes r .. r + radix * cnt, with the twiddle logs of its
+// three multiplies (radix 2: one twiddle, stored three times).  Host only: the
+// device reads the tables of each block (BlockTables) and computes r.
 struct LeoBlock {
-  uint32_t r, cnt, t01, t23, t02, pad0, pad1, pad2;
+  uint32_t r, cnt, l01, l23, l02;
 };
+// One layer.  Its blocks are block0 .. block0 + n_blocks of the plan, block b
+// at lane b << span_shift; cnt = 1 << lg_cnt butterflies per block.
 struct LeoLayer {
-  uint32_t dist, n_blocks, block0, cnt, radix, inverse, pad0, pad1;
+  uint32_t dist, n_blocks, block0, cnt, radix, inverse, lg_cnt, span_shift;
 };
+// Tables of one block, in the order the butterflies use them (t01, t23, t02),
+// each the five dwords q = 0..4 of the byte maps: GF(2^16) as five 16-byte
+// vectors (map0[q], map1[q], map2[q], map3[q]) -- maps low->low, low->high,
+// high->low, high->high -- and GF(2^8) as the five dwords of its one map.
+constexpr uint32_t kBlockTabWords16 = 60;
+constexpr uint32_t kBlockTabWords8 = 16;  // 15 used
 struct LeoGroup {
   uint32_t first, mtrunc, layer0, n_layers;
 };
+
+static uint32_t log2_exact(uint32_t v) {
+  uint32_t l = 0;
+  while ((1u << l) < v) ++l;
+  return l;
+}
 
 struct PlanBuilder {
   const LeoField& F;
   std::vector<LeoLayer> layers;
   std::vector<LeoBlock> blocks;
-  std::vector<uint32_t> tables;  // [n][words]
-  std::map<uint32_t, uint32_t> tab_index;
   explicit PlanBuilder(const LeoField& f) : F(f) {}
-  uint32_t tab(uint32_t log_m) {
-    auto it = tab_index.find(log_m);
-    if (it != tab_index.end()) return it->second;
-    const int W = F.gf16 ? kLeoTabWords16 : kLeoTabWords8;
-    uint32_t idx = (uint32_t)(tables.size() / W);
-    tables.resize(tables.size() + W);
-    leo_table(F, log_m, true, tables.data() + (size_t)idx * W);
-    tab_index[log_m] = idx;
-    return idx;
-  }
   void begin_layer(uint32_t dist, uint32_t cnt, uint32_t radix, bool inverse) {
     LeoLayer L{};
     L.dist = dist;
     L.cnt = cnt;
     L.radix = radix;
     L.inverse = inverse ? 1 : 0;
+    L.lg_cnt = log2_exact(cnt);
+    L.span_shift = log2_exact(radix * dist);
     L.block0 = (uint32_t)blocks.size();
     layers.push_back(L);
   }
   void block(uint32_t r, uint32_t cnt, uint32_t l01, uint32_t l23, uint32_t l02) {
-    LeoBlock b{};
-    b.r = r;
-    b.cnt = cnt;
-    b.t01 = tab(l01);
-    b.t23 = tab(l23);
-    b.t02 = tab(l02);
-    blocks.push_back(b);
+    blocks.push_back(LeoBlock{r, cnt, l01, l23, l02});
     layers.back().n_blocks++;
   }
   // IFFT, encoder form (skew offset base; build_ifft_dit8_plan mod.rs:376-447) or
@@ -246,27 +247,53 @@ static uint32_t ceil_pow2(uint32_t v) {
 }
 
 struct LeoDevPlan {
-  DeviceBuffer layers, blocks, tables;
-  uint32_t n_layers = 0, n_blocks = 0, n_table_words = 0;
+  DeviceBuffer layers, btab;  // LeoLayer[n_layers], block tables [n_blocks][kBlockTabWords*]
+  uint32_t n_layers = 0, n_btab_words = 0;
 };
 
+// Uploads the layers and, per block, its three twiddle tables in the layout of
+// BlockTables.  Every block must sit at lane b << span_shift of its layer, which
+// is how the schedules lay them out (r steps by radix * dist from 0).
 static int upload_plan(const PlanBuilder& b, LeoDevPlan& p) {
+  const LeoField& F = b.F;
+  const uint32_t BW = F.gf16 ? kBlockTabWords16 : kBlockTabWords8;
+  for (const LeoLayer& L : b.layers)
+    for (uint32_t j = 0; j < L.n_blocks; ++j)
+      if (b.blocks[L.block0 + j].r != (j << L.span_shift) || (1u << L.lg_cnt) != L.cnt)
+        return RSGPU_ERR_UNSUPPORTED;
+  std::vector<uint32_t> btab((size_t)b.blocks.size() * BW, 0);
+  std::map<uint32_t, std::vector<uint32_t>> made;  // log -> leo_table words
+  auto table = [&](uint32_t log_m) -> const std::vector<uint32_t>& {
+    auto it = made.find(log_m);
+    if (it != made.end()) return it->second;
+    std::vector<uint32_t> w(F.gf16 ? kLeoTabWords16 : kLeoTabWords8);
+    leo_table(F, log_m, true, w.data());
+    return made.emplace(log_m, std::move(w)).first->second;
+  };
+  for (size_t j = 0; j < b.blocks.size(); ++j) {
+    const LeoBlock& B = b.blocks[j];
+    const uint32_t logs[3] = {B.l01, B.l23, B.l02};
+    uint32_t* dst = btab.data() + j * BW;
+    for (int t = 0; t < 3; ++t) {
+      const std::vector<uint32_t>& w = table(logs[t]);
+      if (F.gf16) {
+        for (int q = 0; q < 5; ++q)
+          for (int m = 0; m < 4; ++m) dst[t * 20 + q * 4 + m] = w[m * kTabWords + q];
+      } else {
+        for (int q = 0; q < 5; ++q) dst[t * 5 + q] = w[q];
+      }
+    }
+  }
   if (p.layers.alloc(std::max<size_t>(1, b.layers.size()) * sizeof(LeoLayer)) != hipSuccess ||
-      p.blocks.alloc(std::max<size_t>(1, b.blocks.size()) * sizeof(LeoBlock)) != hipSuccess ||
-      p.tables.alloc(std::max<size_t>(1, b.tables.size()) * 4) != hipSuccess)
+      p.btab.alloc(std::max<size_t>(1, btab.size()) * 4) != hipSuccess)
     return RSGPU_ERR_OUT_OF_MEMORY;
   if (!b.layers.empty() && hipMemcpy(p.layers.p, b.layers.data(), b.layers.size() * sizeof(LeoLayer),
                                      hipMemcpyHostToDevice) != hipSuccess)
     return RSGPU_ERR_HIP;
-  if (!b.blocks.empty() && hipMemcpy(p.blocks.p, b.blocks.data(), b.blocks.size() * sizeof(LeoBlock),
-                                     hipMemcpyHostToDevice) != hipSuccess)
-    return RSGPU_ERR_HIP;
-  if (!b.tables.empty() &&
-      hipMemcpy(p.tables.p, b.tables.data(), b.tables.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+  if (!btab.empty() && hipMemcpy(p.btab.p, btab.data(), btab.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return RSGPU_ERR_HIP;
   p.n_layers = (uint32_t)b.layers.size();
-  p.n_blocks = (uint32_t)b.blocks.size();
-  p.n_table_words = (uint32_t)b.tables.size();
+  p.n_btab_words = (uint32_t)btab.size();
   return RSGPU_OK;
 }
 
@@ -301,8 +328,7 @@ constexpr uint32_t kLeoInline = 256;
 
 struct LeoArgs {
   const LeoLayer* layers;
-  const LeoBlock* blocks;
-  const uint32_t* tables;
+  const uint32_t* btab;        // block tables (BlockTables layout)
   const uint8_t* const* ptrs;  // k data then parity shard pointers
   const LeoGroup* groups;
   const uint32_t* err_tabs;
@@ -318,7 +344,7 @@ struct LeoArgs {
   uint32_t vec;                // every shard pointer 16-byte aligned
   // A small plan is copied into LDS at dword plan_off when the kernel starts
   // (plan_lds = 1): its layers then read blocks and tables from LDS.
-  uint32_t plan_lds, plan_off, plan_layers, plan_blocks, plan_tab_words;
+  uint32_t plan_lds, plan_off, plan_layers, plan_btab_words;
   // Encode with every data group resident in LDS (group g on lanes g*m ..):
   // one load phase and the fused IFFT layers instead of one pass per group.
   uint32_t resident, fused_layer0, fused_layers;
@@ -509,100 +535,131 @@ typedef const __attribute__((address_space(4))) uint32_t* cword_t;
 // their copy in LDS.
 struct PlanPtrs {
   const LeoLayer* layers;
-  const LeoBlock* blocks;
-  const uint32_t* tables;
+  const uint32_t* btab;
 };
 
-// Copies the plan (layers, blocks, tables: whole 16-byte vectors each) into LDS
-// at dst.  The first barrier of the kernel body orders the copy before any use.
+// Copies the plan (layers, block tables: whole 16-byte vectors each) into LDS at
+// dst.  The first barrier of the kernel body orders the copy before any use.
 __device__ __forceinline__ PlanPtrs stage_plan(const LeoArgs& a, uint32_t* dst) {
-  const uint32_t nl = a.plan_layers * 8, nb = a.plan_blocks * 8, nt = a.plan_tab_words;
+  const uint32_t nl = a.plan_layers * 8, nt = a.plan_btab_words;
   auto copy = [&](uint32_t* to, const uint32_t* from, uint32_t dwords) {
     for (uint32_t i = threadIdx.x; i < dwords / 4; i += blockDim.x)
       reinterpret_cast<uint4*>(to)[i] = reinterpret_cast<const uint4*>(from)[i];
   };
   copy(dst, reinterpret_cast<const uint32_t*>(a.layers), nl);
-  copy(dst + nl, reinterpret_cast<const uint32_t*>(a.blocks), nb);
-  copy(dst + nl + nb, a.tables, nt);
-  return PlanPtrs{reinterpret_cast<const LeoLayer*>(dst), reinterpret_cast<const LeoBlock*>(dst + nl),
-                  dst + nl + nb};
+  copy(dst + nl, a.btab, nt);
+  return PlanPtrs{reinterpret_cast<const LeoLayer*>(dst), dst + nl};
 }
 
-template <bool G16>
-__device__ __forceinline__ Tab<G16> load_tab_c(cword_t p) {
+// One twiddle's tables from a block's tables (BlockTables layout): scalar loads
+// when the whole wave shares the block (S), else vector loads (GF(2^16): five
+// 16-byte loads).
+template <bool G16, bool S>
+__device__ __forceinline__ Tab<G16> block_tab(const uint32_t* p) {
   Tab<G16> t;
-  if constexpr (G16) {
+  if constexpr (S) {
+    const cword_t c = (cword_t)p;
+    if constexpr (G16) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+      for (int q = 0; q < 5; ++q)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) t.a[j][q] = p[j * kTabWords + q];
+        for (int j = 0; j < 4; ++j) t.a[j][q] = c[q * 4 + j];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) t.a[q] = c[q];
+    }
   } else {
+    if constexpr (G16) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) t.a[q] = p[q];
+      for (int q = 0; q < 5; ++q) {
+        const uint4 v = reinterpret_cast<const uint4*>(p)[q];
+        t.a[0][q] = v.x, t.a[1][q] = v.y, t.a[2][q] = v.z, t.a[3][q] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) t.a[q] = p[q];
+    }
   }
   return t;
 }
 
-// The items of one layer.  UNI: every wave's 64 items lie in one block (the
-// layer's cnt * ng is a multiple of 64 and waves start at multiples of 64), so
-// the block and its twiddle tables are read once per wave -- with scalar loads
-// from global memory, or as broadcast reads when the plan is in LDS (PL).
+// The items of one layer: item = (butterfly, group of G words), the word group
+// fastest, so consecutive lanes share a butterfly.  Counts are powers of two
+// (cnt = 1 << lg_cnt butterflies per block, ng = 1 << lg_ng word groups), so the
+// item decomposes with shifts, and block b sits at lane b << span_shift.  UNI:
+// every wave's 64 items lie in one block (cnt * ng a multiple of 64), so its
+// tables are read once per wave -- scalar loads from global memory, or
+// broadcast LDS reads when the plan is in LDS (PL).  Each stage of the
+// butterfly loads its one table and applies it to all G words, so only one
+// table is live at a time.
 template <bool G16, int G, bool UNI, bool PL>
-__device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, const LeoArgs& a,
-                                            const PlanPtrs& P) {
-  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
-  const uint32_t C = a.C, ng = a.C / G;
-  const uint32_t items = L.n_blocks * L.cnt * ng;
+__device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, uint32_t lg_ng,
+                                            const LeoArgs& a, const PlanPtrs& P) {
+  constexpr uint32_t BW = G16 ? kBlockTabWords16 : kBlockTabWords8;
+  constexpr uint32_t TS = G16 ? 20 : 5;  // dwords per table
+  constexpr bool S = UNI && !PL;
+  const uint32_t C = a.C;
+  const uint32_t items = L.n_blocks << (L.lg_cnt + lg_ng);
+  const uint32_t ng_mask = (1u << lg_ng) - 1, cnt_mask = L.cnt - 1;
   for (uint32_t it = threadIdx.x; it < items; it += blockDim.x) {
-    const uint32_t bf = it / ng, wq = (it - bf * ng) * G;
-    uint32_t blk = bf / L.cnt;
+    const uint32_t bf = it >> lg_ng, wq = (it & ng_mask) * G;
+    uint32_t blk = bf >> L.lg_cnt;
     if constexpr (UNI) blk = __builtin_amdgcn_readfirstlane(blk);
-    LeoBlock B;
-    if constexpr (UNI && !PL) {
-      const cword_t q = (cword_t)(P.blocks + L.block0 + blk);
-      B.r = q[0], B.cnt = q[1], B.t01 = q[2], B.t23 = q[3], B.t02 = q[4];
-    } else {
-      B = P.blocks[L.block0 + blk];
-    }
-    const uint32_t i = B.r + (bf - blk * L.cnt);
+    const uint32_t i = (blk << L.span_shift) + (bf & cnt_mask);
     const uint32_t d = L.dist;
-    auto tab = [&](uint32_t t) {
-      if constexpr (UNI && !PL) return load_tab_c<G16>((cword_t)P.tables + (size_t)t * TW);
-      else return load_tab<G16>(P.tables + (size_t)t * TW);
-    };
-    const Tab<G16> t01 = tab(B.t01);
+    const uint32_t* tb = P.btab + (size_t)(L.block0 + blk) * BW;
     if (L.radix == 2) {
       Word<G16> x[G], y[G];
       lds_getv<G16, G>(lds, i * C + wq, x);
       lds_getv<G16, G>(lds, (i + d) * C + wq, y);
+      const Tab<G16> t = block_tab<G16, S>(tb);
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        if (L.inverse) inv2<G16>(x[g], y[g], t01);
-        else fwd2<G16>(x[g], y[g], t01);
+        if (L.inverse) inv2<G16>(x[g], y[g], t);
+        else fwd2<G16>(x[g], y[g], t);
       }
       lds_putv<G16, G>(lds, i * C + wq, x);
       lds_putv<G16, G>(lds, (i + d) * C + wq, y);
     } else {
-      const Tab<G16> t23 = tab(B.t23);
-      const Tab<G16> t02 = tab(B.t02);
       Word<G16> x0[G], x1[G], x2[G], x3[G];
       lds_getv<G16, G>(lds, i * C + wq, x0);
       lds_getv<G16, G>(lds, (i + d) * C + wq, x1);
       lds_getv<G16, G>(lds, (i + 2 * d) * C + wq, x2);
       lds_getv<G16, G>(lds, (i + 3 * d) * C + wq, x3);
+      if (L.inverse) {  // ifft_dit4 (ops.rs:556-586)
+        {
+          const Tab<G16> t01 = block_tab<G16, S>(tb);
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        if (L.inverse) {  // ifft_dit4 (ops.rs:556-586)
-          inv2<G16>(x0[g], x1[g], t01);
-          inv2<G16>(x2[g], x3[g], t23);
+          for (int g = 0; g < G; ++g) inv2<G16>(x0[g], x1[g], t01);
+        }
+        {
+          const Tab<G16> t23 = block_tab<G16, S>(tb + TS);
+#pragma unroll
+          for (int g = 0; g < G; ++g) inv2<G16>(x2[g], x3[g], t23);
+        }
+        const Tab<G16> t02 = block_tab<G16, S>(tb + 2 * TS);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
           inv2<G16>(x1[g], x3[g], t02);
           inv2<G16>(x0[g], x2[g], t02);
-        } else {  // fft_dit4 (ops.rs:516-548)
-          fwd2<G16>(x0[g], x2[g], t02);
-          fwd2<G16>(x1[g], x3[g], t02);
-          fwd2<G16>(x0[g], x1[g], t01);
-          fwd2<G16>(x2[g], x3[g], t23);
         }
+      } else {  // fft_dit4 (ops.rs:516-548)
+        {
+          const Tab<G16> t02 = block_tab<G16, S>(tb + 2 * TS);
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            fwd2<G16>(x0[g], x2[g], t02);
+            fwd2<G16>(x1[g], x3[g], t02);
+          }
+        }
+        {
+          const Tab<G16> t01 = block_tab<G16, S>(tb);
+#pragma unroll
+          for (int g = 0; g < G; ++g) fwd2<G16>(x0[g], x1[g], t01);
+        }
+        const Tab<G16> t23 = block_tab<G16, S>(tb + TS);
+#pragma unroll
+        for (int g = 0; g < G; ++g) fwd2<G16>(x2[g], x3[g], t23);
       }
       lds_putv<G16, G>(lds, i * C + wq, x0);
       lds_putv<G16, G>(lds, (i + d) * C + wq, x1);
@@ -612,16 +669,22 @@ __device__ __forceinline__ void layer_items(uint32_t* lds, const LeoLayer& L, co
   }
 }
 
+// log2 of the word groups per lane row: C and G are powers of two.
+template <int G>
+__device__ __forceinline__ uint32_t lg_groups(const LeoArgs& a) {
+  return (uint32_t)__builtin_ctz(a.C) - (G == 4 ? 2u : G == 2 ? 1u : 0u);
+}
+
 // Runs layers [l0, l0 + nl) over the LDS lanes.  An item is one butterfly on G
 // consecutive words, so its twiddle tables are loaded once for G words.
 template <bool G16, int G, bool PL>
 __device__ __forceinline__ void run_layers(uint32_t* lds, uint32_t l0, uint32_t nl, const LeoArgs& a,
                                            const PlanPtrs& P) {
-  const uint32_t ng = a.C / G;
+  const uint32_t lg_ng = lg_groups<G>(a);
   for (uint32_t l = l0; l < l0 + nl; ++l) {
     const LeoLayer L = P.layers[l];
-    if ((L.cnt * ng) % 64 == 0) layer_items<G16, G, true, PL>(lds, L, a, P);
-    else layer_items<G16, G, false, PL>(lds, L, a, P);
+    if (L.lg_cnt + lg_ng >= 6) layer_items<G16, G, true, PL>(lds, L, lg_ng, a, P);
+    else layer_items<G16, G, false, PL>(lds, L, lg_ng, a, P);
     __syncthreads();
   }
 }
@@ -714,12 +777,12 @@ __device__ __forceinline__ bool wequal(Word<G16> a, Word<G16> b) {
 // first m lanes, which are then FFT'd; lanes 0..parity are the parity.
 template <bool G16, int G, bool PL>
 __device__ __forceinline__ void leo_encode_resident(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
-  const uint32_t C = a.C, m = a.m, ng = C / G;
+  const uint32_t C = a.C, m = a.m, ng = C / G, lg_ng = lg_groups<G>(a);
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   // group g holds data shards g*m .. g*m + min(m, k - g*m) - 1; the rest are zero
   const uint32_t lanes = a.n_groups * m;
   for (uint32_t it = threadIdx.x; it < lanes * ng; it += blockDim.x) {
-    const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+    const uint32_t lane = it >> lg_ng, wq = (it & (ng - 1)) * G;
     Word<G16> v[G];
     if (lane < a.k) {
       gloadv<G16, G>(shard_ptr(a, lane), w0 + wq, a.words, a.vec != 0, v);
@@ -746,7 +809,7 @@ __device__ __forceinline__ void leo_encode_resident(const LeoArgs& a, uint32_t* 
 
 template <bool G16, int G, bool PL>
 __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
-  const uint32_t C = a.C, m = a.m, ng = C / G;
+  const uint32_t C = a.C, m = a.m, ng = C / G, lg_ng = lg_groups<G>(a);
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   uint32_t* A = lds;
   uint32_t* Bv = lds + (G16 && G == 4 ? gf16_lds_dwords(m * C) : m * C * (G16 ? 2u : 1u));
@@ -755,7 +818,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds,
     const LeoGroup Gp = a.groups[g];
     uint32_t* dst = g == 0 ? A : Bv;
     for (uint32_t it = threadIdx.x; it < m * ng; it += blockDim.x) {
-      const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+      const uint32_t lane = it >> lg_ng, wq = (it & (ng - 1)) * G;
       Word<G16> v[G];
       if (lane < Gp.mtrunc) {
         gloadv<G16, G>(shard_ptr(a, Gp.first + lane), w0 + wq, a.words, a.vec != 0, v);
@@ -782,7 +845,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds,
   run_layers<G16, G, PL>(A, a.fft_layer0, a.fft_layers, a, P);
   bool mismatch = false;
   for (uint32_t it = threadIdx.x; it < a.parity * ng; it += blockDim.x) {
-    const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+    const uint32_t lane = it >> lg_ng, wq = (it & (ng - 1)) * G;
     if (w0 + wq >= a.words) continue;
     uint8_t* out = const_cast<uint8_t*>(shard_ptr(a, a.k + lane));
     Word<G16> v[G];
@@ -809,7 +872,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT
     else if (a.C % 2 == 0) leo_encode_body<G16, 2, true>(a, lds, P);
     else leo_encode_body<G16, 1, true>(a, lds, P);
   } else {
-    const PlanPtrs P{a.layers, a.blocks, a.tables};
+    const PlanPtrs P{a.layers, a.btab};
     if (a.C % 4 == 0) leo_encode_body<G16, 4, false>(a, lds, P);
     else if (a.C % 2 == 0) leo_encode_body<G16, 2, false>(a, lds, P);
     else leo_encode_body<G16, 1, false>(a, lds, P);
@@ -820,11 +883,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT
 template <bool G16, int G, bool PL>
 __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
   constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
-  const uint32_t C = a.C, n = a.n, m = a.m, ng = C / G;
+  const uint32_t C = a.C, n = a.n, m = a.m, ng = C / G, lg_ng = lg_groups<G>(a);
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
   // lanes: 0..parity parity shards, m..m+k data shards, scaled by the error locator
   for (uint32_t it = threadIdx.x; it < n * ng; it += blockDim.x) {
-    const uint32_t lane = it / ng, wq = (it - lane * ng) * G;
+    const uint32_t lane = it >> lg_ng, wq = (it & (ng - 1)) * G;
     Word<G16> v[G];
     int shard = -1;
     if (lane < a.parity) shard = (int)(a.k + lane);
@@ -851,7 +914,7 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds,
     const bool live = it < n * ng;
     uint32_t j = 0, wq = 0;
     if (live) {
-      j = it / ng, wq = (it - j * ng) * G;
+      j = it >> lg_ng, wq = (it & (ng - 1)) * G;
       lds_getv<G16, G>(lds, j * C + wq, acc);
       for (uint32_t bit = 1; bit < n; bit <<= 1) {
         if ((j & bit) || j + bit >= n) continue;
@@ -867,7 +930,7 @@ __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds,
   }
   run_layers<G16, G, PL>(lds, a.fft_layer0, a.fft_layers, a, P);
   for (uint32_t it = threadIdx.x; it < (a.k + a.parity) * ng; it += blockDim.x) {
-    const uint32_t s = it / ng, wq = (it - s * ng) * G;
+    const uint32_t s = it >> lg_ng, wq = (it & (ng - 1)) * G;
     if ((shard_flags(a, s) & 3) != 2 || w0 + wq >= a.words) continue;  // missing and required
     const uint32_t lane = s >= a.k ? s - a.k : m + s;
     const Tab<G16> ti = load_tab<G16>(a.inv_tabs + (size_t)lane * TW);
@@ -888,7 +951,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT
     else if (a.C % 2 == 0) leo_decode_body<G16, 2, true>(a, lds, P);
     else leo_decode_body<G16, 1, true>(a, lds, P);
   } else {
-    const PlanPtrs P{a.layers, a.blocks, a.tables};
+    const PlanPtrs P{a.layers, a.btab};
     if (a.C % 4 == 0) leo_decode_body<G16, 4, false>(a, lds, P);
     else if (a.C % 2 == 0) leo_decode_body<G16, 2, false>(a, lds, P);
     else leo_decode_body<G16, 1, false>(a, lds, P);
@@ -980,24 +1043,26 @@ static LeoState& state(const Codec& c) { return *static_cast<LeoState*>(c.leopar
 // the threads busy even when the codec has few work lanes -- or the smaller
 // `want` a dispatched variant asks for.
 static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uint32_t want = 0,
-                           size_t reserve = 0, uint32_t cap_rows = 0) {
+                           size_t reserve = 0, uint32_t cap_rows = 0, uint32_t min_half = 8) {
   const uint32_t wb = s.gf16 ? 8 : 4;
   // room for the GF16 layout's block padding and for a plan copied into LDS
   const uint64_t usable = s.lds_limit - 1024 - reserve;
   uint64_t C = usable / ((uint64_t)rows * wb);
   const uint64_t cap = std::max<uint64_t>(64, 4096 / (cap_rows ? cap_rows : rows));
   if (C > cap) C = cap;
-  // Default: two workgroups per CU when that still leaves 8 words a lane -- a
-  // lone workgroup (one wave per SIMD) leaves its table loads' latency exposed.
+  // Default: two workgroups per CU when that still leaves min_half words a lane
+  // -- a lone workgroup leaves its table loads' and barriers' latency exposed.
   const uint64_t half_room = (s.lds_limit - 1024) / 2;
   const uint64_t half = half_room > reserve ? (half_room - reserve) / ((uint64_t)rows * wb) : 0;
-  if (!want && C > half && half >= 8) C = half;
+  if (!want && C > half && half >= min_half) C = half;
   if (want && C > want) C = want;
   if (C < 1) C = 1;
   if (C > words) C = (uint32_t)words;
-  if (C >= 4) C &= ~(uint64_t)3;  // whole 4-word vectors (16-byte LDS / global accesses)
-  else if (C >= 2) C &= ~(uint64_t)1;
-  return (uint32_t)C;
+  // a power of two (items decompose with shifts); from 4 words on, whole 4-word
+  // vectors (16-byte LDS / global accesses)
+  uint64_t p2 = 1;
+  while (p2 * 2 <= C) p2 *= 2;
+  return (uint32_t)p2;
 }
 
 // Threads per workgroup.  The GF8 kernels take 72 VGPRs, so two 512-thread
@@ -1022,7 +1087,7 @@ static int leo_threads(const LeoState& s, size_t lds, bool decode) {
 // reads.
 constexpr size_t kPlanLdsMax = 16 << 10;
 static size_t plan_lds_bytes(const LeoState& s, const LeoDevPlan& p, uint32_t rows) {
-  const size_t b = ((size_t)p.n_layers + p.n_blocks) * 32 + (size_t)p.n_table_words * 4;
+  const size_t b = (size_t)p.n_layers * 32 + (size_t)p.n_btab_words * 4;
   if (b > kPlanLdsMax) return 0;
   // the tile keeps at least 8 words a lane beside it
   const size_t wb = s.gf16 ? 8 : 4;
@@ -1035,8 +1100,7 @@ static size_t place_plan(const LeoDevPlan& p, size_t plan_bytes, size_t data_byt
   a.plan_lds = 1;
   a.plan_off = (uint32_t)((data_bytes + 15) / 16 * 4);
   a.plan_layers = p.n_layers;
-  a.plan_blocks = p.n_blocks;
-  a.plan_tab_words = p.n_table_words;
+  a.plan_btab_words = p.n_btab_words;
   return (size_t)a.plan_off * 4 + plan_bytes;
 }
 
@@ -1243,6 +1307,7 @@ int init_leopard(Codec& c) {
       const LeoLayer L0 = pb.layers[st->groups[0].layer0 + j];
       LeoLayer L{};
       L.dist = L0.dist, L.cnt = L0.cnt, L.radix = L0.radix, L.inverse = L0.inverse;
+      L.lg_cnt = L0.lg_cnt, L.span_shift = L0.span_shift;
       L.block0 = (uint32_t)pb.blocks.size();
       for (size_t g = 0; g < st->groups.size(); ++g) {
         const LeoLayer Lg = pb.layers[st->groups[g].layer0 + j];
@@ -1291,8 +1356,7 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   int rc = pass_args(a, ptrs, nullptr, s, &off, &bytes);
   if (rc) return rc;
   a.layers = (const LeoLayer*)st.enc.layers.p;
-  a.blocks = (const LeoBlock*)st.enc.blocks.p;
-  a.tables = (const uint32_t*)st.enc.tables.p;
+  a.btab = (const uint32_t*)st.enc.btab.p;
   a.groups = (const LeoGroup*)st.d_groups.p;
   a.words = words;
   a.C = C;
@@ -1444,15 +1508,17 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
   if (rc) return rc;
   const uint64_t words = len / (st.gf16 ? 8 : 4);
   const size_t plan_bytes = plan_lds_bytes(st, e->plan, st.n);
-  const uint32_t C = tile_words(st, st.n, words, variant_tile(variant), plan_bytes);
+  // GF(2^16) decode keeps two workgroups per CU down to 4 words a lane: 1024+256
+  // x 256 KiB reconstruct runs 5% faster at 4 words (two 512-thread groups) than
+  // at 8 (one 1024-thread group).
+  const uint32_t C = tile_words(st, st.n, words, variant_tile(variant), plan_bytes, 0, st.gf16 ? 4 : 8);
   std::vector<const uint8_t*> ptrs(total);
   for (size_t i = 0; i < total; ++i) ptrs[i] = shards[i].ptr;
   LeoArgs a{};
   size_t off = 0, bytes = 0;
   if ((rc = pass_args(a, ptrs, &flags, s, &off, &bytes))) return rc;
   a.layers = (const LeoLayer*)e->plan.layers.p;
-  a.blocks = (const LeoBlock*)e->plan.blocks.p;
-  a.tables = (const uint32_t*)e->plan.tables.p;
+  a.btab = (const uint32_t*)e->plan.btab.p;
   a.err_tabs = (const uint32_t*)e->err_tabs.p;
   a.inv_tabs = (const uint32_t*)e->inv_tabs.p;
   a.words = words;
