@@ -172,9 +172,10 @@ int rsgpu_verify_with_buffer(const rsgpu_codec* codec, const rsgpu_cslice* shard
                              int32_t* ok, void* stream);
 
 /* ---- reconstruct: src/core/reconstruct.rs:539-790 ----
- * Classic codecs solve for at most 128 missing data shards per call (the
- * inversion runs in one workgroup); more return RSGPU_ERR_UNSUPPORTED.  Missing
- * parity shards are not limited. */
+ * Classic codecs solve for at most 8192 missing data shards per call (GF(2^16);
+ * the inversion runs in one workgroup, in LDS up to 128 rows and over a device
+ * work matrix beyond); more return RSGPU_ERR_UNSUPPORTED.  Missing parity shards
+ * are not limited. */
 /* ReedSolomon::reconstruct (all missing shards). */
 int rsgpu_reconstruct(const rsgpu_codec* codec, rsgpu_slot* shards, size_t n_shards, void* stream);
 /* ReedSolomon::reconstruct_data (missing data shards only). */
@@ -230,8 +231,9 @@ void rsgpu_reconstruct_workspace_free(rsgpu_reconstruct_workspace* workspace);
  * RSGPU_UNSUPPORTED_CODEC_FAMILY; expected count other than k:
  * RSGPU_TOO_FEW_SHARDS_PRESENT), and every pair is validated before any XOR, so
  * a rejected call changes nothing.  Classic GF(2^16) codecs are accepted too
- * (the reference has the method for galois_8 only); codecs with k > 256 or more
- * than 256 destinations return RSGPU_ERR_UNSUPPORTED.  Asynchronous on `stream`. */
+ * (the reference has the method for galois_8 only); codecs with k > 256, more
+ * than 256 destinations or more than 128 missing data shards among the expected
+ * return RSGPU_ERR_UNSUPPORTED.  Asynchronous on `stream`. */
 int rsgpu_decode_idx(const rsgpu_codec* codec, const rsgpu_slice* dst, size_t n_dst,
                      const uint8_t* expect_input, size_t n_expect, const rsgpu_cslice* input,
                      size_t n_input, void* stream);
@@ -242,7 +244,8 @@ int rsgpu_decode_idx(const rsgpu_codec* codec, const rsgpu_slice* dst, size_t n_
  * decode solve goes into a library-owned device scratch array of 2 MiB per GPU
  * (held until the process ends): 256 B + the coefficient tables (32 B per
  * coefficient, 128 B for GF(2^16)) + the rows, each rounded up to 256 B, for
- * (missing targets) x k coefficients.  Classic codecs of at most 256 shards;
+ * (missing targets) x k coefficients.  Classic codecs of at most 256 shards and
+ * patterns of at most 128 missing data shards;
  * larger shapes return RSGPU_ERR_UNSUPPORTED, Leopard codecs
  * RSGPU_UNSUPPORTED_CODEC_FAMILY.  Every call on one GPU shares that GPU's
  * scratch: calls that overlap on one GPU (other streams or threads, before the

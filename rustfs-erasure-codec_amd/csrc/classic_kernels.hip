@@ -398,6 +398,87 @@ __global__ __launch_bounds__(256) void decode_inv_kernel(const uint16_t* __restr
   if (tid == 0) *status = RSGPU_OK;
 }
 
+// The same elimination with the [C | I] work matrix in device memory, for more
+// missing data rows than LDS holds.  One workgroup; the pivot row is the first
+// nonzero one at or below r (as in the LDS version), found in parallel.
+__global__ __launch_bounds__(1024) void decode_inv_global_kernel(const uint16_t* __restrict__ matrix,
+                                                                 const DecodeBigJob job, uint16_t* wm,
+                                                                 uint16_t* __restrict__ cinv,
+                                                                 int32_t* __restrict__ status) {
+  __shared__ uint8_t lg[256];
+  __shared__ uint8_t ex[512];
+  __shared__ int pivot_row;
+  const int tid = threadIdx.x, e = job.e;
+  const size_t cols = 2 * (size_t)e;
+  for (int i = tid; i < 256; i += blockDim.x) lg[i] = job.field ? job.field[i] : kGf8.log[i];
+  for (int i = tid; i < 512; i += blockDim.x) ex[i] = job.field ? job.field[256 + i] : kGf8.exp[i];
+  for (size_t idx = tid; idx < (size_t)e * cols; idx += blockDim.x) {
+    const size_t r = idx / cols, c = idx % cols;
+    wm[idx] = c < (size_t)e ? matrix[(size_t)job.par_surv[r] * job.k + job.missing[c]]
+                            : (c - e == r ? (uint16_t)job.one : (uint16_t)0);
+  }
+  __syncthreads();
+  DevField F{lg, ex, job.tower != 0};
+  for (int r = 0; r < e; ++r) {
+    if (tid == 0) pivot_row = e;
+    __syncthreads();
+    for (int rb = r + tid; rb < e; rb += blockDim.x)
+      if (wm[(size_t)rb * cols + r] != 0) atomicMin(&pivot_row, rb);
+    __syncthreads();
+    const int pr = pivot_row;
+    if (pr >= e) {  // uniform: every thread leaves
+      if (tid == 0) *status = RSGPU_INVALID_CUSTOM_MATRIX;
+      return;
+    }
+    if (pr != r)
+      for (size_t c = tid; c < cols; c += blockDim.x) {
+        const uint16_t t = wm[(size_t)r * cols + c];
+        wm[(size_t)r * cols + c] = wm[(size_t)pr * cols + c];
+        wm[(size_t)pr * cols + c] = t;
+      }
+    __syncthreads();
+    const uint16_t scale = F.inv(wm[(size_t)r * cols + r]);
+    __syncthreads();
+    for (size_t c = tid; c < cols; c += blockDim.x) wm[(size_t)r * cols + c] = F.mul(scale, wm[(size_t)r * cols + c]);
+    __syncthreads();
+    for (size_t idx = tid; idx < (size_t)e * cols; idx += blockDim.x) {
+      const size_t rb = idx / cols, c = idx % cols;
+      if (rb == (size_t)r || c == (size_t)r) continue;
+      const uint16_t f = wm[rb * cols + r];
+      if (f) wm[idx] ^= F.mul(f, wm[(size_t)r * cols + c]);
+    }
+    __syncthreads();
+    for (int rb = tid; rb < e; rb += blockDim.x)
+      if (rb != r) wm[(size_t)rb * cols + r] = 0;
+    __syncthreads();
+  }
+  for (size_t idx = tid; idx < (size_t)e * e; idx += blockDim.x)
+    cinv[idx] = wm[(idx / e) * cols + e + idx % e];
+  if (tid == 0) *status = RSGPU_OK;
+}
+
+struct U16Chunk {
+  int32_t n;
+  uint16_t v[1024];
+};
+
+__global__ void store_u16_kernel(uint16_t* dst, const U16Chunk c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c.n) dst[i] = c.v[i];
+}
+
+hipError_t launch_store_u16(uint16_t* dst, const uint16_t* vals, size_t n, hipStream_t s) {
+  for (size_t i0 = 0; i0 < n; i0 += 1024) {
+    U16Chunk c{};
+    c.n = (int32_t)std::min<size_t>(1024, n - i0);
+    for (int i = 0; i < c.n; ++i) c.v[i] = vals[i0 + i];
+    hipLaunchKernelGGL(store_u16_kernel, dim3((c.n + 255) / 256), dim3(256), 0, s, dst + i0, c);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 struct DecodeTargets {
   int32_t t0, n;                     // first output row of this launch, rows in it
   uint16_t target[kBigTargetChunk];  // shard index of each row
@@ -416,7 +497,7 @@ __global__ __launch_bounds__(256) void decode_rows_big_kernel(const uint16_t* __
                                                               uint32_t* __restrict__ tables_out) {
   __shared__ uint8_t lg[256];
   __shared__ uint8_t ex[512];
-  __shared__ uint16_t w[kMaxSolve];
+  extern __shared__ uint16_t w[];  // e entries
   const int tid = threadIdx.x, k = job.k, e = job.e, p = k - e;
   for (int i = tid; i < 256; i += blockDim.x) lg[i] = job.field ? job.field[i] : kGf8.log[i];
   for (int i = tid; i < 512; i += blockDim.x) ex[i] = job.field ? job.field[256 + i] : kGf8.exp[i];
@@ -440,9 +521,7 @@ __global__ __launch_bounds__(256) void decode_rows_big_kernel(const uint16_t* __
   if (col >= k) return;
   uint16_t v;
   if (col < p) {
-    int d = col;  // the col-th present data index
-    for (int j = 0; j < e; ++j)
-      if (job.missing[j] <= d) ++d;
+    const int d = job.present[col];  // the col-th present data index
     // a data target's own column: the unit row of a surviving data shard
     v = target < k ? (d == target ? (uint16_t)job.one : (uint16_t)0) : prow[d];
     for (int i = 0; i < e; ++i) v ^= F.mul(w[i], matrix[(size_t)job.par_surv[i] * k + d]);
@@ -453,10 +532,15 @@ __global__ __launch_bounds__(256) void decode_rows_big_kernel(const uint16_t* __
 }
 
 hipError_t launch_decode_big(const uint16_t* d_matrix, const DecodeBigJob& job,
-                             const std::vector<int>& targets, uint16_t* d_cinv, uint8_t* d_rows,
-                             uint32_t* d_tables, int32_t* d_status, hipStream_t s) {
-  if (job.e > kMaxSolve) return hipErrorInvalidValue;
-  if (job.e > 0) {
+                             const std::vector<int>& targets, uint16_t* d_cinv, uint16_t* d_work,
+                             uint8_t* d_rows, uint32_t* d_tables, int32_t* d_status, hipStream_t s) {
+  if (job.e > kMaxBigSolve || (job.e > kMaxSolve && !d_work)) return hipErrorInvalidValue;
+  if (job.e > kMaxSolve) {
+    hipLaunchKernelGGL(decode_inv_global_kernel, dim3(1), dim3(1024), 0, s, d_matrix, job, d_work, d_cinv,
+                       d_status);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  } else if (job.e > 0) {
     const size_t lds = (size_t)job.e * 2 * job.e * sizeof(uint16_t);
     hipLaunchKernelGGL(decode_inv_kernel, dim3(1), dim3(256), lds, s, d_matrix, job, d_cinv, d_status);
     hipError_t err = hipGetLastError();
@@ -468,8 +552,8 @@ hipError_t launch_decode_big(const uint16_t* d_matrix, const DecodeBigJob& job,
     tg.n = (int32_t)std::min<size_t>(kBigTargetChunk, targets.size() - t0);
     for (int i = 0; i < tg.n; ++i) tg.target[i] = (uint16_t)targets[t0 + i];
     const dim3 grid((unsigned)((job.k + 255) / 256), (unsigned)tg.n);
-    hipLaunchKernelGGL(decode_rows_big_kernel, grid, dim3(256), 0, s, d_matrix, job, tg, d_cinv,
-                       d_rows, d_tables);
+    hipLaunchKernelGGL(decode_rows_big_kernel, grid, dim3(256), (size_t)std::max(job.e, 1) * sizeof(uint16_t), s,
+                       d_matrix, job, tg, d_cinv, d_rows, d_tables);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
   }

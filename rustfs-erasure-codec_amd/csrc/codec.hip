@@ -691,17 +691,34 @@ static void fill_decode_job(const Codec& c, const Plan& plan, DecodeJob& job) {
 
 // A decode entry for `plan` on a pool slot (with c.mu held); its solve is
 // enqueued on s by the caller, who records slot.ready behind it.
+// Byte offsets inside a pool slot: tables and rows of m targets, then for the
+// large-codec solve C^-1, the index lists (missing, parity survivors, present
+// data) and, beyond kMaxSolve missing rows, the [C | I] work matrix.
+struct SlotLayout {
+  size_t rows, cinv, lists, work, total;
+  explicit SlotLayout(const Codec& c) {
+    const bool tower = c.field == RSGPU_FIELD_GF16;
+    const size_t max_e = std::min<size_t>(std::min(c.k, c.m), kMaxBigSolve);
+    auto up = [](size_t n) { return (n + 255) / 256 * 256; };
+    rows = up(c.m * c.k * (size_t)words_per_coef(tower) * 4);
+    cinv = rows + up(c.m * c.k * 2);
+    lists = cinv + up(max_e * max_e * 2);
+    work = lists + up((c.k + max_e) * 2);
+    total = work + (max_e > (size_t)kMaxSolve ? up(2 * max_e * max_e * 2) : 0);
+  }
+};
+
 // `n_coefs` (default: targets x k) is the number of row coefficients the solve
 // writes; an entry with more than the pool's m x k gets a buffer of its own.
 static int new_decode_entry(const Codec& c, const Plan& plan, const std::string& key, hipStream_t s,
                             std::shared_ptr<DecodeEntry>& out, size_t n_coefs = 0) {
   const bool tower = c.field == RSGPU_FIELD_GF16;
   const size_t W4 = (size_t)words_per_coef(tower) * 4;
-  const size_t max_tab = c.m * c.k * W4;
-  const size_t max_e = std::min<size_t>(std::min(c.k, c.m), kMaxSolve);
-  if (!c.decode_pool) {  // one slot size per codec: tables and rows of m targets, C^-1
+  const SlotLayout lay(c);
+  const size_t max_tab = lay.rows;
+  if (!c.decode_pool) {  // one slot size per codec
     c.decode_pool = std::make_shared<DecodePool>();
-    c.decode_pool->slot_bytes = (max_tab + c.m * c.k * 2 + max_e * max_e * 2 + 255) / 256 * 256;
+    c.decode_pool->slot_bytes = lay.total;
   }
   if (!n_coefs) n_coefs = plan.targets.size() * c.k;
   auto e = std::make_shared<DecodeEntry>();
@@ -757,24 +774,33 @@ static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan
   int32_t* st = thread_flag(c.device);
   if (!st) return RSGPU_ERR_OUT_OF_MEMORY;
   hipError_t he;
-  if (c.k > 256 || nt > 256) {  // beyond DecodeJob's lists
+  size_t p = 0;  // present data among the survivors (a sorted prefix)
+  while (p < c.k && plan.valid[p] < (int)c.k) ++p;
+  const size_t n_missing = c.k - p;
+  if (c.k > 256 || nt > 256 || n_missing > (size_t)kMaxSolve) {  // beyond DecodeJob's limits
+    if (n_missing > (size_t)kMaxBigSolve || e->own) return RSGPU_ERR_UNSUPPORTED;
+    const SlotLayout lay(c);
+    uint8_t* base = (uint8_t*)e->slot.mem;
     DecodeBigJob bj{};
     bj.k = (int)c.k;
+    bj.e = (int)n_missing;
     bj.tower = tower ? 1 : 0;
     bj.field = c.ftab ? (const uint8_t*)c.d_ftab.p : nullptr;
     bj.one = c.ftab ? c.ftab->one : 1;
-    size_t p = 0;
-    while (p < c.k && plan.valid[p] < (int)c.k) ++p;
-    bj.e = (int)(c.k - p);
-    if (bj.e > kMaxSolve) return RSGPU_ERR_UNSUPPORTED;
+    // lists [missing e][parity survivors e][present data k - e] into the slot
+    std::vector<uint16_t> lists(c.k + n_missing);
     for (size_t d = 0, kn = 0, j = 0; d < c.k; ++d) {
-      if (kn < p && plan.valid[kn] == (int)d) ++kn;
-      else bj.missing[j++] = (uint16_t)d;
+      if (kn < p && plan.valid[kn] == (int)d) lists[2 * n_missing + kn++] = (uint16_t)d;
+      else lists[j++] = (uint16_t)d;
     }
-    for (int i = 0; i < bj.e; ++i) bj.par_surv[i] = (uint16_t)plan.valid[p + i];
-    uint16_t* cinv = (uint16_t*)(e->rows + c.m * c.k * 2);
-    he = launch_decode_big((const uint16_t*)c.d_matrix.p, bj, plan.targets, cinv, e->rows,
-                           e->tables, st + 1, s);
+    for (size_t i = 0; i < n_missing; ++i) lists[n_missing + i] = (uint16_t)plan.valid[p + i];
+    uint16_t* dl = (uint16_t*)(base + lay.lists);
+    HIP_TRY(launch_store_u16(dl, lists.data(), lists.size(), s));
+    bj.missing = dl;
+    bj.par_surv = dl + n_missing;
+    bj.present = dl + 2 * n_missing;
+    he = launch_decode_big((const uint16_t*)c.d_matrix.p, bj, plan.targets, (uint16_t*)(base + lay.cinv),
+                           (uint16_t*)(base + lay.work), e->rows, e->tables, st + 1, s);
   } else {
     DecodeJob job{};
     fill_decode_job(c, plan, job);
@@ -1201,15 +1227,14 @@ int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_
   std::shared_ptr<const DecoderMatrix> dm;
   {
     std::lock_guard<std::mutex> g(c.mu);
-    auto it = c.dec_matrix.find(key);
-    if (it != c.dec_matrix.end()) dm = it->second;
+    dm = c.dec_matrix.get(key);
   }
   if (!dm) {
     int rc = build_decoder_matrix(c, flags, s, dm);
     if (rc) return rc;
+    std::vector<std::shared_ptr<const DecoderMatrix>> evicted;  // freed after the lock
     std::lock_guard<std::mutex> g(c.mu);
-    if (c.dec_matrix.size() >= 256) c.dec_matrix.clear();
-    c.dec_matrix.emplace(key, dm);
+    c.dec_matrix.put(key, dm, evicted);
   }
   std::vector<const uint8_t*> in;
   std::vector<uint8_t*> out;
@@ -2074,9 +2099,10 @@ int rsgpu_reconstruct_noalloc(const rsgpu_codec* h, rsgpu_slot* shards, size_t n
 
 // Coefficient tables of rsgpu_apply_rows, kept on the device by (device, field,
 // shape, row bytes): callers pass the same parity rows call after call, which
-// then only enqueue the kernels (no allocation, upload or synchronisation).  At
-// most 64 row sets are kept; an evicted set is freed by hipFree, which waits for
-// the device, so no launch still reads it.
+// then only enqueue the kernels (no allocation, upload or synchronisation).  The
+// 64 most recently used row sets are kept; an evicted set is freed (after the
+// cache lock is released) by hipFree, which waits for the device, so no launch
+// still reads it.
 static int apply_rows_tables(int32_t field, const uint8_t* rows, size_t r, size_t k,
                              std::shared_ptr<const DeviceBuffer>& out) {
   const bool tower = field == RSGPU_FIELD_GF16;
@@ -2089,14 +2115,10 @@ static int apply_rows_tables(int32_t field, const uint8_t* rows, size_t r, size_
   key.append(reinterpret_cast<const char*>(rows), nbytes);
   // never destroyed: no hipFree may run in static destruction at process exit
   static std::mutex& mu = *new std::mutex;
-  static auto& cache = *new std::unordered_map<std::string, std::shared_ptr<const DeviceBuffer>>;
+  static auto& cache = *new LruMap<const DeviceBuffer>{64};
   {
     std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) {
-      out = it->second;
-      return RSGPU_OK;
-    }
+    if ((out = cache.get(key))) return RSGPU_OK;
   }
   std::vector<uint16_t> rr(r * k);
   for (size_t i = 0; i < r * k; ++i) rr[i] = tower ? (uint16_t)((rows[2 * i] << 8) | rows[2 * i + 1]) : rows[i];
@@ -2105,9 +2127,10 @@ static int apply_rows_tables(int32_t field, const uint8_t* rows, size_t r, size_
   auto dt = std::make_shared<DeviceBuffer>();
   HIP_TRY(dt->alloc(ht.size() * 4));
   HIP_TRY(hipMemcpy(dt->p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice));
+  std::vector<std::shared_ptr<const DeviceBuffer>> evicted;  // freed after the lock
   std::lock_guard<std::mutex> g(mu);
-  if (cache.size() >= 64) cache.clear();
-  out = cache.emplace(key, std::move(dt)).first->second;
+  cache.put(key, dt, evicted);
+  out = dt;
   return RSGPU_OK;
 }
 

@@ -130,26 +130,68 @@ struct DecodeMultiJob {
 hipError_t launch_decode_rows_multi(const uint16_t* d_matrix, const DecodeMultiJob* d_jobs, int n,
                                     size_t lds, int32_t* d_status, hipStream_t s);
 
-// Decode rows of codecs beyond DecodeJob's 256-entry lists (classic GF(2^16) with
-// k > 256 or more than 256 targets): C^-1 of the missing-data system in one
-// workgroup, then every (target, column) coefficient in a grid.  Survivors are
-// the present data (every index below k not in `missing`) followed by
-// `par_surv`, as make_plan's first-k-present rule orders them.
-constexpr int kMaxSolve = 128;        // missing data rows solvable in one workgroup
+// Decode rows of codecs beyond DecodeJob's limits (classic GF(2^16) with k > 256,
+// more than 256 targets, or more than kMaxSolve missing data shards): C^-1 of
+// the missing-data system by one workgroup -- in LDS up to kMaxSolve rows, in a
+// device work matrix beyond -- then every (target, column) coefficient in a
+// grid.  Survivors are the present data (every index below k not in `missing`)
+// followed by `par_surv`, as make_plan's first-k-present rule orders them.
+constexpr int kMaxSolve = 128;        // missing data rows solvable in LDS
+constexpr int kMaxBigSolve = 8192;    // missing data rows of the device-memory solve
 constexpr int kBigTargetChunk = 1024;  // targets per rows launch
 struct DecodeBigJob {
   int32_t k, e, tower, one;
-  const uint8_t* field;          // as in DecodeJob
-  uint16_t missing[kMaxSolve];   // missing data indices, ascending
-  uint16_t par_surv[kMaxSolve];  // parity survivors solving for them, ascending
+  const uint8_t* field;      // as in DecodeJob
+  const uint16_t* missing;   // device: e missing data indices, ascending
+  const uint16_t* par_surv;  // device: e parity survivors solving for them, ascending
+  const uint16_t* present;   // device: the k - e present data indices, ascending
 };
+// d_work: 2 * e * e elements of device scratch, needed when e > kMaxSolve.
 hipError_t launch_decode_big(const uint16_t* d_matrix, const DecodeBigJob& job,
-                             const std::vector<int>& targets, uint16_t* d_cinv, uint8_t* d_rows,
-                             uint32_t* d_tables, int32_t* d_status, hipStream_t s);
+                             const std::vector<int>& targets, uint16_t* d_cinv, uint16_t* d_work,
+                             uint8_t* d_rows, uint32_t* d_tables, int32_t* d_status, hipStream_t s);
+// Writes n 16-bit values from the host to device memory on stream s through
+// kernel arguments (no staging buffer, nothing to keep alive after the call).
+hipError_t launch_store_u16(uint16_t* dst, const uint16_t* vals, size_t n, hipStream_t s);
 
 // Tables for a host-known row set (encode rows, apply_rows primitive).
 void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out,
                        const FieldTables* field = nullptr);
+
+// Least-recently-used map of shared values with an entry capacity.  An insert
+// past the capacity evicts from the least recent end into `evicted`, which the
+// caller drops after releasing its lock: a value's destructor may free device
+// memory, and a hipFree waits for the whole device.
+template <class V>
+struct LruMap {
+  using Item = std::pair<std::string, std::shared_ptr<V>>;
+  size_t cap = 64;
+  std::list<Item> order;  // front: most recent
+  std::unordered_map<std::string, typename std::list<Item>::iterator> index;
+  std::shared_ptr<V> get(const std::string& key) {
+    auto it = index.find(key);
+    if (it == index.end()) return nullptr;
+    order.splice(order.begin(), order, it->second);
+    return it->second->second;
+  }
+  void put(const std::string& key, std::shared_ptr<V> v, std::vector<std::shared_ptr<V>>& evicted) {
+    auto it = index.find(key);
+    if (it != index.end()) {
+      evicted.push_back(std::move(it->second->second));
+      it->second->second = std::move(v);
+      order.splice(order.begin(), order, it->second);
+      return;
+    }
+    while (!order.empty() && index.size() >= cap) {
+      evicted.push_back(std::move(order.back().second));
+      index.erase(order.back().first);
+      order.pop_back();
+    }
+    order.emplace_front(key, std::move(v));
+    index[key] = order.begin();
+  }
+  size_t size() const { return index.size(); }
+};
 
 struct DeviceBuffer {
   void* p = nullptr;
@@ -249,8 +291,9 @@ struct Codec {
   // Leopard GF(2^8): classic codec over the generator matrix [I; G], built on the
   // first call that dispatches to a matrix-apply variant (guarded by mu).
   mutable std::shared_ptr<Codec> leo_matrix;
-  // Leopard GF(2^8): decoder matrices by (present, target) pattern (guarded by mu).
-  mutable std::unordered_map<std::string, std::shared_ptr<const DecoderMatrix>> dec_matrix;
+  // Leopard GF(2^8): decoder matrices by (present, target) pattern, least
+  // recently used evicted past 256 (guarded by mu).
+  mutable LruMap<const DecoderMatrix> dec_matrix{256};
   // Byte field of the matrix when it is not the 0x11d field (the generator-matrix
   // codec of a Leopard GF(2^8) codec), with a device copy of log + exp.
   std::shared_ptr<const FieldTables> ftab;
@@ -261,6 +304,8 @@ struct Codec {
 };
 
 int words_per_coef(int tower);  // kTabWords or 4 * kTabWords
+
+
 
 // Staging for presolve_decode's job arrays, owned by the caller of one stream
 // and reused across calls; `done` is recorded behind the launches that read it.

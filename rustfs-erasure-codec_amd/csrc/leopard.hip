@@ -20,6 +20,7 @@
 // butterfly shortcut for log_m == 2^n - 1 (ops.rs:466-498 GF8, :382-452 GF16)
 // is exactly a multiplication by zero, so such twiddles get an all-zero table.
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
 #include <map>
 
@@ -299,12 +300,33 @@ static int upload_plan(const PlanBuilder& b, LeoDevPlan& p) {
 
 // Per-pattern decode state: transform plan, per-lane multiply tables (error
 // locator and its inverse) built on the device.
+// The locator tables are written by a kernel on `stream`; `ready` is recorded
+// behind it, and a call on another stream waits for it while it is pending.
 struct LeoDecodeEntry {
   LeoDevPlan plan;
   uint32_t n_ifft = 0, n_fft = 0, input_count = 0;
   DeviceBuffer err_tabs;  // [n][words] multiply-by-err_loc tables (lanes < work_size)
   DeviceBuffer inv_tabs;  // [n][words] multiply-by-(modulus - err_loc)
   DeviceBuffer err_locs;  // [work_size] u32, kept for inspection
+  DeviceBuffer pos;       // erased positions, read by the locator kernel
+  hipEvent_t ready = nullptr;
+  hipStream_t stream = nullptr;
+  ~LeoDecodeEntry() {
+    if (ready) {
+      (void)hipEventSynchronize(ready);
+      (void)hipEventDestroy(ready);
+    }
+  }
+};
+
+// A decode plan being built by one thread; others asking for the same pattern
+// wait for it instead of building it again.
+struct LeoPending {
+  std::mutex m;
+  std::condition_variable cv;
+  bool done = false;
+  int rc = RSGPU_OK;
+  std::shared_ptr<LeoDecodeEntry> e;
 };
 
 struct LeoState {
@@ -319,8 +341,11 @@ struct LeoState {
   // device copies of the field log/exp tables (for the error locator and tables)
   DeviceBuffer d_log, d_exp;
   size_t lds_limit = 65536;
+  // decode plans by presence pattern: the 64 most recently used, plus the ones
+  // in flight (guarded by mu; built outside it)
   std::mutex mu;
-  std::map<std::string, std::shared_ptr<LeoDecodeEntry>> decode_cache;
+  LruMap<LeoDecodeEntry> decode_cache{64};
+  std::map<std::string, std::shared_ptr<LeoPending>> in_flight;
 };
 
 // ---------------------------------------------------------------- kernels
@@ -1407,18 +1432,12 @@ int leopard_verify(const Codec& c, const rsgpu_cslice* shards, size_t len, int32
   return launch_encode(c, ptrs, len, 1, d_flag, s, variant);
 }
 
-// Decode plan per erasure pattern (DecodePlan::new, leopard_gf8/decode.rs:130-152;
-// build_decode_plan, leopard_gf16/decode.rs:507-560).
-static int decode_entry(const Codec& c, const std::vector<uint8_t>& present, hipStream_t s,
-                        std::shared_ptr<LeoDecodeEntry>& out) {
+// Builds the decode plan of one erasure pattern (DecodePlan::new,
+// leopard_gf8/decode.rs:130-152; build_decode_plan, leopard_gf16/decode.rs:507-560):
+// the transform schedule, then the locator tables by a kernel on stream s.
+static int build_decode_entry(const Codec& c, const std::vector<uint8_t>& present, hipStream_t s,
+                              std::shared_ptr<LeoDecodeEntry>& out) {
   LeoState& st = state(c);
-  std::string key(present.begin(), present.end());
-  std::lock_guard<std::mutex> g(st.mu);
-  auto it = st.decode_cache.find(key);
-  if (it != st.decode_cache.end()) {
-    out = it->second;
-    return RSGPU_OK;
-  }
   const LeoField& F = st.gf16 ? field16() : field8();
   auto e = std::make_shared<LeoDecodeEntry>();
   uint32_t input_count = st.m;
@@ -1438,27 +1457,84 @@ static int decode_entry(const Codec& c, const std::vector<uint8_t>& present, hip
   for (uint32_t i = 0; i < st.k; ++i)
     if (!present[i]) pos.push_back(st.m + i);
   const int TW = st.gf16 ? kLeoTabWords16 : kLeoTabWords8;
-  DeviceBuffer dpos;
-  if (dpos.alloc(std::max<size_t>(1, pos.size()) * 4) != hipSuccess ||
+  if (e->pos.alloc(std::max<size_t>(1, pos.size()) * 4) != hipSuccess ||
       e->err_tabs.alloc((size_t)st.n * TW * 4) != hipSuccess ||
       e->inv_tabs.alloc((size_t)st.n * TW * 4) != hipSuccess ||
       e->err_locs.alloc((size_t)st.work_size * 4) != hipSuccess)
     return RSGPU_ERR_OUT_OF_MEMORY;
-  if (!pos.empty() && hipMemcpyAsync(dpos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess)
+  if (!pos.empty() && hipMemcpy(e->pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     return RSGPU_ERR_HIP;
   ErrJob job{st.work_size, (uint32_t)pos.size(), st.gf16 ? 1u : 0u, F.modulus};
   hipLaunchKernelGGL(leo_errloc_kernel, dim3((st.work_size + 255) / 256), dim3(256), 0, s,
-                     (const uint32_t*)st.d_log.p, (const uint32_t*)st.d_exp.p, (const uint32_t*)dpos.p,
+                     (const uint32_t*)st.d_log.p, (const uint32_t*)st.d_exp.p, (const uint32_t*)e->pos.p,
                      job, (uint32_t*)e->err_locs.p, (uint32_t*)e->err_tabs.p, (uint32_t*)e->inv_tabs.p);
   hipError_t he = hipGetLastError();
   if (he != hipSuccess) {
     set_hip_error(he, "leo_errloc_kernel");
     return RSGPU_ERR_HIP;
   }
-  if (hipStreamSynchronize(s) != hipSuccess) return RSGPU_ERR_HIP;  // dpos freed on return
-  if (st.decode_cache.size() >= 64) st.decode_cache.clear();
-  st.decode_cache[key] = e;
-  out = e;
+  HIP_TRY(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(e->ready, s));
+  e->stream = s;
+  out = std::move(e);
+  return RSGPU_OK;
+}
+
+// Orders a launch on s after the entry's locator kernel when that kernel ran on
+// another stream and is still pending.
+static int wait_decode_entry(const LeoDecodeEntry& e, hipStream_t s) {
+  if (e.stream == s || hipEventQuery(e.ready) == hipSuccess) return RSGPU_OK;
+  HIP_TRY(hipStreamWaitEvent(s, e.ready, 0));
+  return RSGPU_OK;
+}
+
+// The decode plan of a presence pattern: from the cache, from a build another
+// thread has in flight, or built here.  Builds run outside the codec mutex, so a
+// hit never waits behind a miss; entries evicted to make room are freed after
+// the mutex is released (the reference shares GF16 plans the same way and
+// dedupes builds in flight, leopard_gf16/decode.rs:586-684).
+static int decode_entry(const Codec& c, const std::vector<uint8_t>& present, hipStream_t s,
+                        std::shared_ptr<LeoDecodeEntry>& out) {
+  LeoState& st = state(c);
+  const std::string key(present.begin(), present.end());
+  std::shared_ptr<LeoPending> pend;
+  bool builder = false;
+  {
+    std::lock_guard<std::mutex> g(st.mu);
+    if ((out = st.decode_cache.get(key))) return wait_decode_entry(*out, s);
+    auto it = st.in_flight.find(key);
+    if (it != st.in_flight.end()) {
+      pend = it->second;
+    } else {
+      pend = std::make_shared<LeoPending>();
+      st.in_flight.emplace(key, pend);
+      builder = true;
+    }
+  }
+  if (!builder) {
+    std::unique_lock<std::mutex> lk(pend->m);
+    pend->cv.wait(lk, [&] { return pend->done; });
+    if (pend->rc) return pend->rc;
+    out = pend->e;
+    return wait_decode_entry(*out, s);
+  }
+  std::shared_ptr<LeoDecodeEntry> e;
+  const int rc = build_decode_entry(c, present, s, e);
+  std::vector<std::shared_ptr<LeoDecodeEntry>> evicted;  // freed after the lock below
+  {
+    std::lock_guard<std::mutex> g(st.mu);
+    st.in_flight.erase(key);
+    if (!rc) st.decode_cache.put(key, e, evicted);
+  }
+  {
+    std::lock_guard<std::mutex> lk(pend->m);
+    pend->done = true;
+    pend->rc = rc;
+    pend->e = e;
+  }
+  pend->cv.notify_all();
+  if (rc) return rc;
+  out = std::move(e);
   return RSGPU_OK;
 }
 

@@ -120,3 +120,57 @@ def test_host_threads_share_codecs():
         t.join()
     assert not errors, errors[:5]
     assert rs.cache_stats()["entries"] <= 16
+
+
+def test_leopard_gf16_plans_under_concurrent_host_threads():
+    """Four host threads share one Leopard GF(2^16) codec, each on its own stream.
+    They ask for 96 erasure patterns (past the 64-plan cache, so least recently
+    used plans are evicted while others are in use), with every thread starting
+    on the same pattern (one build in flight, the others wait for it).  Every
+    rebuilt shard equals the original."""
+    import threading
+
+    announce("leopard gf16 decode plans under concurrent host threads")
+    k, m, n = 20, 6, 4096
+    rng0 = np.random.default_rng(16)
+    data = [rng0.integers(0, 256, n, dtype=np.uint8) for _ in range(k)]
+    rc, par = O.leo_encode(16, k, m, data)
+    assert rc == 0
+    full = np.stack(data + list(par))
+    leo = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf16")
+    dev = torch.from_numpy(full).cuda()
+    torch.cuda.synchronize()
+    prng = np.random.default_rng(17)
+    patterns = []
+    while len(patterns) < 96:
+        p = tuple(sorted(prng.choice(k + m, int(prng.integers(1, m + 1)), replace=False).tolist()))
+        if p not in patterns:
+            patterns.append(p)
+    errors = []
+    start = threading.Barrier(4)
+
+    def worker(tid):
+        try:
+            s = torch.cuda.Stream()
+            rng = np.random.default_rng(200 + tid)
+            start.wait()
+            for it in range(60):
+                missing = patterns[0] if it == 0 else patterns[int(rng.integers(len(patterns)))]
+                slots = [rsgpu.ShardSlot(torch.full((n,), 0xA5, dtype=torch.uint8, device="cuda")
+                                         if i in missing else dev[i], present=i not in missing)
+                         for i in range(k + m)]
+                s.wait_stream(torch.cuda.current_stream())
+                leo.reconstruct(slots, stream=s)
+                s.synchronize()
+                for i in missing:
+                    if not (slots[i].data.cpu().numpy() == full[i]).all():
+                        errors.append((tid, it, missing, i))
+        except Exception as e:  # reported below
+            errors.append((tid, repr(e)))
+
+    ts = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors[:5]

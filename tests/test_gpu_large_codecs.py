@@ -1,7 +1,8 @@
 """GPU parity at the classic codecs' large sizes: GF(2^8) at the 256-shard limit
-and GF(2^16) with more than 256 data shards or reconstruct targets (the decode
-solve then runs as C^-1 in one workgroup plus a grid over the coefficients),
-against the oracle's encode and reconstruct."""
+and GF(2^16) with more than 256 data shards, more than 256 reconstruct targets or
+more than 128 missing data shards (the decode solve then runs as C^-1 by one
+workgroup -- in LDS, or over a device work matrix beyond 128 rows -- plus a grid
+over the coefficients), against the oracle's encode and reconstruct."""
 import numpy as np
 import pytest
 
@@ -59,15 +60,33 @@ def test_gf16_beyond_256(k, m, n_missing_data, n_missing_parity):
     _check("gf16", k, m, 256 + 2, [missing], k * 3 + m)
 
 
-def test_gf16_more_missing_data_than_one_solve_is_unsupported():
-    announce("gf16 large unsupported solve size")
-    k, m, n = 300, 200, 64
+@pytest.mark.parametrize("k,m,n_missing_data,n_missing_parity",
+                         [(300, 200, 150, 0), (300, 200, 200, 0), (300, 200, 129, 60),
+                          (200, 150, 140, 10), (250, 6, 6, 0)])
+def test_gf16_more_missing_data_than_one_lds_solve(k, m, n_missing_data, n_missing_parity):
+    """More than 128 missing data shards (the reference solves any pattern of up to
+    m erasures through its full inverse, reconstruct.rs:1164-1173): C^-1 runs as a
+    Gauss-Jordan over a device work matrix.  Codeword round trip, then the bytes of
+    a non-codeword against the oracle."""
+    announce(f"gf16 big solve {k}+{m} missing {n_missing_data}d {n_missing_parity}p")
+    rng = np.random.default_rng(k + m + n_missing_data + n_missing_parity)
+    missing = set(rng.choice(k, n_missing_data, replace=False).tolist()) | \
+        set((k + rng.choice(m, n_missing_parity, replace=False)).tolist())
+    _check("gf16", k, m, 128 + 2, [missing], k * 5 + m)
+    n = 64
+    rc, mat = O.build_matrix(O.GF16, O.VANDERMONDE, k, m)
+    shards = rand_shards(rng, k + m, n)  # not a codeword
+    present = [0 if i in missing else 1 for i in range(k + m)]
+    rc, exp = O.reconstruct(O.GF16, mat, k, m, [x.copy() for x in shards], present)
+    assert rc == 0
     rs = rsgpu.ReedSolomon(k, m, field="gf16")
-    sh = [torch.zeros(n, dtype=torch.uint8, device="cuda") for _ in range(k + m)]
-    work = [None if i < 150 else sh[i] for i in range(k + m)]  # 150 data rows > 128
-    with pytest.raises(rsgpu.RsError) as ei:
-        rs.reconstruct(work)
-    assert ei.value.name == "Unsupported"
+    slots = [rsgpu.ShardSlot(to_dev(shards[i]) if i not in missing else
+                             torch.full((n,), 0xA5, dtype=torch.uint8, device="cuda"),
+                             present=i not in missing) for i in range(k + m)]
+    rs.reconstruct(slots)
+    torch.cuda.synchronize()
+    for i in sorted(missing):
+        assert (to_host(slots[i].data) == exp[i]).all(), i
 
 
 def test_batch_wide_stripes_with_tails():
