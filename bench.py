@@ -207,33 +207,69 @@ def measured_traffic(bytes_per_launch):
     return d["traffic_bytes_per_launch"]
 
 
-def cpu_baseline(k, m, shard, stripes, threads):
-    """The reference's SIMD CPU encode restated in C (oracle/cpu_simd.c: GFNI affine
-    rows on AVX-512, AVX2 nibble tables otherwise; 64 KiB chunks spread over a
-    thread pool), on `stripes` distinct stripes so the data is not cache-resident."""
+def cpu_baseline(k, m, shard, stripes, threads, erasures):
+    """The reference's SIMD CPU path restated in C (oracle/cpu_simd.c: GFNI affine
+    rows on AVX-512, AVX2 nibble tables otherwise; 16/64/256 KiB chunks spread over
+    a thread pool), on `stripes` distinct host-resident stripes so the data is not
+    cache-resident: encode, verify (recompute and compare) and reconstruct (the
+    reference's data-then-parity stages over the GPU run's erasure patterns, decode
+    rows computed once per pattern as its inversion cache holds them).  Plus the
+    64 x 10+4 x 64 KiB set with 4 random erasures per stripe."""
     import numpy as np
 
     import oracle_lib as O
 
     _, mat = O.build_matrix(O.GF8, O.VANDERMONDE, k, m)
-    rng = np.random.default_rng(1)
-    data = [[np.frombuffer(rng.bytes(shard), np.uint8) for _ in range(k)] for _ in range(stripes)]
-    par = [[np.zeros(shard, np.uint8) for _ in range(m)] for _ in range(stripes)]
-    isa, _ = O.cpu_encode(mat, k, m, data[0], threads=threads, parity=par[0])  # warm
-    # check one stripe against the scalar oracle before timing
-    exp = O.encode(O.GF8, mat, k, m, data[0])
-    assert all(np.array_equal(a, b) for a, b in zip(par[0], exp)), "CPU baseline differs from oracle"
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        for s in range(stripes):
-            O.cpu_encode(mat, k, m, data[s], threads=threads, parity=par[s])
-    dt = time.perf_counter() - t0
-    return {"value": round(reps * stripes * k * shard / dt / 1e9, 3), "unit": "GB/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{reps} x {stripes} distinct stripes of RS {k}+{m} x {shard // 1024} KiB encode; "
-                      f"C restatement of the reference's SIMD row kernels ({O.CPU_ISA[isa]}), "
-                      f"{threads} threads"}
+
+    def run(S, n, erased, min_s=1.0):
+        rng = np.random.default_rng(S + n)
+        data = [[np.frombuffer(rng.bytes(S), np.uint8).copy() for _ in range(k)] for _ in range(n)]
+        par = [[np.zeros(S, np.uint8) for _ in range(m)] for _ in range(n)]
+        isa, _ = O.cpu_encode(mat, k, m, data[0], threads=threads, parity=par[0])  # warm
+        exp = O.encode(O.GF8, mat, k, m, data[0])
+        assert all(np.array_equal(a, b) for a, b in zip(par[0], exp)), "CPU baseline differs from oracle"
+        out = {}
+
+        def timeit(fn):
+            fn()
+            reps, t0 = 0, time.perf_counter()
+            while True:
+                fn()
+                reps += 1
+                dt = time.perf_counter() - t0
+                if dt >= min_s:
+                    return reps * n * k * S / dt / 1e9, reps
+
+        out["encode"], reps_e = timeit(lambda: [O.cpu_encode(mat, k, m, data[s], threads=threads, parity=par[s])
+                                                for s in range(n)])
+        full = [data[s] + par[s] for s in range(n)]
+        oks = []
+        out["verify"], _ = timeit(lambda: oks.extend(O.cpu_verify(mat, k, m, full[s], threads=threads)[1]
+                                                     for s in range(n)))
+        assert all(oks), "CPU verify reported a mismatch on encoded stripes"
+        present = [[0 if i in erased[s] else 1 for i in range(k + m)] for s in range(n)]
+        rows = [O.cpu_data_rows(mat, k, m, present[s]) for s in range(n)]
+        ref = [[x.copy() for x in full[s]] for s in range(n)]
+        for s in range(n):
+            for i in erased[s]:
+                full[s][i][:] = 0xA5
+        out["reconstruct"], _ = timeit(lambda: [O.cpu_reconstruct(mat, k, m, full[s], present[s], rows[s],
+                                                                  threads=threads) for s in range(n)])
+        assert all(np.array_equal(a, b) for s in range(n) for a, b in zip(full[s], ref[s])), \
+            "CPU reconstruct differs from the encoded stripes"
+        return {kk: round(v, 3) for kk, v in out.items()}, isa
+
+    main, isa = run(shard, stripes, erasures[:stripes])
+    rng = np.random.default_rng(4)
+    er1 = [set(rng.choice(k + m, 4, replace=False).tolist()) for _ in range(64)]
+    small, _ = run(64 << 10, 64, er1)
+    return {"value": main["encode"], "unit": "GB/s", "cores": threads, "kind": "port",
+            "encode": main["encode"], "verify": main["verify"], "reconstruct": main["reconstruct"],
+            "config1_64x10+4x64KiB": small,
+            "sample": f"{stripes} distinct host-resident stripes of RS {k}+{m} x {shard // 1024} KiB, repeated "
+                      f"for >= 1 s per op (reconstruct: the GPU run's 1..{m} erasures per stripe), and "
+                      f"64 x {k}+{m} x 64 KiB with 4 erasures; C restatement of the reference's SIMD row kernels "
+                      f"({O.CPU_ISA[isa]}), {threads} threads; data GB/s"}
 
 
 def leopard_section(torch, rsgpu, DSP, disp):
@@ -633,7 +669,7 @@ def main():
     serving = serving_section(torch, [int(r) for r in args.rates.split(",")]) if args.rates else None
     host = host_section(rsgpu)
 
-    cpu = None if args.no_cpu else cpu_baseline(k, m, S, args.cpu_stripes, args.cpu_threads)
+    cpu = None if args.no_cpu else cpu_baseline(k, m, S, args.cpu_stripes, args.cpu_threads, er)
     rec_moved = N * k * S + n_erased * S
     out = {
         "metric": "data GB/s RS 10+4 encode/reconstruct, batched vs plain, bit-exact; p99 ms",

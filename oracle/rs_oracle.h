@@ -136,6 +136,11 @@ uint64_t ora_checksum_shards(const uint8_t* const* shards, const size_t* lens, s
 int ora_cpu_isa(void);
 int ora_cpu_encode(const uint16_t* matrix, int k, int m, const uint8_t* const* data,
                    uint8_t* const* parity, size_t len, int threads, int isa);
+int ora_cpu_verify(const uint16_t* matrix, int k, int m, const uint8_t* const* shards, size_t len,
+                   int threads, int isa, int* ok);
+int ora_cpu_reconstruct(const uint16_t* matrix, int k, int m, uint8_t* const* shards,
+                        const uint8_t* present, const uint8_t* data_rows, size_t len, int threads,
+                        int isa);
 
 #ifdef __cplusplus
 }

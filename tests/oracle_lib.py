@@ -84,6 +84,11 @@ def _declare(L):
     L.ora_cpu_isa.restype = C.c_int
     L.ora_cpu_encode.argtypes = [u16p, C.c_int, C.c_int, ptrs, ptrs, C.c_size_t, C.c_int, C.c_int]
     L.ora_cpu_encode.restype = C.c_int
+    L.ora_cpu_verify.argtypes = [u16p, C.c_int, C.c_int, ptrs, C.c_size_t, C.c_int, C.c_int,
+                                 C.POINTER(C.c_int)]
+    L.ora_cpu_verify.restype = C.c_int
+    L.ora_cpu_reconstruct.argtypes = [u16p, C.c_int, C.c_int, ptrs, u8p, u8p, C.c_size_t, C.c_int, C.c_int]
+    L.ora_cpu_reconstruct.restype = C.c_int
     L.ora_checksum_shards.restype = C.c_uint64
     L.ora_checksum_shards.argtypes = [ptrs, C.POINTER(C.c_size_t), C.c_size_t]
 
@@ -308,3 +313,29 @@ def cpu_encode(matrix, k, m, data, threads=1, isa=0, parity=None):
     used = lib().ora_cpu_encode(_u16(matrix), k, m, _ptrs(data), _ptrs(parity), data[0].size,
                                 threads, isa)
     return used, parity
+
+
+def cpu_verify(matrix, k, m, shards, threads=1, isa=0):
+    """The CPU baseline's verify; returns (isa used, ok)."""
+    shards = [np.ascontiguousarray(x, np.uint8) for x in shards]
+    matrix = np.ascontiguousarray(matrix, np.uint16)
+    ok = C.c_int(0)
+    used = lib().ora_cpu_verify(_u16(matrix), k, m, _ptrs(shards), shards[0].size, threads, isa, C.byref(ok))
+    return used, bool(ok.value)
+
+
+def cpu_data_rows(matrix, k, m, present):
+    """Decode rows (over the first k present shards) of the missing data shards,
+    as one uint8 array of n_missing_data x k: the baseline's cached inverse."""
+    rc, valid, dec = decode_matrix(GF8, matrix, k, m, present)
+    assert rc == 0
+    rows = [dec[d] for d in range(k) if not present[d]]
+    return np.ascontiguousarray(np.array(rows, np.uint8).reshape(-1) if rows else np.zeros(1, np.uint8))
+
+
+def cpu_reconstruct(matrix, k, m, shards, present, data_rows, threads=1, isa=0):
+    """The CPU baseline's reconstruct into the missing entries of `shards` (in place)."""
+    matrix = np.ascontiguousarray(matrix, np.uint16)
+    pres = np.ascontiguousarray(present, np.uint8)
+    return lib().ora_cpu_reconstruct(_u16(matrix), k, m, _ptrs(shards), _u8(pres), _u8(data_rows),
+                                     shards[0].size, threads, isa)
