@@ -63,6 +63,8 @@ def parse():
                          "reconstruct with m erasures, default kernel choice")
     ap.add_argument("--mfma-only", action="store_true",
                     help="only the int8 matrix-core vs VALU encode comparison")
+    ap.add_argument("--host-only", action="store_true",
+                    help="only the host-boundary section (host-pointer and block-stream rates)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the tuned batch encode launches plus a 1 GiB device copy "
                          "(the reference kernel for scaling rocprofv3 byte counters)")
@@ -368,10 +370,9 @@ def host_section(rsgpu):
     """PCIe-inclusive rates (never `value`) on 10+4 with host-resident shards: the
     host-pointer entry points (a two-slot pipeline over 512 KiB column chunks:
     threaded copies into pinned staging, both PCIe directions and the kernels
-    overlap) at 4 MiB and 64 MiB shards, and the block-stream encode (4 MiB blocks
-    through pinned double-buffered staging, reader/writer callbacks in Python)."""
-    import io
-
+    overlap) at 4 MiB and 64 MiB shards, and the block streams (4 MiB blocks
+    through pinned double-buffered staging; the library's C callbacks over host
+    memory, so no Python runs per block)."""
     import numpy as np
 
     k, m = 10, 4
@@ -398,14 +399,32 @@ def host_section(rsgpu):
             out[f"reconstruct_host_{tag}_4_erasures_gbs"] = round(
                 k * S / per_call(lambda: rs.reconstruct_host(shards, present), reps) / 1e9, 2)
         del shards
-    S2 = 64 << 20
-    data = [rng.bytes(S2) for _ in range(k)]
-    t0 = time.perf_counter()
-    writers = [io.BytesIO() for _ in range(m)]
-    rs.encode_stream([io.BytesIO(d) for d in data], writers, block_size=4 << 20)
-    t_stream = time.perf_counter() - t0
-    out.update({"encode_stream_gbs": round(k * S2 / t_stream / 1e9, 2),
-                "encode_stream_bytes_per_shard": S2, "stream_block_bytes": 4 << 20})
+    # block streams over host memory through the library's native C callbacks
+    # (HostStream): 64 MiB per shard in 4 MiB blocks, each I/O mode
+    S2, blk = 64 << 20, 4 << 20
+    data = [np.frombuffer(rng.bytes(S2), np.uint8).copy() for _ in range(k)]
+    parity = [np.zeros(S2, np.uint8) for _ in range(m)]
+    HS = rsgpu.HostStream
+    for mode in ("auto", "serial", "parallel"):
+        t = per_call(lambda: rs.encode_stream([HS(d) for d in data], [HS(p) for p in parity],
+                                              block_size=blk, io_mode=mode), 2)
+        out[f"encode_stream_{mode}_gbs"] = round(k * S2 / t / 1e9, 2)
+    full = data + parity
+    t = per_call(lambda: rs.verify_stream([HS(x) for x in full], block_size=blk), 2)
+    out["verify_stream_auto_gbs"] = round(k * S2 / t / 1e9, 2)
+    lost = (0, 5, 10, 13)
+    rebuilt = {i: np.zeros(S2, np.uint8) for i in lost}
+
+    def rebuild():
+        rs.reconstruct_stream_from([None if i in lost else HS(full[i]) for i in range(k + m)],
+                                   [HS(rebuilt[i]) if i in lost else None for i in range(k + m)],
+                                   block_size=blk)
+
+    t = per_call(rebuild, 2)
+    out["reconstruct_stream_4_erasures_auto_gbs"] = round(k * S2 / t / 1e9, 2)
+    out["stream_outputs_match"] = bool(all((rebuilt[i] == full[i]).all() for i in lost))
+    out.update({"stream_bytes_per_shard": S2, "stream_block_bytes": blk,
+                "stream_callbacks": "native C (rsgpu_mem_stream_read / _write)"})
     return out
 
 
@@ -469,6 +488,14 @@ def main():
             t = DSP.OpCase(torch, rs, fam, k, m, S, op, e, seed=1).time([-1])[-1]
             out[op] = {"us": round(t, 1), "data_gbs": round(k * S / t / 1e3, 1)}
         print(json.dumps({"leopard_case": {"family": fam, "k": k, "m": m, "shard": S, **out}}))
+        return
+    if args.host_only:
+        import torch
+
+        import rsgpu
+
+        torch.cuda.init()
+        print(json.dumps({"host_boundary": host_section(rsgpu)}))
         return
     if args.mfma_only:
         import torch

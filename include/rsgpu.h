@@ -281,9 +281,34 @@ int rsgpu_leopard_reconstruct(int32_t bits, size_t data_shards, size_t parity_sh
  * Failures return RSGPU_ERR_IO (callback failure) or a codec status, with the
  * StreamError of the reference (stream.rs:107-171) in *err: kind, shard index
  * and the callback's return value.  Leopard codecs return
- * RSGPU_UNSUPPORTED_CODEC_FAMILY as in the reference.  Blocks are staged through
- * two pinned host buffers so host I/O overlaps the device copies and kernels. */
+ * RSGPU_UNSUPPORTED_CODEC_FAMILY as in the reference.
+ *
+ * Pipelining: blocks are staged through two pinned host buffers.  While the
+ * device copies and encodes block i, the host reads block i+1; while the device
+ * works on block i+1, the host writes the outputs of block i.  Consequences for
+ * stream positions (the reference, stream.rs:918-925, promises none on error):
+ *   - the outputs of block i are written before an error of block i+1 (a read
+ *     error, a short shard) is returned, as in the reference;
+ *   - when a write of block i fails, or verify finds block i inconsistent, the
+ *     readers have also consumed block i+1 (the reference stops after block i);
+ *   - on success every reader is at its end, as in the reference.
+ *
+ * I/O mode (StreamOptions::io_mode, stream.rs:49-104, 214-226): SERIAL calls
+ * every callback on the calling thread, one shard after another.  PARALLEL calls
+ * the callbacks of one block's shards concurrently from library worker threads
+ * (at most one call per shard at a time, and a block's calls end before the
+ * next block's start); the callbacks must then be safe to call concurrently for
+ * different shards, as the reference requires `Read + Send`.  AUTO (default)
+ * is PARALLEL when block_size >= 4 MiB and at least 10 streams take part in that
+ * direction (reads or writes), else SERIAL.  When several shards of a parallel
+ * block fail, the lowest shard index is reported. */
 enum { RSGPU_STREAM_OK = 0, RSGPU_STREAM_READ = 1, RSGPU_STREAM_WRITE = 2, RSGPU_STREAM_CODEC = 3 };
+enum { RSGPU_STREAM_IO_AUTO = 0, RSGPU_STREAM_IO_SERIAL = 1, RSGPU_STREAM_IO_PARALLEL = 2 };
+typedef struct rsgpu_stream_options {
+  size_t block_size;   /* bytes per shard per block; default 4 MiB, clamped per call */
+  int32_t io_mode;     /* RSGPU_STREAM_IO_* */
+  int32_t reserved;    /* 0 */
+} rsgpu_stream_options;
 typedef struct rsgpu_stream_error {
   int32_t kind;        /* RSGPU_STREAM_* */
   int32_t shard_index;
@@ -291,19 +316,41 @@ typedef struct rsgpu_stream_error {
 } rsgpu_stream_error;
 typedef int64_t (*rsgpu_read_cb)(void* user, size_t shard, uint8_t* buf, size_t len);
 typedef int64_t (*rsgpu_write_cb)(void* user, size_t shard, const uint8_t* buf, size_t len);
+/* StreamOptions::default() (stream.rs:73-80): 4 MiB blocks, AUTO.  A NULL
+ * options pointer in the calls below means these defaults. */
+void rsgpu_stream_options_default(rsgpu_stream_options* options);
 /* ReedSolomon::encode_stream (stream.rs:677-752): readers 0..k-1 are the data
  * streams, writers 0..m-1 the parity streams (error index k + j). */
 int rsgpu_encode_stream(const rsgpu_codec* codec, size_t n_data, size_t n_parity, rsgpu_read_cb read,
-                        rsgpu_write_cb write, void* user, size_t block_size, rsgpu_stream_error* err);
+                        rsgpu_write_cb write, void* user, const rsgpu_stream_options* options,
+                        rsgpu_stream_error* err);
 /* ReedSolomon::verify_stream (stream.rs:760-812). */
 int rsgpu_verify_stream(const rsgpu_codec* codec, size_t n_shards, rsgpu_read_cb read, void* user,
-                        size_t block_size, int32_t* ok, rsgpu_stream_error* err);
+                        const rsgpu_stream_options* options, int32_t* ok, rsgpu_stream_error* err);
 /* ReedSolomon::reconstruct_stream_from (stream.rs:948-1031): has_reader[i] /
  * has_writer[i] give the Some/None layout of the reference's two slices. */
 int rsgpu_reconstruct_stream_from(const rsgpu_codec* codec, size_t n_readers, size_t n_writers,
                                   const uint8_t* has_reader, const uint8_t* has_writer,
                                   rsgpu_read_cb read, rsgpu_write_cb write, void* user,
-                                  size_t block_size, rsgpu_stream_error* err);
+                                  const rsgpu_stream_options* options, rsgpu_stream_error* err);
+/* In-memory shard streams for the calls above, the C form of the reference's
+ * std::io::Cursor streams (stream.rs:839-846): pass rsgpu_mem_stream_read /
+ * rsgpu_mem_stream_write as the callbacks and a rsgpu_mem_streams as `user`.
+ * A read copies from data[pos..len) and advances pos (0 at the end).  A write
+ * copies into data[pos..len) and advances pos; a full buffer accepts 0 bytes,
+ * which the stream calls report as a write error (the WriteZero of a Cursor
+ * over a fixed slice).  Safe in PARALLEL mode: each call touches one shard. */
+typedef struct rsgpu_mem_stream {
+  uint8_t* data;
+  size_t len;
+  size_t pos;
+} rsgpu_mem_stream;
+typedef struct rsgpu_mem_streams {
+  rsgpu_mem_stream* readers;  /* indexed by the reader's shard argument */
+  rsgpu_mem_stream* writers;  /* indexed by the writer's shard argument */
+} rsgpu_mem_streams;
+int64_t rsgpu_mem_stream_read(void* streams, size_t shard, uint8_t* buf, size_t len);
+int64_t rsgpu_mem_stream_write(void* streams, size_t shard, const uint8_t* buf, size_t len);
 
 /* ---- host-pointer variants (copy through library-owned device staging) ---- */
 int rsgpu_encode_host(const rsgpu_codec* codec, uint8_t* const* shards, size_t n_shards,

@@ -281,6 +281,9 @@ int check_len(const Codec& c, size_t len) {
 }
 
 }  // namespace
+
+void host_parallel_for(size_t n, const std::function<void(size_t)>& fn) { CopyPool::get().run(n, fn); }
+
 }  // namespace rsgpu
 
 using namespace rsgpu;

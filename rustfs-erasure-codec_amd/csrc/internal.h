@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <cstddef>
+#include <functional>
 #include <list>
 #include <memory>
 #include <mutex>
@@ -153,6 +154,10 @@ hipError_t launch_decode_big(const uint16_t* d_matrix, const DecodeBigJob& job,
 // Writes n 16-bit values from the host to device memory on stream s through
 // kernel arguments (no staging buffer, nothing to keep alive after the call).
 hipError_t launch_store_u16(uint16_t* dst, const uint16_t* vals, size_t n, hipStream_t s);
+
+// Calls fn(0..n-1) on the library's persistent host worker threads and the
+// calling thread; returns when every call has finished.
+void host_parallel_for(size_t n, const std::function<void(size_t)>& fn);
 
 // Tables for a host-known row set (encode rows, apply_rows primitive).
 void build_tables_host(const uint16_t* rows, int r, int k, bool tower, std::vector<uint32_t>& out,

@@ -545,35 +545,45 @@ class ReedSolomon:
                                  _stream(stream)))
 
     # ---- block streams (src/core/stream.rs:677-1031)
-    def encode_stream(self, data, parity, block_size=4 << 20):
+    # block_size and io_mode are the reference's StreamOptions (stream.rs:49-104);
+    # io_mode "auto" | "serial" | "parallel".  With HostStream readers and writers
+    # the library's native in-memory callbacks run (no Python per block);
+    # other streams go through Python callbacks (readinto/read, write).
+    def encode_stream(self, data, parity, block_size=4 << 20, io_mode="auto"):
         """Reads the k data streams block by block and writes the m parity streams."""
-        io = _StreamIO(list(data), list(parity), writer_offset=0)
+        io = _stream_io(list(data), list(parity))
         err = _StreamErr()
-        rc = _lib.rsgpu_encode_stream(self._h, len(io.readers), len(io.writers), io.rd, io.wr, None,
-                                      block_size, C.byref(err))
+        rc = _lib.rsgpu_encode_stream(self._h, len(io.readers), len(io.writers), io.rd, io.wr, io.user,
+                                      C.byref(_stream_options(block_size, io_mode)), C.byref(err))
+        io.finish()
         io.raise_for(rc, err, writer_base=self._k)
 
-    def verify_stream(self, shards, block_size=4 << 20):
-        io = _StreamIO(list(shards), [], writer_offset=0)
+    def verify_stream(self, shards, block_size=4 << 20, io_mode="auto"):
+        io = _stream_io(list(shards), [])
         err = _StreamErr()
         ok = C.c_int32()
-        rc = _lib.rsgpu_verify_stream(self._h, len(io.readers), io.rd, None, block_size, C.byref(ok),
+        rc = _lib.rsgpu_verify_stream(self._h, len(io.readers), io.rd, io.user,
+                                      C.byref(_stream_options(block_size, io_mode)), C.byref(ok),
                                       C.byref(err))
+        io.finish()
         io.raise_for(rc, err)
         return bool(ok.value)
 
-    def reconstruct_stream_from(self, readers, writers, block_size=4 << 20):
+    def reconstruct_stream_from(self, readers, writers, block_size=4 << 20, io_mode="auto"):
         """readers[i] / writers[i]: a stream or None (the reference's Option layout)."""
         readers, writers = list(readers), list(writers)
-        io = _StreamIO(readers, writers, writer_offset=0)
+        io = _stream_io(readers, writers)
         hr = (C.c_uint8 * max(1, len(readers)))(*[r is not None for r in readers])
         hw = (C.c_uint8 * max(1, len(writers)))(*[w is not None for w in writers])
         err = _StreamErr()
         rc = _lib.rsgpu_reconstruct_stream_from(self._h, len(readers), len(writers), hr, hw, io.rd,
-                                                io.wr, None, block_size, C.byref(err))
+                                                io.wr, io.user,
+                                                C.byref(_stream_options(block_size, io_mode)),
+                                                C.byref(err))
+        io.finish()
         io.raise_for(rc, err)
 
-    def reconstruct_stream(self, shards, block_size=4 << 20):
+    def reconstruct_stream(self, shards, block_size=4 << 20, io_mode="auto"):
         """In-memory form (stream.rs:839-930): `shards` are io.BytesIO; non-empty ones are
         present (read from position 0), empty ones receive the rebuilt bytes."""
         total = self.total_shard_count()
@@ -595,7 +605,7 @@ class ReedSolomon:
             else:
                 readers.append(None)
                 writers.append(_AppendWriter(s))
-        self.reconstruct_stream_from(readers, writers, block_size)
+        self.reconstruct_stream_from(readers, writers, block_size, io_mode)
 
     # ---- kernel-variant dispatch (every variant produces the same bytes)
     def set_variant(self, variant):
@@ -632,10 +642,131 @@ class _StreamErr(C.Structure):
 
 _READ_CB = C.CFUNCTYPE(C.c_int64, _vp, _sz, C.POINTER(C.c_uint8), _sz)
 _WRITE_CB = C.CFUNCTYPE(C.c_int64, _vp, _sz, C.POINTER(C.c_uint8), _sz)
-_lib.rsgpu_encode_stream.argtypes = [_vp, _sz, _sz, _READ_CB, _WRITE_CB, _vp, _sz, C.POINTER(_StreamErr)]
-_lib.rsgpu_verify_stream.argtypes = [_vp, _sz, _READ_CB, _vp, _sz, _i32p, C.POINTER(_StreamErr)]
-_lib.rsgpu_reconstruct_stream_from.argtypes = [_vp, _sz, _sz, _u8p, _u8p, _READ_CB, _WRITE_CB, _vp, _sz,
+IO_MODES = {"auto": 0, "serial": 1, "parallel": 2}
+
+
+class StreamOptions(C.Structure):
+    _fields_ = [("block_size", _sz), ("io_mode", C.c_int32), ("reserved", C.c_int32)]
+
+
+def _stream_options(block_size, io_mode):
+    mode = IO_MODES[io_mode] if isinstance(io_mode, str) else int(io_mode)
+    return StreamOptions(int(block_size), mode, 0)
+
+
+class _MemStream(C.Structure):
+    _fields_ = [("data", _vp), ("len", _sz), ("pos", _sz)]
+
+
+class _MemStreams(C.Structure):
+    _fields_ = [("readers", C.POINTER(_MemStream)), ("writers", C.POINTER(_MemStream))]
+
+
+_SOPT = C.POINTER(StreamOptions)
+_lib.rsgpu_stream_options_default.argtypes = [_SOPT]
+_lib.rsgpu_stream_options_default.restype = None
+_lib.rsgpu_mem_stream_read.argtypes = [_vp, _sz, C.POINTER(C.c_uint8), _sz]
+_lib.rsgpu_mem_stream_read.restype = C.c_int64
+_lib.rsgpu_mem_stream_write.argtypes = [_vp, _sz, C.POINTER(C.c_uint8), _sz]
+_lib.rsgpu_mem_stream_write.restype = C.c_int64
+_lib.rsgpu_encode_stream.argtypes = [_vp, _sz, _sz, _READ_CB, _WRITE_CB, _vp, _SOPT, C.POINTER(_StreamErr)]
+_lib.rsgpu_verify_stream.argtypes = [_vp, _sz, _READ_CB, _vp, _SOPT, _i32p, C.POINTER(_StreamErr)]
+_lib.rsgpu_reconstruct_stream_from.argtypes = [_vp, _sz, _sz, _u8p, _u8p, _READ_CB, _WRITE_CB, _vp, _SOPT,
                                                C.POINTER(_StreamErr)]
+# the library's own in-memory callbacks, as C function pointers
+_NATIVE_RD = _READ_CB(C.cast(_lib.rsgpu_mem_stream_read, _vp).value)
+_NATIVE_WR = _WRITE_CB(C.cast(_lib.rsgpu_mem_stream_write, _vp).value)
+
+
+def stream_options_default():
+    o = StreamOptions()
+    _lib.rsgpu_stream_options_default(C.byref(o))
+    return o
+
+
+class HostStream:
+    """A shard stream over host memory, the reference's io::Cursor over a fixed
+    slice: `buf` is a contiguous uint8 numpy array (or bytes / bytearray, read-only
+    for readers).  Reads take buf[pos:]; writes fill buf[pos:] and fail once it is
+    full.  `pos` advances as bytes move; value() is buf[:pos]."""
+
+    def __init__(self, buf, pos=0):
+        if not isinstance(buf, np.ndarray):
+            buf = np.frombuffer(buf, np.uint8)
+        if buf.dtype != np.uint8 or buf.ndim != 1 or not buf.flags.c_contiguous:
+            raise ValueError("HostStream needs a contiguous 1-D uint8 buffer")
+        self.buf, self.pos = buf, int(pos)
+
+    def value(self):
+        return self.buf[:self.pos]
+
+    # Python-callback form, for calls that mix HostStream with other streams
+    def readinto(self, b):
+        n = max(0, min(len(b), self.buf.size - self.pos))
+        b[:n] = self.buf[self.pos:self.pos + n].tobytes()
+        self.pos += n
+        return n
+
+    def write(self, b):
+        n = max(0, min(len(b), self.buf.size - self.pos))
+        if n == 0 and len(b):
+            raise OSError("failed to write whole buffer")
+        self.buf[self.pos:self.pos + n] = np.frombuffer(b, np.uint8, n)
+        self.pos += n
+        return n
+
+
+def _raise_stream(rc, err, exc, writer_base):
+    """StreamError (or RsError) for a stream call's status; exc maps (kind, local
+    index) to the exception a Python callback raised."""
+    if rc == 0:
+        return
+    if rc == 105:  # RSGPU_ERR_IO
+        kind = "read" if err.kind == 1 else "write"
+        idx = err.shard_index
+        local = idx - writer_base if kind == "write" else idx
+        e = exc.get((kind, local))
+        if e is None:
+            e = OSError(f"{kind} returned {err.io_error}")
+        raise StreamError(kind, idx, e)
+    if 1 <= rc <= 16:
+        raise StreamError("codec", err.shard_index, RsError(rc))
+    raise RsError(rc)
+
+
+class _NativeIO:
+    """The library's in-memory callbacks over HostStream buffers."""
+
+    def __init__(self, readers, writers):
+        self.readers, self.writers = readers, writers
+
+        def table(streams):
+            t = (_MemStream * max(1, len(streams)))()
+            for i, s in enumerate(streams):
+                if s is not None:
+                    t[i].data, t[i].len, t[i].pos = s.buf.ctypes.data, s.buf.size, s.pos
+            return t
+
+        self._r, self._w = table(readers), table(writers)
+        self._ms = _MemStreams(self._r, self._w)
+        self.user = C.cast(C.pointer(self._ms), _vp)
+        self.rd, self.wr = _NATIVE_RD, _NATIVE_WR
+
+    def finish(self):
+        for t, streams in ((self._r, self.readers), (self._w, self.writers)):
+            for i, s in enumerate(streams):
+                if s is not None:
+                    s.pos = t[i].pos
+
+    def raise_for(self, rc, err, writer_base=0):
+        _raise_stream(rc, err, {}, writer_base)
+
+
+def _stream_io(readers, writers):
+    live = [s for s in readers + writers if s is not None]
+    if live and all(isinstance(s, HostStream) for s in live):
+        return _NativeIO(readers, writers)
+    return _StreamIO(readers, writers)
 
 
 class _AppendWriter:
@@ -654,11 +785,14 @@ class _AppendWriter:
 
 
 class _StreamIO:
-    """ctypes callbacks over Python readers (readinto/read) and writers (write)."""
+    """ctypes callbacks over Python readers (readinto/read) and writers (write).
+    In parallel I/O mode the library calls them from its worker threads; ctypes
+    takes the GIL for each call."""
 
-    def __init__(self, readers, writers, writer_offset=0):
+    def __init__(self, readers, writers):
         self.readers, self.writers = readers, writers
         self.exc = {}
+        self.user = None
 
         def rd(_user, shard, buf, n):
             try:
@@ -687,20 +821,11 @@ class _StreamIO:
 
         self.rd, self.wr = _READ_CB(rd), _WRITE_CB(wr)
 
+    def finish(self):
+        pass
+
     def raise_for(self, rc, err, writer_base=0):
-        if rc == 0:
-            return
-        if rc == 105:  # RSGPU_ERR_IO
-            kind = "read" if err.kind == 1 else "write"
-            idx = err.shard_index
-            local = idx - writer_base if kind == "write" else idx
-            e = self.exc.get((kind, local))
-            if e is None:
-                e = OSError(f"{kind} returned {err.io_error}")
-            raise StreamError(kind, idx, e)
-        if 1 <= rc <= 16:
-            raise StreamError("codec", err.shard_index, RsError(rc))
-        raise RsError(rc)
+        _raise_stream(rc, err, self.exc, writer_base)
 
 
 _lib.rsgpu_variant_count.argtypes = []

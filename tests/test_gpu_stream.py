@@ -172,3 +172,132 @@ def test_stream_errors_match_reference():
     with pytest.raises(rsgpu.StreamError) as ei:
         rs.encode_stream([io.BytesIO(ref_data(100))] * 3, [io.BytesIO(), Full()])
     assert ei.value.kind == "write" and ei.value.shard_index == k + 1
+
+
+def _host_streams(arrs):
+    return [rsgpu.HostStream(a) for a in arrs]
+
+
+@pytest.mark.parametrize("io_mode", ["auto", "serial", "parallel"])
+@pytest.mark.parametrize("k,m,total,block", [(10, 4, (8 << 20) + 4097, 4 << 20), (3, 2, 10 * 1024 + 5, 4096),
+                                             (12, 4, 3 << 20, 1 << 20)])
+def test_native_memory_streams_match_oracle(io_mode, k, m, total, block):
+    """HostStream readers/writers run the library's C callbacks: parity, verify and
+    a 2-erasure rebuild equal the oracle's bytes in every I/O mode."""
+    announce(f"stream native {io_mode} {k}+{m} total={total} block={block}")
+    rng = np.random.default_rng(total + k + len(io_mode))
+    data = [rng.integers(0, 256, total, dtype=np.uint8) for _ in range(k)]
+    parity = [np.zeros(total, np.uint8) for _ in range(m)]
+    rs = rsgpu.ReedSolomon(k, m)
+    readers, writers = _host_streams(data), _host_streams(parity)
+    rs.encode_stream(readers, writers, block_size=block, io_mode=io_mode)
+    exp = oracle_parity(k, m, [d.tobytes() for d in data])
+    assert all(r.pos == total for r in readers) and all(w.pos == total for w in writers)
+    for j in range(m):
+        assert parity[j].tobytes() == exp[j], j
+    full = data + parity
+    assert rs.verify_stream(_host_streams(full), block_size=block, io_mode=io_mode)
+    lost = (1, k + 1)
+    out = {i: np.zeros(total, np.uint8) for i in lost}
+    rd = [None if i in lost else rsgpu.HostStream(full[i]) for i in range(k + m)]
+    wr = [rsgpu.HostStream(out[i]) if i in lost else None for i in range(k + m)]
+    rs.reconstruct_stream_from(rd, wr, block_size=block, io_mode=io_mode)
+    for i in lost:
+        assert (out[i] == full[i]).all(), i
+    bad = [f.copy() for f in full]
+    bad[k][total - 1] ^= 1
+    assert not rs.verify_stream(_host_streams(bad), block_size=block, io_mode=io_mode)
+
+
+@pytest.mark.parametrize("io_mode", ["serial", "parallel"])
+def test_python_streams_in_every_io_mode(io_mode):
+    """Python callbacks run on the library's worker threads in parallel mode."""
+    announce(f"stream python {io_mode}")
+    k, m, n = 10, 4, (4 << 20) * 2 + 100
+    rng = np.random.default_rng(5)
+    data = [rng.integers(0, 256, n, dtype=np.uint8).tobytes() for _ in range(k)]
+    rs = rsgpu.ReedSolomon(k, m)
+    writers = [io.BytesIO() for _ in range(m)]
+    rs.encode_stream([io.BytesIO(d) for d in data], writers, io_mode=io_mode)
+    assert [w.getvalue() for w in writers] == oracle_parity(k, m, data)
+
+
+def test_parallel_errors_report_the_lowest_shard():
+    announce("stream parallel errors")
+    k, m = 10, 4
+    rs = rsgpu.ReedSolomon(k, m)
+
+    class Fail:
+        def readinto(self, b):
+            raise IOError("bad sector")
+
+    readers = [io.BytesIO(ref_data(5000)) for _ in range(k)]
+    readers[3], readers[7] = Fail(), Fail()
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.encode_stream(readers, [io.BytesIO() for _ in range(m)], block_size=1024, io_mode="parallel")
+    assert ei.value.kind == "read" and ei.value.shard_index == 3
+    # fixed-size writers: the third (1000 bytes) is full within block 0 -> a write error at k + 2
+    data = [np.frombuffer(ref_data(3000), np.uint8).copy() for _ in range(k)]
+    outs = [np.zeros(3000, np.uint8), np.zeros(2000, np.uint8), np.zeros(1000, np.uint8),
+            np.zeros(3000, np.uint8)]
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.encode_stream(_host_streams(data), _host_streams(outs), block_size=1024, io_mode="parallel")
+    assert ei.value.kind == "write" and ei.value.shard_index == k + 2
+    with pytest.raises(rsgpu.RsError):
+        rs.encode_stream(_host_streams(data), _host_streams(outs), io_mode=3)
+
+
+def test_reader_positions_after_errors():
+    """The documented positions: after a write error in block i or a failing verify
+    of block i, readers have consumed block i+1 (the read-ahead); on success they
+    are at the end; the outputs of earlier blocks are always written."""
+    announce("stream reader positions")
+    k, m, blk = 3, 2, 1024
+    rs = rsgpu.ReedSolomon(k, m)
+    data = [np.frombuffer(ref_data(4 * blk), np.uint8).copy() for _ in range(k)]
+    outs = [np.zeros(4 * blk, np.uint8), np.zeros(blk + 10, np.uint8)]
+    readers = _host_streams(data)
+    with pytest.raises(rsgpu.StreamError) as ei:
+        rs.encode_stream(readers, _host_streams(outs), block_size=blk, io_mode="serial")
+    assert ei.value.kind == "write" and ei.value.shard_index == k + 1
+    assert [r.pos for r in readers] == [3 * blk] * k  # blocks 0..2: the write of block 1 failed
+    exp = oracle_parity(k, m, [d.tobytes() for d in data])
+    assert outs[0][:blk].tobytes() == exp[0][:blk]  # block 0 is written everywhere
+    par = oracle_parity(k, m, [d.tobytes() for d in data])
+    full = data + [np.frombuffer(p, np.uint8).copy() for p in par]
+    full[0][10] ^= 1  # block 0 is inconsistent
+    readers = _host_streams(full)
+    assert not rs.verify_stream(readers, block_size=blk, io_mode="serial")
+    assert [r.pos for r in readers] == [2 * blk] * (k + m)
+    full[0][10] ^= 1
+    readers = _host_streams(full)
+    assert rs.verify_stream(readers, block_size=blk)
+    assert [r.pos for r in readers] == [4 * blk] * (k + m)
+
+
+def test_staging_is_reused_across_sizes_and_nested_calls():
+    """Calls on one thread reuse (and grow) its staging; a reader that itself runs
+    a stream call gets staging of its own."""
+    announce("stream staging reuse")
+    k, m = 4, 2
+    rs = rsgpu.ReedSolomon(k, m)
+    rng = np.random.default_rng(11)
+    for total, blk in ((5000, 1024), ((16 << 20) + 3, 16 << 20), (70000, 4096)):
+        data = [rng.integers(0, 256, total, dtype=np.uint8) for _ in range(k)]
+        parity = [np.zeros(total, np.uint8) for _ in range(m)]
+        rs.encode_stream(_host_streams(data), _host_streams(parity), block_size=blk)
+        assert [p.tobytes() for p in parity] == oracle_parity(k, m, [d.tobytes() for d in data])
+
+    inner_data = [np.frombuffer(ref_data(3000), np.uint8).copy() for _ in range(k)]
+    inner_out = [np.zeros(3000, np.uint8) for _ in range(m)]
+
+    class Nested(io.BytesIO):
+        def readinto(self, b):
+            rs.encode_stream(_host_streams(inner_data), _host_streams(inner_out), block_size=1024)
+            return super().readinto(b)
+
+    data = [rng.integers(0, 256, 9000, dtype=np.uint8).tobytes() for _ in range(k)]
+    writers = [io.BytesIO() for _ in range(m)]
+    rs.encode_stream([Nested(data[0])] + [io.BytesIO(d) for d in data[1:]], writers, block_size=2048)
+    assert [w.getvalue() for w in writers] == oracle_parity(k, m, data)
+    assert [o.tobytes() for o in inner_out] == oracle_parity(k, m, [d.tobytes() for d in inner_data])
