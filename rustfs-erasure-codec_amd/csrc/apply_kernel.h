@@ -16,9 +16,23 @@
 // workgroups for short shards (each re-reads the inputs, from L2), large NV and
 // R = r give the fewest passes over long shards.
 #pragma once
+#include <algorithm>
+
 #include "kernel_common.h"
 
 namespace rsgpu {
+
+// s_waitcnt immediate for vmcnt(16) with expcnt and lgkmcnt left open (gfx9
+// encoding: vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+constexpr unsigned kWaitVm16 = (1u << 14) | (7u << 4) | (15u << 8);
+
+// LDS bytes of one launch: the 16-byte table chunks of every (input, output)
+// pair, rounded up to whole staging passes of LB chunks.
+template <int LB, int R, bool TOWER>
+size_t apply_lds_bytes(int k) {
+  const size_t chunks = (size_t)k * R * (TOWER ? 4 * kTabWords : kTabWords) / 4;
+  return (chunks + LB - 1) / LB * LB * 16;
+}
 
 template <int LB, int R, int NV, int MODE, bool TOWER>
 __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
@@ -26,110 +40,185 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   // Inputs per load group: every lane issues KB*NV 128-bit loads back to back
   // before the arithmetic (64 VGPRs of input in flight for every NV).
   constexpr int KB = NV == 1 ? 16 : (NV == 2 ? 8 : 4);
+  static_assert(KB * NV == 16, "the staging wait counts 16 shard loads");
+  // [input][output][W]: the coefficient tables of this workgroup's outputs, the
+  // same block layout as in global memory.  Sized by apply_lds_bytes.
+  extern __shared__ __align__(16) uint32_t s_tab[];
   const int r0 = blockIdx.y * R;  // first output of this workgroup
+  const int rn = a.r - r0 < R ? a.r - r0 : R;
   const uint64_t tile = (uint64_t)LB * NV;
-  const uint64_t n_tiles = (a.n16 + tile - 1) / tile;
-  for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    uint64_t v[NV];
-    bool ok[NV];
-#pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      v[j] = t * tile + (uint64_t)j * LB + threadIdx.x;
-      ok[j] = v[j] < a.n16;
-      v[j] = (a.v_begin + v[j]) * 16;  // byte offset
-    }
-    uint32_t acc[R][NV][4];
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-      for (int j = 0; j < NV; ++j)
-#pragma unroll
-        for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
 
-    for (int i0 = 0; i0 < a.k; i0 += KB) {
-      uint4 x[KB][NV];
+  // Table staging: 16-byte chunks go straight from global memory into LDS
+  // (global_load_lds, no registers), issued before the shard loads so that both
+  // round trips overlap.  Lane t of a pass fills chunk c0 + t; lanes past the
+  // last chunk re-read it into their own (unused) slot and rows past the last
+  // output re-read row rn - 1, so every address stays inside the tables.
+  {
+    constexpr int CW = W / 4;  // chunks per coefficient table
+    const int n_chunks = a.k * R * CW;
+    const int wave = (int)threadIdx.x / 64;
+    for (int c0 = 0; c0 < n_chunks; c0 += LB) {
+      int c = c0 + (int)threadIdx.x;
+      c = c < n_chunks ? c : n_chunks - 1;
+      const int blk = c / CW, part = c % CW, i = blk / R, r = blk % R;
+      const int rr = r < rn ? r : rn - 1;
+      const uint32_t* src =
+          a.tables + (size_t)(r0 + rr) * a.tab_row_stride + (size_t)(a.tab_col0 + i) * W + part * 4;
+      __builtin_amdgcn_global_load_lds(src, s_tab + (size_t)(c0 + wave * 64) * 4, 16, 0, 0);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+
+  // One tile per workgroup: the launch covers every tile (blockIdx.x, offset by
+  // v_begin for very long shards).
+  uint64_t v[NV];
+  bool ok[NV];
 #pragma unroll
-      for (int b = 0; b < KB; ++b)
+  for (int j = 0; j < NV; ++j) {
+    const uint64_t u = (uint64_t)blockIdx.x * tile + (uint64_t)j * LB + threadIdx.x;
+    ok[j] = u < a.n16;
+    // a lane past the end loads the last vector instead (in bounds, unused), so
+    // the loads need no per-lane branch
+    v[j] = (a.v_begin + (ok[j] ? u : a.n16 - 1)) * 16;  // byte offset
+  }
+  uint32_t acc[R][NV][4];
 #pragma unroll
-        for (int j = 0; j < NV; ++j)
-          x[b][j] = (i0 + b < a.k && ok[j]) ? load16(a.in[i0 + b] + v[j]) : make_uint4(0, 0, 0, 0);
+  for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int b = 0; b < KB; ++b) {
-        if (i0 + b >= a.k) continue;  // uniform; `continue` keeps the loop fully unrollable
-        ctab_t tcol = as_const(a.tables) + (size_t)(a.tab_col0 + i0 + b) * W +
-                      (size_t)r0 * a.tab_row_stride;
+    for (int j = 0; j < NV; ++j)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
-          uint32_t w[4];
-          if constexpr (TOWER) {
-            deinterleave(x[b][j], w);
-          } else {
-            w[0] = x[b][j].x;
-            w[1] = x[b][j].y;
-            w[2] = x[b][j].z;
-            w[3] = x[b][j].w;
-          }
-          Sel s[4];
+      for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
+
+  uint4 x[KB][NV];
+  // Issues the loads of inputs [i0, i0 + KB): the pointers first (kernel-argument
+  // reads, batched), then the shard vectors in input order.  Slots past the last
+  // input re-read the last input (the same lines: cache hits, never used), so
+  // every load is unconditional, the compiler counts them exactly, and input
+  // b's arithmetic waits for its own loads only while the later ones fly.
+  auto load_group = [&](int i0) __attribute__((always_inline)) {
+    const uint8_t* p[KB];
+    const uint8_t* last = a.in[a.k - 1];
 #pragma unroll
-          for (int d = 0; d < 4; ++d) s[d] = selectors(w[d]);
+    for (int b = 0; b < KB; ++b) p[b] = a.in[i0 + b];  // i0 + b < kMaxIn: inside ApplyArgs::in
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            if (r0 + r >= a.r) continue;  // last row group of an uneven split
-            ctab_t tc = tcol + (size_t)r * a.tab_row_stride;
-            if constexpr (TOWER) {
-              // [A B; C D] per tower_coefficients: x' = A x ^ B c, c' = C x ^ D c
+    for (int b = 0; b < KB; ++b) p[b] = i0 + b < a.k ? p[b] : last;
 #pragma unroll
-              for (int d = 0; d < 2; ++d) {
-                acc[r][j][d] ^= gmul(tc + 0 * kTabWords, s[d]) ^ gmul(tc + 1 * kTabWords, s[d + 2]);
-                acc[r][j][d + 2] ^= gmul(tc + 2 * kTabWords, s[d]) ^ gmul(tc + 3 * kTabWords, s[d + 2]);
-              }
-            } else {
+    for (int b = 0; b < KB; ++b) {
 #pragma unroll
-              for (int d = 0; d < 4; ++d) acc[r][j][d] ^= gmul(tc, s[d]);
-            }
-          }
+      for (int j = 0; j < NV; ++j) x[b][j] = load16(p[b] + v[j]);
+      __builtin_amdgcn_sched_barrier(0);  // keep the issue order
+    }
+  };
+  // Input b's tables for all R outputs are read from LDS first (16-byte reads),
+  // so their latency overlaps the selector arithmetic.  Rows past the last
+  // output (an uneven row split) hold copies of row rn - 1: computed, never
+  // stored, and no branch splits the outputs.
+  constexpr int NT = TOWER ? 4 : 1;  // byte maps per coefficient
+  auto compute_group = [&](int i0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+      if (i0 + b >= a.k) continue;  // uniform; `continue` keeps the loop fully unrollable
+      const uint32_t* tcol = s_tab + (size_t)(i0 + b) * R * W;
+      uint4 tq[R][NT];
+      uint32_t t4[R][NT];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+          const uint32_t* t = tcol + r * W + q * kTabWords;
+          tq[r][q] = *reinterpret_cast<const uint4*>(t);
+          t4[r][q] = t[4];
         }
-      }
-    }
-
-    bool mismatch = false;
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-      if (!ok[j]) continue;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (r0 + r >= a.r) continue;
-        uint4 o;
+      for (int j = 0; j < NV; ++j) {
+        uint32_t w[4];
         if constexpr (TOWER) {
-          o = interleave(acc[r][j]);
+          deinterleave(x[b][j], w);
         } else {
-          o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
+          w[0] = x[b][j].x;
+          w[1] = x[b][j].y;
+          w[2] = x[b][j].z;
+          w[3] = x[b][j].w;
         }
-        uint8_t* dst = a.out[r0 + r] + v[j];
-        if constexpr (MODE == kWrite) {
-          store16(dst, o);
-        } else if constexpr (MODE == kAccumulate) {
-          uint4 p = load16(dst);
-          store16(dst, make_uint4(p.x ^ o.x, p.y ^ o.y, p.z ^ o.z, p.w ^ o.w));
-        } else {
-          uint4 p = load16(dst);
-          mismatch |= ((p.x ^ o.x) | (p.y ^ o.y) | (p.z ^ o.z) | (p.w ^ o.w)) != 0;
+        Sel s[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) s[d] = selectors(w[d]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if constexpr (TOWER) {
+            // [A B; C D] per tower_coefficients: x' = A x ^ B c, c' = C x ^ D c
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+              acc[r][j][d] ^= gmul(tq[r][0], t4[r][0], s[d]) ^ gmul(tq[r][1], t4[r][1], s[d + 2]);
+              acc[r][j][d + 2] ^= gmul(tq[r][2], t4[r][2], s[d]) ^ gmul(tq[r][3], t4[r][3], s[d + 2]);
+            }
+          } else {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) acc[r][j][d] ^= gmul(tq[r][0], t4[r][0], s[d]);
+          }
         }
       }
     }
-    if constexpr (MODE == kVerify) {
-      if (mismatch) atomicOr(a.flag, 1);
+  };
+
+  load_group(0);
+  // The staging loads are older than the 16 shard loads just issued: once at
+  // most 16 loads are outstanding they have landed in LDS.  The barrier then
+  // publishes every wave's chunks.
+  __builtin_amdgcn_s_waitcnt(kWaitVm16);
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+  compute_group(0);
+  for (int i0 = KB; i0 < a.k; i0 += KB) {
+    load_group(i0);
+    compute_group(i0);
+  }
+
+  bool mismatch = false;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    if (!ok[j]) continue;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r >= rn) continue;
+      uint4 o;
+      if constexpr (TOWER) {
+        o = interleave(acc[r][j]);
+      } else {
+        o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
+      }
+      uint8_t* dst = a.out[r0 + r] + v[j];
+      if constexpr (MODE == kWrite) {
+        store16(dst, o);
+      } else if constexpr (MODE == kAccumulate) {
+        uint4 p = load16(dst);
+        store16(dst, make_uint4(p.x ^ o.x, p.y ^ o.y, p.z ^ o.z, p.w ^ o.w));
+      } else {
+        uint4 p = load16(dst);
+        mismatch |= ((p.x ^ o.x) | (p.y ^ o.y) | (p.z ^ o.z) | (p.w ^ o.w)) != 0;
+      }
     }
+  }
+  if constexpr (MODE == kVerify) {
+    if (mismatch) atomicOr(a.flag, 1);
   }
 }
 
 template <int LB, int R, int NV, int MODE, bool TOWER>
 static hipError_t launch_apply_t(const ApplyArgs& a, hipStream_t s) {
+  // one workgroup per tile; launches of at most 2^30 tiles, each shifted by v_begin
+  constexpr uint64_t kMaxTiles = 1ull << 30;
   const uint64_t tile = (uint64_t)LB * NV;
-  uint64_t tiles = (a.n16 + tile - 1) / tile;
-  unsigned gx = (unsigned)(tiles < 1048576 ? (tiles ? tiles : 1) : 1048576);
-  unsigned gy = (unsigned)((a.r + R - 1) / R);
-  hipLaunchKernelGGL((apply_kernel<LB, R, NV, MODE, TOWER>), dim3(gx, gy), dim3(LB), 0, s, a);
+  const uint64_t tiles = (a.n16 + tile - 1) / tile;
+  const unsigned gy = (unsigned)((a.r + R - 1) / R);
+  const size_t lds = apply_lds_bytes<LB, R, TOWER>(a.k);
+  for (uint64_t t0 = 0; t0 < tiles; t0 += kMaxTiles) {
+    ApplyArgs part = a;
+    part.v_begin = a.v_begin + t0 * tile;
+    part.n16 = std::min<uint64_t>(a.n16 - t0 * tile, kMaxTiles * tile);
+    const unsigned gx = (unsigned)std::min<uint64_t>(tiles - t0, kMaxTiles);
+    hipLaunchKernelGGL((apply_kernel<LB, R, NV, MODE, TOWER>), dim3(gx, gy), dim3(LB), lds, s, part);
+  }
   return hipGetLastError();
 }
 

@@ -74,7 +74,10 @@ static const ApplyFn kApplyFns[2][kNumTiles] = {
 
 hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s) {
   if (a.n16 == 0) return hipSuccess;
-  if (tile < 0 || tile >= kNumTiles || a.r < 1 || a.r > kMaxOut) return hipErrorInvalidValue;
+  if (tile < 0 || tile >= kNumTiles || a.r < 1 || a.r > kMaxOut || a.k < 1 || a.k > kMaxIn)
+    return hipErrorInvalidValue;
+  // the kernel stages its tables into LDS with 16-byte direct loads
+  if (((uintptr_t)a.tables & 15) != 0 || (a.tab_row_stride & 3) != 0) return hipErrorInvalidValue;
   if (rows < 1 || rows > a.r) rows = a.r;
   return kApplyFns[tower ? 1 : 0][tile](a, mode, rows, s);
 }
