@@ -17,6 +17,7 @@
 // R = r give the fewest passes over long shards.
 #pragma once
 #include <algorithm>
+#include <cstddef>
 
 #include "kernel_common.h"
 
@@ -56,6 +57,10 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   {
     constexpr int CW = W / 4;  // chunks per coefficient table
     const int n_chunks = a.k * R * CW;
+    // tables in device memory, or inline in this launch's kernel arguments
+    const uint32_t* tab = a.tables ? a.tables
+                                   : (const uint32_t*)((const char*)__builtin_amdgcn_kernarg_segment_ptr() +
+                                                       offsetof(ApplyArgs, tab_inline));
     const int wave = (int)threadIdx.x / 64;
     for (int c0 = 0; c0 < n_chunks; c0 += LB) {
       int c = c0 + (int)threadIdx.x;
@@ -63,7 +68,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
       const int blk = c / CW, part = c % CW, i = blk / R, r = blk % R;
       const int rr = r < rn ? r : rn - 1;
       const uint32_t* src =
-          a.tables + (size_t)(r0 + rr) * a.tab_row_stride + (size_t)(a.tab_col0 + i) * W + part * 4;
+          tab + (size_t)(r0 + rr) * a.tab_row_stride + (size_t)(a.tab_col0 + i) * W + part * 4;
       __builtin_amdgcn_global_load_lds(src, s_tab + (size_t)(c0 + wave * 64) * 4, 16, 0, 0);
     }
   }

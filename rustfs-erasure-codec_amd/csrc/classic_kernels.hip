@@ -482,6 +482,28 @@ hipError_t launch_store_u16(uint16_t* dst, const uint16_t* vals, size_t n, hipSt
   return hipSuccess;
 }
 
+struct U32Chunk {
+  int32_t n;
+  uint32_t v[512];
+};
+
+__global__ void store_u32_kernel(uint32_t* dst, const U32Chunk c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c.n) dst[i] = c.v[i];
+}
+
+hipError_t launch_store_u32(uint32_t* dst, const uint32_t* vals, size_t n, hipStream_t s) {
+  for (size_t i0 = 0; i0 < n; i0 += 512) {
+    U32Chunk c{};
+    c.n = (int32_t)std::min<size_t>(512, n - i0);
+    for (int i = 0; i < c.n; ++i) c.v[i] = vals[i0 + i];
+    hipLaunchKernelGGL(store_u32_kernel, dim3((c.n + 255) / 256), dim3(256), 0, s, dst + i0, c);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 struct DecodeTargets {
   int32_t t0, n;                     // first output row of this launch, rows in it
   uint16_t target[kBigTargetChunk];  // shard index of each row

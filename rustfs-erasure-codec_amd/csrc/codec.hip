@@ -4,6 +4,7 @@
 // launch plumbing for the matrix-apply kernels.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -270,10 +271,25 @@ static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 // `col0` the table column of in[0], so a subset of a wider table can be applied.
 // mode kAccumulate XORs into the outputs from the first input on.
 // Takes plain arrays of k_total inputs and r_total outputs and allocates nothing.
+// True when one launch of the vector kernel covers the whole call with its
+// tables inline in the kernel arguments: 16-byte aligned buffers and length, at
+// most kMaxIn inputs and kMaxOut outputs, and r x k tables within kInlineTabWords.
+static bool apply_inline_fits(int k_total, int r_total, bool tower, const uint8_t* const* in,
+                              uint8_t* const* out, size_t len) {
+  if (len % 16 || k_total > kMaxIn || r_total > kMaxOut ||
+      (size_t)k_total * r_total * words_per_coef(tower) > (size_t)kInlineTabWords)
+    return false;
+  for (int i = 0; i < k_total; ++i)
+    if (!aligned16(in[i])) return false;
+  for (int j = 0; j < r_total; ++j)
+    if (!aligned16(out[j])) return false;
+  return true;
+}
+
 static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
                      const uint8_t* const* in, uint8_t* const* out,
                      size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1,
-                     int table_cols = -1, int col0 = 0) {
+                     int table_cols = -1, int col0 = 0, const uint32_t* h_tables = nullptr) {
   if (table_cols < 0) table_cols = k_total;
   const int W = words_per_coef(tower);
   bool vec_ok = true;
@@ -282,6 +298,9 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
   if (mode == kVerify && k_total > kMaxIn) return RSGPU_ERR_UNSUPPORTED;  // handled by caller
   const uint64_t n16 = vec_ok ? len / 16 : 0;
   const size_t tail_begin = (size_t)n16 * 16;
+  // host tables (same layout as d_tables) travel inline when one launch covers the call
+  const bool inline_tabs = h_tables && apply_inline_fits(k_total, r_total, tower, in, out, len);
+  if (!inline_tabs && !d_tables) return RSGPU_ERR_UNSUPPORTED;
   // Default tiling: 256-lane workgroups with every output, two vectors per lane
   // only when that still leaves >= 8 workgroups per CU.
   if (variant < 0 || variant >= kNumApplyVariants) variant = n16 >= (uint64_t)kBlock * 2 * 2048 ? 1 : 0;
@@ -292,17 +311,27 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
       const int kn = std::min(kMaxIn, k_total - k0);
       const int m = (mode == kVerify) ? kVerify : (k0 == 0 && mode == kWrite ? kWrite : kAccumulate);
       if (n16) {
-        ApplyArgs a{};
+        ApplyArgs a;
+        std::memset(&a, 0, offsetof(ApplyArgs, tab_inline));
         for (int i = 0; i < kn; ++i) a.in[i] = in[k0 + i];
         for (int j = 0; j < rn; ++j) a.out[j] = out[r0 + j];
-        a.tables = d_tables + (size_t)r0 * table_cols * W;
         a.flag = flag;
         a.n16 = n16;
         a.v_begin = 0;
         a.k = kn;
-        a.tab_row_stride = table_cols * W;
-        a.tab_col0 = col0 + k0;
         a.r = rn;
+        if (inline_tabs) {  // this launch's rn x kn block, rows kn * W apart
+          for (int j = 0; j < rn; ++j)
+            std::memcpy(a.tab_inline + (size_t)j * kn * W,
+                        h_tables + ((size_t)(r0 + j) * table_cols + col0 + k0) * W, (size_t)kn * W * 4);
+          a.tables = nullptr;
+          a.tab_row_stride = kn * W;
+          a.tab_col0 = 0;
+        } else {
+          a.tables = d_tables + (size_t)r0 * table_cols * W;
+          a.tab_row_stride = table_cols * W;
+          a.tab_col0 = col0 + k0;
+        }
         HIP_TRY(launch_apply(a, m, tile, split ? std::min(split, rn) : rn, tower ? 1 : 0, s));
       }
       if (tail_begin < len) {
@@ -747,27 +776,118 @@ static int new_decode_entry(const Codec& c, const Plan& plan, const std::string&
 // Orders a read of `e` on stream s after its solve when the solve was enqueued
 // on another stream and has not completed yet.
 static int wait_ready(const DecodeEntry& e, hipStream_t s) {
-  if (e.solve_stream == s || hipEventQuery(e.slot.ready) == hipSuccess) return RSGPU_OK;
+  if (!e.slot.ready || e.solve_stream == s || hipEventQuery(e.slot.ready) == hipSuccess) return RSGPU_OK;
   HIP_TRY(hipStreamWaitEvent(s, e.slot.ready, 0));
   return RSGPU_OK;
 }
 
+// Patterns whose decode rows are solved on the host and whose tables travel in
+// the kernel arguments: the library's own byte fields (no Leopard generator-
+// matrix field), and targets x k tables within kInlineTabWords.
+static bool host_solve_eligible(const Codec& c, size_t n_targets) {
+  return c.family == RSGPU_FAMILY_CLASSIC && !c.ftab && c.k <= (size_t)kMaxIn &&
+         n_targets <= (size_t)kMaxOut &&
+         n_targets * c.k * (size_t)words_per_coef(c.field == RSGPU_FIELD_GF16) <= (size_t)kInlineTabWords;
+}
+
+// The decode rows of decode_rows_body on the host (the same solve: [C | B | I]
+// Gauss-Jordan over the first k survivors, recovery_plan.rs:329-387, composed
+// parity rows per recovery_plan.rs:483-515), then their v_perm tables.  The rows
+// are unique for an invertible C, so they equal the device solve's bytes.
+static bool host_decode_tables(const Codec& c, const Plan& plan, std::vector<uint32_t>& tables) {
+  const size_t k = c.k, nt = plan.targets.size();
+  const Field F{c.field == RSGPU_FIELD_GF16};
+  size_t p = 0;  // present data among the survivors (a sorted prefix of valid)
+  while (p < k && plan.valid[p] < (int)k) ++p;
+  const size_t e = k - p, cols = e + p + e;
+  std::vector<int> missing;
+  for (size_t d = 0, kn = 0; d < k && missing.size() < e; ++d) {
+    if (kn < p && plan.valid[kn] == (int)d) ++kn;
+    else missing.push_back((int)d);
+  }
+  std::vector<uint16_t> wm(e * cols, 0);
+  for (size_t r = 0; r < e; ++r) {
+    const uint16_t* prow = c.matrix.data() + (size_t)plan.valid[p + r] * k;
+    for (size_t j = 0; j < e; ++j) wm[r * cols + j] = prow[missing[j]];
+    for (size_t j = 0; j < p; ++j) wm[r * cols + e + j] = prow[plan.valid[j]];
+    wm[r * cols + e + p + r] = 1;
+  }
+  if (e && !gauss_jordan(F, wm.data(), (int)e, (int)cols)) return false;
+  std::vector<uint16_t> rows(nt * k, 0);
+  for (size_t t = 0; t < nt; ++t) {
+    const int target = plan.targets[t];
+    uint16_t* row = rows.data() + t * k;
+    if (target < (int)k) {
+      const size_t j = std::find(missing.begin(), missing.end(), target) - missing.begin();
+      for (size_t col = 0; col < k; ++col)
+        row[col] = j < e ? wm[j * cols + e + col] : (plan.valid[col] == target ? 1 : 0);
+    } else {
+      const uint16_t* prow = c.matrix.data() + (size_t)target * k;
+      for (size_t col = 0; col < k; ++col) {
+        uint16_t v = col < p ? prow[plan.valid[col]] : 0;
+        for (size_t j = 0; j < e; ++j) v ^= F.mul(prow[missing[j]], wm[j * cols + e + col]);
+        row[col] = v;
+      }
+    }
+  }
+  build_tables_host(rows.data(), (int)nt, (int)k, c.field == RSGPU_FIELD_GF16, tables, nullptr);
+  return true;
+}
+
+// A device copy of a host-solved entry's tables, made on first need (a batch,
+// an unaligned or wide call): a pool slot filled by a kernel that carries the
+// words in its arguments, ordered on s.
+static int ensure_device_tables(const Codec& c, DecodeEntry& e, hipStream_t s) {
+  std::lock_guard<std::mutex> g(e.dev_mu);
+  if (e.tables) return wait_ready(e, s);
+  {
+    std::lock_guard<std::mutex> cg(c.mu);
+    if (!c.decode_pool) {
+      c.decode_pool = std::make_shared<DecodePool>();
+      c.decode_pool->slot_bytes = SlotLayout(c).total;
+    }
+    e.pool = c.decode_pool;
+  }
+  HIP_TRY(e.pool->acquire(e.slot));
+  HIP_TRY(launch_store_u32((uint32_t*)e.slot.mem, e.h_tables.data(), e.h_tables.size(), s));
+  HIP_TRY(hipEventRecord(e.slot.ready, s));
+  e.solve_stream = s;
+  e.rows = nullptr;  // only the tables are copied
+  e.tables = (uint32_t*)e.slot.mem;
+  return RSGPU_OK;
+}
+
+// The cache entry of `plan`.  With inline_ok the caller can take host tables
+// (then a miss is solved on the host when eligible); otherwise the entry has
+// device tables on return, ordered before work on s.
 static int get_decode_entry(const Codec& c, const rsgpu_slot* shards, const Plan& plan,
-                            hipStream_t s, std::shared_ptr<DecodeEntry>& out) {
+                            hipStream_t s, std::shared_ptr<DecodeEntry>& out, bool inline_ok = false) {
   const std::string key = plan_key(c, shards, plan);
-  std::lock_guard<std::mutex> g(c.mu);
+  std::unique_lock<std::mutex> g(c.mu);
   ++c.requests;
   auto it = c.index.find(key);
   if (it != c.index.end()) {
     ++c.hits;
     c.lru.splice(c.lru.begin(), c.lru, it->second);
     out = it->second->second;
-    return wait_ready(*out, s);
+    g.unlock();
+    if (out->h_tables.empty()) return wait_ready(*out, s);  // a device solve's entry
+    return inline_ok ? RSGPU_OK : ensure_device_tables(c, *out, s);
   }
   ++c.misses;
   if (c.matrix_mode == RSGPU_MATRIX_CUSTOM && !custom_solvable(c, plan)) return RSGPU_INVALID_CUSTOM_MATRIX;
   const bool tower = c.field == RSGPU_FIELD_GF16;
   const size_t nt = plan.targets.size();
+  if (inline_ok && host_solve_eligible(c, nt)) {
+    auto h = std::make_shared<DecodeEntry>();
+    h->valid = plan.valid;
+    h->targets = plan.targets;
+    if (!host_decode_tables(c, plan, h->h_tables)) return RSGPU_INVALID_CUSTOM_MATRIX;
+    h->bytes = h->h_tables.size() * 4 + nt * c.k * (tower ? 2 : 1) + key.size();
+    out = h;
+    cache_insert(c, key, h);
+    return RSGPU_OK;
+  }
   std::shared_ptr<DecodeEntry> e;
   int rc = new_decode_entry(c, plan, key, s, e);
   if (rc) return rc;
@@ -928,14 +1048,28 @@ static int reconstruct_impl(const Codec& c, rsgpu_slot* shards, size_t n, int go
   Plan plan;
   int rc = make_plan(c, shards, n, goal, required, n_required, plan);
   if (rc || plan.all_present || plan.targets.empty()) return rc;
+  const size_t nt = plan.targets.size();
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  const uint8_t* in[kMaxIn];
+  uint8_t* out[kMaxOut];
+  std::vector<const uint8_t*> vin;
+  std::vector<uint8_t*> vout;
+  const uint8_t** pin = in;
+  uint8_t** pout = out;
+  if (c.k > (size_t)kMaxIn || nt > (size_t)kMaxOut) {  // wide codecs: heap lists
+    vin.resize(c.k);
+    vout.resize(nt);
+    pin = vin.data();
+    pout = vout.data();
+  }
+  for (size_t i = 0; i < c.k; ++i) pin[i] = shards[plan.valid[i]].ptr;
+  for (size_t t = 0; t < nt; ++t) pout[t] = shards[plan.targets[t]].ptr;
+  const bool inline_ok = apply_inline_fits((int)c.k, (int)nt, tower, pin, pout, plan.shard_len);
   std::shared_ptr<DecodeEntry> e;
-  if ((rc = get_decode_entry(c, shards, plan, s, e))) return rc;
-  std::vector<const uint8_t*> in(c.k);
-  std::vector<uint8_t*> out(plan.targets.size());
-  for (size_t i = 0; i < c.k; ++i) in[i] = shards[plan.valid[i]].ptr;
-  for (size_t t = 0; t < plan.targets.size(); ++t) out[t] = shards[plan.targets[t]].ptr;
-  rc = run_apply(e->tables, (int)c.k, (int)plan.targets.size(),
-                 c.field == RSGPU_FIELD_GF16, in, out, plan.shard_len, kWrite, nullptr, s, variant);
+  if ((rc = get_decode_entry(c, shards, plan, s, e, inline_ok))) return rc;
+  const bool inline_tabs = inline_ok && !e->h_tables.empty();
+  rc = run_apply(inline_tabs ? nullptr : e->tables, (int)c.k, (int)nt, tower, pin, pout, plan.shard_len, kWrite,
+                 nullptr, s, variant, -1, 0, inline_tabs ? e->h_tables.data() : nullptr);
   if (rc) return rc;
   // An entry evicted meanwhile retires its slot when `e` goes out of scope; the
   // slot is reused only after a device-wide synchronisation.
@@ -990,7 +1124,8 @@ static int prepare_workspace(const Codec& c, const rsgpu_slot* shards, size_t n,
   w->targets = plan.targets;
   // an empty request is a validated no-op without a solve (recovery_plan.rs:488-495)
   if (!plan.all_present && !plan.targets.empty())
-    if ((rc = get_decode_entry(c, shards, plan, s, w->entry))) return rc;
+    if ((rc = get_decode_entry(c, shards, plan, s, w->entry, host_solve_eligible(c, plan.targets.size()))))
+      return rc;
   *out = w.release();
   return RSGPU_OK;
 }
@@ -1045,10 +1180,15 @@ static int reconstruct_with_workspace_impl(const Codec& c, rsgpu_slot* shards, s
     if (!pin[i]) return RSGPU_ERR_INVALID_ARGUMENT;
   }
   for (size_t t = 0; t < nt; ++t) pout[t] = shards[w.targets[t]].ptr;
-  int rc = wait_ready(*w.entry, s);
+  const bool tower = c.field == RSGPU_FIELD_GF16;
+  const DecodeEntry& e = *w.entry;
+  const bool inline_tabs = !e.h_tables.empty() && apply_inline_fits((int)c.k, (int)nt, tower, pin, pout, len);
+  int rc = inline_tabs ? RSGPU_OK
+                       : (e.h_tables.empty() ? wait_ready(e, s) : ensure_device_tables(c, *w.entry, s));
   if (rc) return rc;
-  rc = run_apply(w.entry->tables, (int)c.k, (int)nt, c.field == RSGPU_FIELD_GF16, pin, pout, len, kWrite,
-                 nullptr, s, pick_variant(c, kOpReconstruct, total - c.k, len));
+  rc = run_apply(inline_tabs ? nullptr : e.tables, (int)c.k, (int)nt, tower, pin, pout, len, kWrite, nullptr, s,
+                 pick_variant(c, kOpReconstruct, total - c.k, len), -1, 0,
+                 inline_tabs ? e.h_tables.data() : nullptr);
   if (rc) return rc;
   for (int t : w.targets) shards[t].present = 1;
   return RSGPU_OK;

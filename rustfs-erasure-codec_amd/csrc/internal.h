@@ -28,10 +28,15 @@ constexpr int kBlock = 256;
 
 enum ApplyMode : int { kWrite = 0, kAccumulate = 1, kVerify = 2 };
 
+// Coefficient-table words that can travel inside ApplyArgs itself (2 KiB: 64
+// GF(2^8) coefficients, e.g. 10 inputs x 4 outputs), so a launch needs no table
+// in device memory.
+constexpr int kInlineTabWords = 512;
+
 struct ApplyArgs {
   const uint8_t* in[kMaxIn];
   uint8_t* out[kMaxOut];   // destinations, or the stored parity when verifying
-  const uint32_t* tables;  // row-major [row][input][kTabWords]
+  const uint32_t* tables;  // row-major [row][input][kTabWords]; nullptr: tab_inline
   int32_t* flag;           // verify: set nonzero on any mismatch
   uint64_t n16;            // 16-byte column vectors per shard
   uint64_t v_begin;        // first vector handled by this launch
@@ -39,6 +44,9 @@ struct ApplyArgs {
   int32_t tab_row_stride;  // dwords between consecutive table rows
   int32_t tab_col0;        // table column of in[0]
   int32_t r;               // outputs in this launch (<= kMaxOut)
+  // the tables when `tables` is null (same layout, rows tab_row_stride apart);
+  // the kernel reads them straight from its kernel-argument segment
+  uint32_t tab_inline[kInlineTabWords];
 };
 
 struct ByteArgs {
@@ -154,6 +162,8 @@ hipError_t launch_decode_big(const uint16_t* d_matrix, const DecodeBigJob& job,
 // Writes n 16-bit values from the host to device memory on stream s through
 // kernel arguments (no staging buffer, nothing to keep alive after the call).
 hipError_t launch_store_u16(uint16_t* dst, const uint16_t* vals, size_t n, hipStream_t s);
+// dst[0..n) = vals, the values carried in kernel arguments (no staging buffer)
+hipError_t launch_store_u32(uint32_t* dst, const uint32_t* vals, size_t n, hipStream_t s);
 
 // Calls fn(0..n-1) on the library's persistent host worker threads and the
 // calling thread; returns when every call has finished.
@@ -244,8 +254,12 @@ struct DecodeEntry {
   std::shared_ptr<DecodePool> pool;
   DecodePool::Slot slot{nullptr, nullptr};
   std::unique_ptr<DeviceBuffer> own;  // set when the entry does not use a pool slot
-  uint32_t* tables = nullptr;
+  uint32_t* tables = nullptr;  // device tables; null while only h_tables exist
   uint8_t* rows = nullptr;
+  // [targets][k][words] solved on the host, for patterns whose tables travel in
+  // the kernel arguments (empty otherwise)
+  std::vector<uint32_t> h_tables;
+  std::mutex dev_mu;  // guards the device copy made for such an entry on demand
   hipStream_t solve_stream = nullptr;
   std::vector<int> valid;  // survivor indices
   std::vector<int> targets;
