@@ -209,6 +209,127 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   }
 }
 
+// The default GF(2^8) tiling: one dword of every shard per lane.  The same
+// arithmetic as the 16-byte tiles, spread over four times the waves, so the
+// memory, LDS and VALU latencies of one wave hide behind the others' work.
+__device__ __forceinline__ uint32_t load4(const uint8_t* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+}
+__device__ __forceinline__ void store4(uint8_t* p, uint32_t v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<uint32_t*>(p));
+}
+
+constexpr int kDwordLanes = 256;
+
+template <int R, int MODE>
+__global__ __launch_bounds__(kDwordLanes) void apply_dword_kernel(const ApplyArgs a) {
+  constexpr int LB = kDwordLanes;
+  constexpr int W = kTabWords;
+  constexpr int KB = 16;  // inputs per load group: the staging wait counts 16 loads
+  extern __shared__ __align__(16) uint32_t s_tab[];  // [input][output][W]
+  const int r0 = blockIdx.y * R;
+  const int rn = a.r - r0 < R ? a.r - r0 : R;
+  // table staging, as in apply_kernel
+  {
+    constexpr int CW = W / 4;
+    const int n_chunks = a.k * R * CW;
+    const uint32_t* tab = a.tables ? a.tables
+                                   : (const uint32_t*)((const char*)__builtin_amdgcn_kernarg_segment_ptr() +
+                                                       offsetof(ApplyArgs, tab_inline));
+    const int wave = (int)threadIdx.x / 64;
+    for (int c0 = 0; c0 < n_chunks; c0 += LB) {
+      int c = c0 + (int)threadIdx.x;
+      c = c < n_chunks ? c : n_chunks - 1;
+      const int blk = c / CW, part = c % CW, i = blk / R, r = blk % R;
+      const int rr = r < rn ? r : rn - 1;
+      const uint32_t* src = tab + (size_t)(r0 + rr) * a.tab_row_stride + (size_t)(a.tab_col0 + i) * W + part * 4;
+      __builtin_amdgcn_global_load_lds(src, s_tab + (size_t)(c0 + wave * 64) * 4, 16, 0, 0);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  const uint64_t n4 = a.n16 * 4;  // dwords per shard
+  const uint64_t u = (uint64_t)blockIdx.x * LB + threadIdx.x;
+  const bool ok = u < n4;
+  const uint64_t off = (a.v_begin * 4 + (ok ? u : n4 - 1)) * 4;  // past the end: the last dword
+  uint32_t acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0;
+  uint32_t x[KB];
+  auto load_group = [&](int i0) __attribute__((always_inline)) {
+    const uint8_t* p[KB];
+    const uint8_t* last = a.in[a.k - 1];
+#pragma unroll
+    for (int b = 0; b < KB; ++b) p[b] = a.in[i0 + b];  // i0 + b < kMaxIn: inside ApplyArgs::in
+#pragma unroll
+    for (int b = 0; b < KB; ++b) p[b] = i0 + b < a.k ? p[b] : last;
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+      x[b] = load4(p[b] + off);
+      __builtin_amdgcn_sched_barrier(0);  // keep the issue order
+    }
+  };
+  auto compute_group = [&](int i0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int b = 0; b < KB; ++b) {
+      if (i0 + b >= a.k) continue;
+      const uint32_t* tcol = s_tab + (size_t)(i0 + b) * R * W;
+      uint4 tq[R];
+      uint32_t t4[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        tq[r] = *reinterpret_cast<const uint4*>(tcol + r * W);
+        t4[r] = tcol[r * W + 4];
+      }
+      const Sel sl = selectors(x[b]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] ^= gmul(tq[r], t4[r], sl);
+    }
+  };
+  load_group(0);
+  __builtin_amdgcn_s_waitcnt(kWaitVm16);
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+  compute_group(0);
+  for (int i0 = KB; i0 < a.k; i0 += KB) {
+    load_group(i0);
+    compute_group(i0);
+  }
+  if (!ok) return;
+  bool mismatch = false;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (r >= rn) continue;
+    uint8_t* dst = a.out[r0 + r] + off;
+    if constexpr (MODE == kWrite) {
+      store4(dst, acc[r]);
+    } else if constexpr (MODE == kAccumulate) {
+      store4(dst, load4(dst) ^ acc[r]);
+    } else {
+      mismatch |= load4(dst) != acc[r];
+    }
+  }
+  if constexpr (MODE == kVerify) {
+    if (mismatch) atomicOr(a.flag, 1);
+  }
+}
+
+template <int R, int MODE>
+static hipError_t launch_apply_dword_t(const ApplyArgs& a, hipStream_t s) {
+  constexpr uint64_t kMaxTiles = 1ull << 30;  // launches of at most 2^30 workgroups
+  const uint64_t n4 = a.n16 * 4;
+  const uint64_t tiles = (n4 + kDwordLanes - 1) / kDwordLanes;
+  const unsigned gy = (unsigned)((a.r + R - 1) / R);
+  const size_t lds = apply_lds_bytes<kDwordLanes, R, false>(a.k);
+  for (uint64_t t0 = 0; t0 < tiles; t0 += kMaxTiles) {
+    ApplyArgs part = a;  // v_begin and n16 stay in 16-byte vectors: kMaxTiles * 256 is a multiple of 4
+    part.v_begin = a.v_begin + t0 * kDwordLanes / 4;
+    part.n16 = std::min<uint64_t>(a.n16 - t0 * kDwordLanes / 4, kMaxTiles * kDwordLanes / 4);
+    const unsigned gx = (unsigned)std::min<uint64_t>(tiles - t0, kMaxTiles);
+    hipLaunchKernelGGL((apply_dword_kernel<R, MODE>), dim3(gx, gy), dim3(kDwordLanes), lds, s, part);
+  }
+  return hipGetLastError();
+}
+
 template <int LB, int R, int NV, int MODE, bool TOWER>
 static hipError_t launch_apply_t(const ApplyArgs& a, hipStream_t s) {
   // one workgroup per tile; launches of at most 2^30 tiles, each shifted by v_begin

@@ -67,6 +67,8 @@ RSGPU_DECL(0, 0) RSGPU_DECL(1, 0) RSGPU_DECL(2, 0) RSGPU_DECL(3, 0) RSGPU_DECL(4
 RSGPU_DECL(0, 1) RSGPU_DECL(1, 1) RSGPU_DECL(2, 1) RSGPU_DECL(3, 1) RSGPU_DECL(4, 1)
 #undef RSGPU_DECL
 
+hipError_t launch_apply_dword(const ApplyArgs& a, int mode, int rows, hipStream_t s);
+
 using ApplyFn = hipError_t (*)(const ApplyArgs&, int, int, hipStream_t);
 static const ApplyFn kApplyFns[2][kNumTiles] = {
     {launch_apply_t0_w0, launch_apply_t1_w0, launch_apply_t2_w0, launch_apply_t3_w0, launch_apply_t4_w0},
@@ -74,11 +76,13 @@ static const ApplyFn kApplyFns[2][kNumTiles] = {
 
 hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s) {
   if (a.n16 == 0) return hipSuccess;
-  if (tile < 0 || tile >= kNumTiles || a.r < 1 || a.r > kMaxOut || a.k < 1 || a.k > kMaxIn)
+  const bool dword = tile == kTileDword && !tower;
+  if ((!dword && (tile < 0 || tile >= kNumTiles)) || a.r < 1 || a.r > kMaxOut || a.k < 1 || a.k > kMaxIn)
     return hipErrorInvalidValue;
   // the kernel stages its tables into LDS with 16-byte direct loads
   if (((uintptr_t)a.tables & 15) != 0 || (a.tab_row_stride & 3) != 0) return hipErrorInvalidValue;
   if (rows < 1 || rows > a.r) rows = a.r;
+  if (dword) return launch_apply_dword(a, mode, rows, s);
   return kApplyFns[tower ? 1 : 0][tile](a, mode, rows, s);
 }
 

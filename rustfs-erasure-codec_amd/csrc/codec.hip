@@ -301,10 +301,17 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
   // host tables (same layout as d_tables) travel inline when one launch covers the call
   const bool inline_tabs = h_tables && apply_inline_fits(k_total, r_total, tower, in, out, len);
   if (!inline_tabs && !d_tables) return RSGPU_ERR_UNSUPPORTED;
-  // Default tiling: 256-lane workgroups with every output, two vectors per lane
-  // only when that still leaves >= 8 workgroups per CU.
-  if (variant < 0 || variant >= kNumApplyVariants) variant = n16 >= (uint64_t)kBlock * 2 * 2048 ? 1 : 0;
-  const int tile = variant % kNumTiles, split = kRowSplit[variant / kNumTiles];
+  // Default tiling: GF(2^8) one dword per lane (apply_dword_kernel); tower
+  // elements 256-lane workgroups with every output, two vectors per lane only
+  // when that still leaves >= 8 workgroups per CU.
+  int tile, split;
+  if (variant < 0 || variant >= kNumApplyVariants) {
+    tile = tower ? (n16 >= (uint64_t)kBlock * 2 * 2048 ? 1 : 0) : kTileDword;
+    split = 0;
+  } else {
+    tile = variant % kNumTiles;
+    split = kRowSplit[variant / kNumTiles];
+  }
   for (int r0 = 0; r0 < r_total; r0 += kMaxOut) {
     const int rn = std::min(kMaxOut, r_total - r0);
     for (int k0 = 0; k0 < k_total; k0 += kMaxIn) {

@@ -73,6 +73,9 @@ struct TileShape {
 };
 constexpr int kNumTiles = 5;
 constexpr TileShape kTileShapes[kNumTiles] = {{256, 1}, {256, 2}, {64, 1}, {64, 2}, {64, 4}};
+// The default tiling of GF(2^8) calls (no variant chosen): 256-lane workgroups,
+// one dword of every shard per lane (apply_dword_kernel).
+constexpr int kTileDword = kNumTiles;
 constexpr int kNumRowSplits = 3;
 constexpr int kRowSplit[kNumRowSplits] = {0, 2, 1};
 constexpr int kNumApplyVariants = kNumTiles * kNumRowSplits;
