@@ -149,7 +149,10 @@ int rsgpu_encode_single(const rsgpu_codec* codec, size_t i_data, const rsgpu_sli
 int rsgpu_encode_single_sep(const rsgpu_codec* codec, size_t i_data, rsgpu_cslice single,
                             const rsgpu_slice* parity, size_t n_parity, void* stream);
 /* ReedSolomon::update: new_data[i].ptr == NULL marks an unchanged shard (None);
- * parity ^= c * (old ^ new) for every changed shard. */
+ * parity ^= c * (old ^ new) for every changed shard.  Asynchronous on `stream`,
+ * nothing allocated: the gathered coefficient columns travel in the kernel
+ * arguments, or (wide updates, unaligned buffers) each changed column is applied
+ * from the codec's own tables. */
 int rsgpu_update(const rsgpu_codec* codec, const rsgpu_cslice* old_data, size_t n_old,
                  const rsgpu_cslice* new_data, size_t n_new, const rsgpu_slice* parity,
                  size_t n_parity, void* stream);

@@ -1892,27 +1892,34 @@ int rsgpu_update(const rsgpu_codec* h, const rsgpu_cslice* old_data, size_t n_ol
     if (!parity[j].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
     out[j] = parity[j].ptr;
   }
-  // gather table columns [i, i] for every changed i from the host copy
+  // parity_j ^= c_ji * old_i ^ c_ji * new_i for every changed i
   const bool tower = c.field == RSGPU_FIELD_GF16;
   const size_t W = (size_t)words_per_coef(tower), nc = 2 * changed.size();
-  std::vector<uint32_t> tab(c.m * nc * W);
   std::vector<const uint8_t*> in;
-  for (size_t q = 0; q < changed.size(); ++q) {
-    const size_t i = changed[q];
+  for (size_t i : changed) {
     in.push_back(old_data[i].ptr);
     in.push_back(new_data[i].ptr);
-    for (size_t j = 0; j < c.m; ++j)
-      for (int t = 0; t < 2; ++t)
-        std::memcpy(&tab[(j * nc + 2 * q + t) * W], &c.h_parity_tables[(j * c.k + i) * W], W * 4);
   }
   hipStream_t s = (hipStream_t)stream;
-  DeviceBuffer dt;
-  HIP_TRY(dt.alloc(tab.size() * 4));
-  HIP_TRY(hipMemcpyAsync(dt.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
-  rc = run_apply((const uint32_t*)dt.p, (int)nc, (int)c.m, tower, in, out, len, kAccumulate, nullptr, s,
-                 pick_variant(c, kOpEncode, 0, len));
-  HIP_TRY(hipStreamSynchronize(s));  // the gathered tables are freed on return
-  return rc;
+  const int variant = pick_variant(c, kOpEncode, 0, len);
+  if (apply_inline_fits((int)nc, (int)c.m, tower, in.data(), out.data(), len)) {
+    // one launch, the gathered columns [i, i] inline in its arguments
+    std::vector<uint32_t> tab(c.m * nc * W);
+    for (size_t q = 0; q < changed.size(); ++q)
+      for (size_t j = 0; j < c.m; ++j)
+        for (int t = 0; t < 2; ++t)
+          std::memcpy(&tab[(j * nc + 2 * q + t) * W], &c.h_parity_tables[(j * c.k + changed[q]) * W], W * 4);
+    return run_apply(nullptr, (int)nc, (int)c.m, tower, in.data(), out.data(), len, kAccumulate, nullptr, s,
+                     variant, -1, 0, tab.data());
+  }
+  // otherwise column i of the codec's own device tables, old then new: two
+  // accumulating launches per changed shard, nothing allocated
+  for (size_t q = 0; q < changed.size(); ++q)
+    for (int t = 0; t < 2; ++t)
+      if ((rc = run_apply((const uint32_t*)c.d_parity_tables.p, 1, (int)c.m, tower, &in[2 * q + t], out.data(),
+                          len, kAccumulate, nullptr, s, variant, (int)c.k, (int)changed[q])))
+        return rc;
+  return RSGPU_OK;
 }
 
 // ------------------------------------------------- format identity, split, join
