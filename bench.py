@@ -743,6 +743,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(bytes_per_launch),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE runs of this kernel and "
+                                       "workload, scaled by a 1 GiB copy (profiles/"
+                                       "batch_kernel_encode_10x4_1MiB_traffic.json); not re-measured here",
                      "kernel": "batch_kernel (512 x RS 10+4 x 1 MiB encode, one launch)",
                      "launches_timed": int(len(d_batch)),
                      "bytes_per_launch": bytes_per_launch,

@@ -165,7 +165,10 @@ int rsgpu_update(const rsgpu_codec* codec, const rsgpu_cslice* old_data, size_t 
 int rsgpu_verify(const rsgpu_codec* codec, const rsgpu_cslice* shards, size_t n_shards,
                  int32_t* ok, void* stream);
 /* Asynchronous verify: writes 0 (match) or nonzero (mismatch) to the device
- * int32 at `device_flag` on `stream`. */
+ * int32 at `device_flag` on `stream`, whatever it held before.  Classic codecs
+ * of at most 64 data shards clear and report in the same launch (a per-stream
+ * call id and a 16-byte library workspace per (device, stream) order the clear
+ * before any report); other codecs clear with a launch of their own first. */
 int rsgpu_verify_async(const rsgpu_codec* codec, const rsgpu_cslice* shards, size_t n_shards,
                        int32_t* device_flag, void* stream);
 /* ReedSolomon::verify_with_buffer (verify.rs:98-118): parity is recomputed into

@@ -48,6 +48,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   const int r0 = blockIdx.y * R;  // first output of this workgroup
   const int rn = a.r - r0 < R ? a.r - r0 : R;
   const uint64_t tile = (uint64_t)LB * NV;
+  if constexpr (MODE == kVerify) verify_claim(a.clear_ws, a.call_id, a.flag);
 
   // Table staging: 16-byte chunks go straight from global memory into LDS
   // (global_load_lds, no registers), issued before the shard loads so that both
@@ -183,29 +184,39 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     if (!ok[j]) continue;
+    uint4 o[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      if (r >= rn) continue;
-      uint4 o;
       if constexpr (TOWER) {
-        o = interleave(acc[r][j]);
+        o[r] = interleave(acc[r][j]);
       } else {
-        o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
+        o[r] = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
       }
-      uint8_t* dst = a.out[r0 + r] + v[j];
-      if constexpr (MODE == kWrite) {
-        store16(dst, o);
-      } else if constexpr (MODE == kAccumulate) {
-        uint4 p = load16(dst);
-        store16(dst, make_uint4(p.x ^ o.x, p.y ^ o.y, p.z ^ o.z, p.w ^ o.w));
-      } else {
-        uint4 p = load16(dst);
-        mismatch |= ((p.x ^ o.x) | (p.y ^ o.y) | (p.z ^ o.z) | (p.w ^ o.w)) != 0;
+    }
+    if constexpr (MODE == kWrite) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r < rn) store16(a.out[r0 + r] + v[j], o[r]);
+    } else {
+      // the stored outputs, all loaded before any is used (rows past rn re-read
+      // row rn - 1, unused)
+      uint4 p[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) p[r] = load16(a.out[r0 + (r < rn ? r : rn - 1)] + v[j]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r >= rn) continue;
+        if constexpr (MODE == kAccumulate) {
+          store16(a.out[r0 + r] + v[j], make_uint4(p[r].x ^ o[r].x, p[r].y ^ o[r].y, p[r].z ^ o[r].z,
+                                                  p[r].w ^ o[r].w));
+        } else {
+          mismatch |= ((p[r].x ^ o[r].x) | (p[r].y ^ o[r].y) | (p[r].z ^ o[r].z) | (p[r].w ^ o[r].w)) != 0;
+        }
       }
     }
   }
   if constexpr (MODE == kVerify) {
-    if (mismatch) atomicOr(a.flag, 1);
+    if (mismatch) verify_report(a.clear_ws, a.call_id, a.flag);
   }
 }
 
@@ -229,6 +240,7 @@ __global__ __launch_bounds__(kDwordLanes) void apply_dword_kernel(const ApplyArg
   extern __shared__ __align__(16) uint32_t s_tab[];  // [input][output][W]
   const int r0 = blockIdx.y * R;
   const int rn = a.r - r0 < R ? a.r - r0 : R;
+  if constexpr (MODE == kVerify) verify_claim(a.clear_ws, a.call_id, a.flag);
   // table staging, as in apply_kernel
   {
     constexpr int CW = W / 4;
@@ -296,20 +308,25 @@ __global__ __launch_bounds__(kDwordLanes) void apply_dword_kernel(const ApplyArg
   }
   if (!ok) return;
   bool mismatch = false;
+  if constexpr (MODE == kWrite) {
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (r >= rn) continue;
-    uint8_t* dst = a.out[r0 + r] + off;
-    if constexpr (MODE == kWrite) {
-      store4(dst, acc[r]);
-    } else if constexpr (MODE == kAccumulate) {
-      store4(dst, load4(dst) ^ acc[r]);
-    } else {
-      mismatch |= load4(dst) != acc[r];
+    for (int r = 0; r < R; ++r)
+      if (r < rn) store4(a.out[r0 + r] + off, acc[r]);
+  } else {
+    // the stored outputs, all loaded before any is used (rows past rn re-read row
+    // rn - 1, unused)
+    uint32_t p[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) p[r] = load4(a.out[r0 + (r < rn ? r : rn - 1)] + off);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r >= rn) continue;
+      if constexpr (MODE == kAccumulate) store4(a.out[r0 + r] + off, p[r] ^ acc[r]);
+      else mismatch |= p[r] != acc[r];
     }
   }
   if constexpr (MODE == kVerify) {
-    if (mismatch) atomicOr(a.flag, 1);
+    if (mismatch) verify_report(a.clear_ws, a.call_id, a.flag);
   }
 }
 

@@ -286,10 +286,17 @@ static bool apply_inline_fits(int k_total, int r_total, bool tower, const uint8_
   return true;
 }
 
+// The in-kernel flag clear of one verify call (verify_claim in kernel_common.h).
+struct VerifyClear {
+  uint64_t* ws;
+  uint64_t id;
+};
+
 static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
                      const uint8_t* const* in, uint8_t* const* out,
                      size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1,
-                     int table_cols = -1, int col0 = 0, const uint32_t* h_tables = nullptr) {
+                     int table_cols = -1, int col0 = 0, const uint32_t* h_tables = nullptr,
+                     const VerifyClear* vc = nullptr) {
   if (table_cols < 0) table_cols = k_total;
   const int W = words_per_coef(tower);
   bool vec_ok = true;
@@ -301,6 +308,8 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
   // host tables (same layout as d_tables) travel inline when one launch covers the call
   const bool inline_tabs = h_tables && apply_inline_fits(k_total, r_total, tower, in, out, len);
   if (!inline_tabs && !d_tables) return RSGPU_ERR_UNSUPPORTED;
+  // the byte kernel alone does not clear a verify flag
+  if (mode == kVerify && vc && !n16) HIP_TRY(launch_fill_i32(flag, 0, 1, s));
   // Default tiling: GF(2^8) one dword per lane (apply_dword_kernel); tower
   // elements 256-lane workgroups with every output, two vectors per lane only
   // when that still leaves >= 8 workgroups per CU.
@@ -327,6 +336,10 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
         a.v_begin = 0;
         a.k = kn;
         a.r = rn;
+        if (mode == kVerify && vc) {
+          a.clear_ws = vc->ws;
+          a.call_id = vc->id;
+        }
         if (inline_tabs) {  // this launch's rn x kn block, rows kn * W apart
           for (int j = 0; j < rn; ++j)
             std::memcpy(a.tab_inline + (size_t)j * kn * W,
@@ -513,6 +526,55 @@ static int encode_sep_impl(const Codec& c, const rsgpu_cslice* data, size_t nd,
                    c.field == RSGPU_FIELD_GF16, in, out, len, kWrite, nullptr, s, variant);
 }
 
+// ------------------------------------------------------------ verify tickets
+// Per (device, stream): the two-word device workspace of the in-kernel flag clear
+// and the stream's next call id.  Work on one stream runs in order, and the
+// slot's mutex is held from taking an id until the call's launches are enqueued,
+// so ids reach the device in increasing order.  Workspaces come from 16 KiB
+// device chunks zeroed once, kept for the life of the process.
+struct VerifySlot {
+  std::mutex mu;
+  uint64_t* ws = nullptr;
+  uint64_t next = 0;
+};
+constexpr size_t kVerifyChunkSlots = 1024;
+static std::mutex g_verify_mu;
+static std::map<std::pair<int, hipStream_t>, std::unique_ptr<VerifySlot>> g_verify_slots;
+static std::vector<std::tuple<int, uint64_t*, size_t>> g_verify_chunks;  // device, base, slots used
+
+static VerifySlot* verify_slot(hipStream_t s) {
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return nullptr;
+  // the calling thread's last (device, stream): no global lock on a repeat
+  thread_local int t_device = -1;
+  thread_local hipStream_t t_stream = nullptr;
+  thread_local VerifySlot* t_slot = nullptr;
+  if (t_slot && t_device == device && t_stream == s) return t_slot;
+  std::lock_guard<std::mutex> g(g_verify_mu);
+  const auto key = std::make_pair(device, s);
+  auto it = g_verify_slots.find(key);
+  if (it != g_verify_slots.end()) {
+    t_device = device, t_stream = s, t_slot = it->second.get();
+    return t_slot;
+  }
+  std::tuple<int, uint64_t*, size_t>* chunk = nullptr;
+  for (auto& ch : g_verify_chunks)
+    if (std::get<0>(ch) == device && std::get<2>(ch) < kVerifyChunkSlots) chunk = &ch;
+  if (!chunk) {
+    void* p = nullptr;
+    if (hipMalloc(&p, kVerifyChunkSlots * 16) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, kVerifyChunkSlots * 16) != hipSuccess) return nullptr;  // once per chunk
+    g_verify_chunks.emplace_back(device, (uint64_t*)p, 0);
+    chunk = &g_verify_chunks.back();
+  }
+  auto slot = std::make_unique<VerifySlot>();
+  slot->ws = std::get<1>(*chunk) + 2 * std::get<2>(*chunk)++;
+  VerifySlot* out = slot.get();
+  g_verify_slots.emplace(key, std::move(slot));
+  t_device = device, t_stream = s, t_slot = out;
+  return out;
+}
+
 static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* d_flag,
                        hipStream_t s, int variant = kPick) {
   const size_t total = c.k + c.m;
@@ -530,14 +592,23 @@ static int verify_impl(const Codec& c, const rsgpu_cslice* shards, size_t n, int
     if ((rc = leo_matrix_codec(c, s, mc))) return rc;
     return verify_impl(*mc, shards, n, d_flag, s, variant);
   }
-  HIP_TRY(launch_fill_i32(d_flag, 0, 1, s));
-  if (c.family != RSGPU_FAMILY_CLASSIC)
-    return leopard_verify(c, shards, shards[0].len, d_flag, s, variant);
   std::vector<const uint8_t*> in(c.k);
   std::vector<uint8_t*> out(c.m);
   for (size_t i = 0; i < c.k; ++i) in[i] = shards[i].ptr;
   for (size_t i = 0; i < c.m; ++i) out[i] = const_cast<uint8_t*>(shards[c.k + i].ptr);
   const bool tower = c.field == RSGPU_FIELD_GF16;
+  if (c.family == RSGPU_FAMILY_CLASSIC && c.k <= (size_t)kMaxIn) {
+    // one launch clears the flag and compares (verify_claim / verify_report)
+    if (VerifySlot* vs = verify_slot(s)) {
+      std::lock_guard<std::mutex> g(vs->mu);
+      const VerifyClear vc{vs->ws, ++vs->next};
+      return run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m, tower, in.data(), out.data(),
+                       shards[0].len, kVerify, d_flag, s, variant, -1, 0, nullptr, &vc);
+    }
+  }
+  HIP_TRY(launch_fill_i32(d_flag, 0, 1, s));
+  if (c.family != RSGPU_FAMILY_CLASSIC)
+    return leopard_verify(c, shards, shards[0].len, d_flag, s, variant);
   if (c.k <= (size_t)kMaxIn)
     return run_apply((const uint32_t*)c.d_parity_tables.p, (int)c.k, (int)c.m, tower, in, out,
                      shards[0].len, kVerify, d_flag, s, variant);

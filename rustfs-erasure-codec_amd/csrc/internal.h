@@ -44,10 +44,15 @@ struct ApplyArgs {
   int32_t tab_row_stride;  // dwords between consecutive table rows
   int32_t tab_col0;        // table column of in[0]
   int32_t r;               // outputs in this launch (<= kMaxOut)
+  // verify only: the flag protocol of one call (verify_claim / verify_report);
+  // null: the caller cleared the flag
+  uint64_t* clear_ws;
+  uint64_t call_id;
   // the tables when `tables` is null (same layout, rows tab_row_stride apart);
   // the kernel reads them straight from its kernel-argument segment
   uint32_t tab_inline[kInlineTabWords];
 };
+static_assert(offsetof(ApplyArgs, tab_inline) % 16 == 0, "16-byte direct loads of the inline tables");
 
 struct ByteArgs {
   const uint8_t* in[kMaxIn];

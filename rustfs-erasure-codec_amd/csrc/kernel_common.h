@@ -45,6 +45,32 @@ __device__ __forceinline__ uint32_t gmul(const uint4& q, uint32_t t4, const Sel&
   return xor3(a, b, c);
 }
 
+// The verify flag of one call, cleared and reported by the same launch.  Every
+// launch of a verify call carries the call's id (ids grow per stream) and a
+// two-word workspace: ws[0] the last call claimed, ws[1] the last call whose
+// flag is cleared.  The lane that claims the id first clears the flag and
+// publishes the id.  Workgroup (0, 0) claims at its start, so a codeword's flag
+// ends cleared; a lane that finds a mismatch claims too if the id is not yet
+// published, then waits for it before ORing into the flag.  A waiting lane waits
+// only on the lane that won the claim, which has started and finishes the clear
+// without waiting on anything.
+__device__ __forceinline__ void verify_try_clear(uint64_t* ws, uint64_t id, int32_t* flag) {
+  if (__hip_atomic_fetch_max(ws, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < id) {
+    __hip_atomic_store(flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(ws + 1, id, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+__device__ __forceinline__ void verify_claim(uint64_t* ws, uint64_t id, int32_t* flag) {
+  if (ws && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) verify_try_clear(ws, id, flag);
+}
+__device__ __forceinline__ void verify_report(uint64_t* ws, uint64_t id, int32_t* flag) {
+  if (ws && __hip_atomic_load(ws + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < id) {
+    verify_try_clear(ws, id, flag);
+    while (__hip_atomic_load(ws + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < id) __builtin_amdgcn_s_sleep(2);
+  }
+  __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Shard bytes are streamed exactly once per operation, so loads and stores are
 // nontemporal: they skip L2 retention and measured +8% read bandwidth on gfx950.
 typedef unsigned int v4u_t __attribute__((ext_vector_type(4)));
