@@ -357,6 +357,31 @@ typedef struct rsgpu_mem_streams {
 } rsgpu_mem_streams;
 int64_t rsgpu_mem_stream_read(void* streams, size_t shard, uint8_t* buf, size_t len);
 int64_t rsgpu_mem_stream_write(void* streams, size_t shard, const uint8_t* buf, size_t len);
+/* ReedSolomon::reconstruct_stream (stream.rs:839-900), the in-memory form over
+ * the reference's Cursor<Vec<u8>> shards: rsgpu_vec is a byte vector (data,
+ * len, cap) with a cursor position.  A shard with len != 0 is present and is
+ * read from position 0 (pos is reset first, as the reference resets the
+ * cursor); pos ends where the reading stopped.  A shard with len == 0 is
+ * missing: the rebuilt bytes are appended at data[len..] and pos is not moved
+ * (the reference writes through the cursor's Vec).  When a vector lacks room,
+ * grow(user, shard, vec, need) must make vec->cap >= need (moving data, setting
+ * data and cap; len and pos stay) and return 0; the first append of a shard asks
+ * for the common present length when all present shards have one (the
+ * reference's reserve_exact).  grow may be NULL when the caller pre-sized every
+ * missing vector; a vector that cannot grow fails as a write error of that
+ * shard.  In PARALLEL I/O grow is called concurrently for different shards.
+ * Errors as the reference: count errors, then UnsupportedCodecFamily for
+ * Leopard; no present shard returns RSGPU_OK with nothing written; fewer than
+ * k present is TooFewShardsPresent; then as rsgpu_reconstruct_stream_from. */
+typedef struct rsgpu_vec {
+  uint8_t* data;
+  size_t len;
+  size_t cap;
+  size_t pos;
+} rsgpu_vec;
+typedef int (*rsgpu_grow_cb)(void* user, size_t shard, rsgpu_vec* vec, size_t need);
+int rsgpu_reconstruct_stream(const rsgpu_codec* codec, rsgpu_vec* shards, size_t n_shards, rsgpu_grow_cb grow,
+                             void* user, const rsgpu_stream_options* options, rsgpu_stream_error* err);
 
 /* ---- host-pointer variants (copy through library-owned device staging) ---- */
 int rsgpu_encode_host(const rsgpu_codec* codec, uint8_t* const* shards, size_t n_shards,

@@ -125,6 +125,16 @@ pub struct rsgpu_mem_streams {
     pub readers: *mut rsgpu_mem_stream,
     pub writers: *mut rsgpu_mem_stream,
 }
+#[repr(C)]
+#[derive(Clone, Copy, Debug)]
+pub struct rsgpu_vec {
+    pub data: *mut u8,
+    pub len: usize,
+    pub cap: usize,
+    pub pos: usize,
+}
+pub type rsgpu_grow_cb =
+    unsafe extern "C" fn(user: *mut c_void, shard: usize, vec: *mut rsgpu_vec, need: usize) -> c_int;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]
@@ -432,6 +442,15 @@ extern "C" {
     ) -> c_int;
     pub fn rsgpu_mem_stream_read(streams: *mut c_void, shard: usize, buf: *mut u8, len: usize) -> i64;
     pub fn rsgpu_mem_stream_write(streams: *mut c_void, shard: usize, buf: *const u8, len: usize) -> i64;
+    pub fn rsgpu_reconstruct_stream(
+        codec: *const rsgpu_codec,
+        shards: *mut rsgpu_vec,
+        n_shards: usize,
+        grow: Option<rsgpu_grow_cb>,
+        user: *mut c_void,
+        options: *const rsgpu_stream_options,
+        err: *mut rsgpu_stream_error,
+    ) -> c_int;
 
     // ---- shards in host memory: chunked pinned pipeline
     pub fn rsgpu_encode_host(

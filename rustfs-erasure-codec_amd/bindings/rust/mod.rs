@@ -303,6 +303,14 @@ impl GpuCodec {
     ) -> Result<(), crate::stream::StreamError> {
         stream_io::reconstruct_stream_from(self.raw, readers, writers, opts)
     }
+
+    pub fn reconstruct_stream(
+        &self,
+        shards: &mut [std::io::Cursor<Vec<u8>>],
+        opts: &crate::stream::StreamOptions,
+    ) -> Result<(), crate::stream::StreamError> {
+        stream_io::reconstruct_stream(self.raw, shards, opts)
+    }
 }
 
 /// A prepared reconstruction plan (`ReconstructWorkspace`).

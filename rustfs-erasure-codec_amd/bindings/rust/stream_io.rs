@@ -203,3 +203,64 @@ pub fn reconstruct_stream_from<R: Read + Send, W: Write + Send>(
     };
     result(rc, &err, &s, 0)
 }
+
+/// What `grow_cb` reaches through `user`: the caller's cursors.  The library asks
+/// for room for one shard at a time per thread (different shards concurrently in
+/// parallel I/O), so each cursor is touched by one thread at a time.
+struct Grow {
+    shards: *mut io::Cursor<Vec<u8>>,
+}
+
+/// Makes `vec->cap >= need` for shard `shard` through the cursor's own `Vec`: the
+/// bytes the library appended so far become the vector's length, the reservation
+/// grows it, and the new buffer goes back to the library.
+unsafe extern "C" fn grow_cb(user: *mut c_void, shard: usize, v: *mut rsgpu_vec, need: usize) -> i32 {
+    let g = &*(user as *const Grow);
+    let r = catch_unwind(AssertUnwindSafe(|| {
+        let v = &mut *v;
+        let bytes = (*g.shards.add(shard)).get_mut();
+        bytes.set_len(v.len);
+        bytes.reserve_exact(need - v.len);
+        v.data = bytes.as_mut_ptr();
+        v.cap = bytes.capacity();
+    }));
+    if r.is_ok() { 0 } else { -1 }
+}
+
+/// `ReedSolomon::reconstruct_stream` (stream.rs:839-900): non-empty cursors are
+/// present and are read from position 0; empty ones receive the rebuilt bytes,
+/// appended through their `Vec` with the position unmoved.  On error the bytes
+/// already appended stay, as the reference's writers keep earlier blocks.
+pub fn reconstruct_stream(
+    codec: *const rsgpu_codec,
+    shards: &mut [io::Cursor<Vec<u8>>],
+    opts: &StreamOptions,
+) -> Result<(), StreamError> {
+    let mut vecs: Vec<rsgpu_vec> = shards
+        .iter_mut()
+        .map(|c| {
+            let pos = c.position() as usize;
+            let bytes = c.get_mut();
+            rsgpu_vec { data: bytes.as_mut_ptr(), len: bytes.len(), cap: bytes.capacity(), pos }
+        })
+        .collect();
+    let present: Vec<bool> = vecs.iter().map(|v| v.len != 0).collect();
+    let g = Grow { shards: shards.as_mut_ptr() };
+    let o = options(opts);
+    let mut err = rsgpu_stream_error::default();
+    let rc = unsafe {
+        rsgpu_reconstruct_stream(codec, vecs.as_mut_ptr(), vecs.len(), Some(grow_cb),
+                                 &g as *const Grow as *mut c_void, &o, &mut err)
+    };
+    for ((c, v), p) in shards.iter_mut().zip(vecs.iter()).zip(present) {
+        if p {
+            c.set_position(v.pos as u64);
+        } else {
+            // SAFETY: the library wrote v.len initialised bytes into this vector's
+            // buffer (grow_cb keeps data and cap in step with the Vec).
+            unsafe { c.get_mut().set_len(v.len) };
+        }
+    }
+    let none: Streams<io::Empty, io::Sink> = Streams::new(Vec::new(), Vec::new());
+    result(rc, &err, &none, 0)
+}

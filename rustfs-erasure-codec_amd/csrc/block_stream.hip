@@ -253,6 +253,44 @@ int hip_fail(hipError_t e, const char* what) {
     if (_e != hipSuccess) return hip_fail(_e, #expr);       \
   } while (0)
 
+// Callbacks of rsgpu_reconstruct_stream over the caller's byte vectors.  A read
+// copies from data[pos..len) and advances pos.  A write appends at data[len..),
+// first asking `grow` for room: the first write of a shard asks for at least
+// `reserve` bytes (the reference's reserve_exact of the common length,
+// stream.rs:423-432), later ones for exactly what the block needs.  Each call
+// touches one shard's vector (and its `reserved` byte), so PARALLEL I/O is safe.
+struct VecStreams {
+  rsgpu_vec* v;
+  rsgpu_grow_cb grow;
+  void* user;
+  size_t reserve;
+  std::vector<uint8_t> reserved;
+};
+
+int64_t vec_read(void* u, size_t shard, uint8_t* buf, size_t len) {
+  rsgpu_vec& v = static_cast<VecStreams*>(u)->v[shard];
+  const size_t n = v.pos < v.len ? std::min(len, v.len - v.pos) : 0;
+  if (n) std::memcpy(buf, v.data + v.pos, n);
+  v.pos += n;
+  return (int64_t)n;
+}
+
+int64_t vec_write(void* u, size_t shard, const uint8_t* buf, size_t len) {
+  VecStreams& S = *static_cast<VecStreams*>(u);
+  rsgpu_vec& v = S.v[shard];
+  size_t need = v.len + len;
+  if (!S.reserved[shard]) {
+    S.reserved[shard] = 1;
+    need = std::max(need, v.len + S.reserve);
+  }
+  if (need > v.cap) {
+    if (!S.grow || S.grow(S.user, shard, &v, need) != 0 || v.cap < need || !v.data) return -1;
+  }
+  std::memcpy(v.data + v.len, buf, len);
+  v.len += len;
+  return (int64_t)len;
+}
+
 std::vector<size_t> iota(size_t n, size_t from = 0) {
   std::vector<size_t> v(n);
   for (size_t i = 0; i < n; ++i) v[i] = from + i;
@@ -470,6 +508,44 @@ int rsgpu_reconstruct_stream_from(const rsgpu_codec* h, size_t n_readers, size_t
     return write_shards(wr, user, missing, missing, missing, par_wr, X, set, len, err);
   };
   return pipeline(X, read, submit, write);
+}
+
+// ReedSolomon::reconstruct_stream (stream.rs:839-900): the count check, the
+// Leopard refusal, then presence by length (an empty vector is missing); no
+// present shard is Ok with nothing written, fewer than k is TooFewShardsPresent.
+// Present vectors are read from position 0; missing ones are appended to, with
+// the common length as their first reservation when every present shard has
+// the same length.  The rest is reconstruct_stream_from.
+int rsgpu_reconstruct_stream(const rsgpu_codec* h, rsgpu_vec* shards, size_t n_shards, rsgpu_grow_cb grow,
+                             void* user, const rsgpu_stream_options* options, rsgpu_stream_error* err) {
+  set_err(err, RSGPU_STREAM_OK, 0, 0);
+  Options o;
+  if (!h || (!shards && n_shards) || resolve(options, &o)) return RSGPU_ERR_INVALID_ARGUMENT;
+  const Codec& c = h->c;
+  const size_t n = c.k + c.m;
+  int rc = count_check(n_shards, n, err);
+  if (rc) return rc;
+  if (is_leopard(c)) return codec_err(err, 0, RSGPU_UNSUPPORTED_CODEC_FAMILY);
+  size_t present = 0, known = 0;
+  bool equal = true;
+  for (size_t i = 0; i < n; ++i) {
+    const size_t len = shards[i].len;
+    if (!len) continue;
+    ++present;
+    if (!known) known = len;
+    equal = equal && len == known;
+  }
+  if (!present) return RSGPU_OK;
+  if (present < c.k) return codec_err(err, 0, RSGPU_TOO_FEW_SHARDS_PRESENT);
+  std::vector<uint8_t> has_reader(n), has_writer(n);
+  for (size_t i = 0; i < n; ++i) {
+    has_reader[i] = shards[i].len != 0;
+    has_writer[i] = !has_reader[i];
+    if (has_reader[i]) shards[i].pos = 0;
+  }
+  VecStreams S{shards, grow, user, equal ? known : 0, std::vector<uint8_t>(n, 0)};
+  return rsgpu_reconstruct_stream_from(h, n, n, has_reader.data(), has_writer.data(), vec_read, vec_write, &S,
+                                       options, err);
 }
 
 }  // extern "C"

@@ -584,28 +584,57 @@ class ReedSolomon:
         io.raise_for(rc, err)
 
     def reconstruct_stream(self, shards, block_size=4 << 20, io_mode="auto"):
-        """In-memory form (stream.rs:839-930): `shards` are io.BytesIO; non-empty ones are
-        present (read from position 0), empty ones receive the rebuilt bytes."""
-        total = self.total_shard_count()
-        if len(shards) != total:
-            raise StreamError("codec", 0, RsError(1 if len(shards) < total else 2))
-        if self.codec_family() != "classic":
-            raise StreamError("codec", 0, RsError(15))
-        present = [len(s.getbuffer()) != 0 for s in shards]
-        if not any(present):
-            return
-        if sum(present) < self.data_shard_count():
-            raise StreamError("codec", 0, RsError(10))
-        readers, writers = [], []
-        for s, p in zip(shards, present):
-            if p:
-                s.seek(0)
-                readers.append(s)
-                writers.append(None)
+        """In-memory form (stream.rs:839-900) through rsgpu_reconstruct_stream:
+        `shards` are io.BytesIO; non-empty ones are present (read from position 0,
+        left where the reading stopped), empty ones receive the rebuilt bytes
+        (appended, their position unmoved)."""
+        n = len(shards)
+        vecs = (_Vec * max(1, n))()
+        views, arrays, grown = [], [], {}
+        for i, s in enumerate(shards):
+            mv = s.getbuffer()
+            if len(mv):
+                arr = (C.c_char * len(mv)).from_buffer(mv)
+                views.append(mv)
+                arrays.append(arr)
+                vecs[i].data, vecs[i].len, vecs[i].cap = C.addressof(arr), len(mv), len(mv)
+                vecs[i].pos = s.tell()
             else:
-                readers.append(None)
-                writers.append(_AppendWriter(s))
-        self.reconstruct_stream_from(readers, writers, block_size, io_mode)
+                mv.release()
+
+        def grow(_user, shard, vec, need):
+            try:
+                v = vec.contents
+                buf = C.create_string_buffer(need)
+                if v.len:
+                    C.memmove(buf, v.data, v.len)
+                grown[shard] = buf
+                v.data, v.cap = C.addressof(buf), need
+                return 0
+            except Exception:  # reported as a write error of this shard
+                return -1
+
+        cb = _GROW_CB(grow)
+        err = _StreamErr()
+        try:
+            rc = _lib.rsgpu_reconstruct_stream(self._h, vecs, n, cb, None,
+                                               C.byref(_stream_options(block_size, io_mode)),
+                                               C.byref(err))
+        finally:
+            del arrays[:]
+            for mv in views:
+                mv.release()
+            for i, s in enumerate(shards):
+                if vecs[i].data is None:
+                    continue
+                if i in grown:
+                    pos = s.tell()
+                    s.seek(0, 2)
+                    s.write(C.string_at(vecs[i].data, vecs[i].len))
+                    s.seek(pos)
+                else:
+                    s.seek(vecs[i].pos)
+        _raise_stream(rc, err, {}, 0)
 
     # ---- kernel-variant dispatch (every variant produces the same bytes)
     def set_variant(self, variant):
@@ -673,6 +702,13 @@ _lib.rsgpu_encode_stream.argtypes = [_vp, _sz, _sz, _READ_CB, _WRITE_CB, _vp, _S
 _lib.rsgpu_verify_stream.argtypes = [_vp, _sz, _READ_CB, _vp, _SOPT, _i32p, C.POINTER(_StreamErr)]
 _lib.rsgpu_reconstruct_stream_from.argtypes = [_vp, _sz, _sz, _u8p, _u8p, _READ_CB, _WRITE_CB, _vp, _SOPT,
                                                C.POINTER(_StreamErr)]
+class _Vec(C.Structure):
+    _fields_ = [("data", _vp), ("len", _sz), ("cap", _sz), ("pos", _sz)]
+
+
+_GROW_CB = C.CFUNCTYPE(C.c_int, _vp, _sz, C.POINTER(_Vec), _sz)
+_lib.rsgpu_reconstruct_stream.argtypes = [_vp, C.POINTER(_Vec), _sz, _GROW_CB, _vp, _SOPT,
+                                          C.POINTER(_StreamErr)]
 # the library's own in-memory callbacks, as C function pointers
 _NATIVE_RD = _READ_CB(C.cast(_lib.rsgpu_mem_stream_read, _vp).value)
 _NATIVE_WR = _WRITE_CB(C.cast(_lib.rsgpu_mem_stream_write, _vp).value)
@@ -767,21 +803,6 @@ def _stream_io(readers, writers):
     if live and all(isinstance(s, HostStream) for s in live):
         return _NativeIO(readers, writers)
     return _StreamIO(readers, writers)
-
-
-class _AppendWriter:
-    """Appends to a BytesIO without moving its position (the reference writes
-    through the cursor's Vec, stream.rs:423-436)."""
-
-    def __init__(self, bio):
-        self.bio = bio
-
-    def write(self, b):
-        pos = self.bio.tell()
-        self.bio.seek(0, 2)
-        n = self.bio.write(b)
-        self.bio.seek(pos)
-        return n
 
 
 class _StreamIO:

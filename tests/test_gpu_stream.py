@@ -301,3 +301,76 @@ def test_staging_is_reused_across_sizes_and_nested_calls():
     rs.encode_stream([Nested(data[0])] + [io.BytesIO(d) for d in data[1:]], writers, block_size=2048)
     assert [w.getvalue() for w in writers] == oracle_parity(k, m, data)
     assert [o.tobytes() for o in inner_out] == oracle_parity(k, m, [d.tobytes() for d in inner_data])
+
+
+@pytest.mark.parametrize("io_mode", ["serial", "parallel"])
+def test_reconstruct_stream_c_entry_presized_and_full(io_mode):
+    """rsgpu_reconstruct_stream called through the C ABI with grow == NULL: missing
+    vectors sized by the caller receive the rebuilt bytes; present vectors are read
+    from position 0 and end at their length; a missing vector without room is a
+    write error of that shard (stream.rs:839-900)."""
+    import ctypes as C
+    announce(f"stream reconstruct C cursors {io_mode}")
+    k, m, n = 5, 3, 9000
+    data = [ref_data(n)[i:] + ref_data(i) for i in range(k)]
+    full = data + oracle_parity(k, m, data)
+    rs = rsgpu.ReedSolomon(k, m)
+    missing = {0, 3, 6}
+    bufs = [bytearray(n) if i in missing else bytearray(full[i]) for i in range(k + m)]
+    arrs = [(C.c_uint8 * n).from_buffer(b) for b in bufs]
+    vecs = (rsgpu._Vec * (k + m))()
+    for i in range(k + m):
+        vecs[i].data, vecs[i].cap = C.addressof(arrs[i]), n
+        vecs[i].len = 0 if i in missing else n
+        vecs[i].pos = 7  # reset to 0 for present shards, kept for missing ones
+    opts = rsgpu._stream_options(2048, io_mode)
+    err = rsgpu._StreamErr()
+    null_grow = rsgpu._GROW_CB()  # a NULL function pointer
+    rc = rsgpu._lib.rsgpu_reconstruct_stream(rs._h, vecs, k + m, null_grow, None, C.byref(opts), C.byref(err))
+    assert rc == 0, (rc, err.kind, err.shard_index)
+    for i in range(k + m):
+        assert bytes(bufs[i]) == full[i], i
+        assert vecs[i].len == n
+        assert vecs[i].pos == (7 if i in missing else n), i
+    # a missing vector one block short of room: write error of that shard, the
+    # earlier shards' blocks already appended
+    small = bytearray(1024)
+    arr_small = (C.c_uint8 * 1024).from_buffer(small)
+    for i in range(k + m):
+        vecs[i].len = 0 if i in missing else n
+        vecs[i].pos = 0
+    vecs[3].data, vecs[3].cap = C.addressof(arr_small), 1024
+    rc = rsgpu._lib.rsgpu_reconstruct_stream(rs._h, vecs, k + m, null_grow, None, C.byref(opts), C.byref(err))
+    assert rc == 105 and err.kind == 2 and err.shard_index == 3
+    assert vecs[3].len == 0  # not even the first block fits
+    assert vecs[0].len >= 2048 and bytes(bufs[0][:2048]) == full[0][:2048]
+
+
+def test_reconstruct_stream_c_entry_errors():
+    """Count errors, the Leopard refusal, the empty layout and unequal present
+    lengths of rsgpu_reconstruct_stream, in the reference's order."""
+    import ctypes as C
+    announce("stream reconstruct C cursor errors")
+    k, m = 3, 2
+    rs = rsgpu.ReedSolomon(k, m)
+    err = rsgpu._StreamErr()
+    grow = rsgpu._GROW_CB(lambda *a: -1)
+
+    def call(codec, vecs, n):
+        return rsgpu._lib.rsgpu_reconstruct_stream(codec._h, vecs, n, grow, None, None, C.byref(err))
+
+    vecs = (rsgpu._Vec * 6)()
+    assert call(rs, vecs, 4) == 1 and err.kind == 3 and err.shard_index == 0  # TooFewShards
+    assert call(rs, vecs, 6) == 2  # TooManyShards
+    assert call(rs, vecs, 5) == 0  # nothing present: nothing to do
+    leo = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf8")
+    assert call(leo, vecs, 5) == 15  # UnsupportedCodecFamily, before presence is looked at
+    b = [bytearray(b"abcd"), bytearray(b"abcd"), bytearray(b"abc")]
+    arrs = [(C.c_uint8 * len(x)).from_buffer(x) for x in b]
+    for i in range(3):
+        vecs[i].data, vecs[i].len, vecs[i].cap, vecs[i].pos = C.addressof(arrs[i]), len(b[i]), len(b[i]), 0
+    vecs[3] = rsgpu._Vec()
+    vecs[4] = rsgpu._Vec()
+    assert call(rs, vecs, 5) == 9 and err.shard_index == 2  # IncorrectShardSize at the short shard
+    vecs[2] = rsgpu._Vec()
+    assert call(rs, vecs, 5) == 10  # TooFewShardsPresent
