@@ -904,54 +904,122 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 2 * NT
   }
 }
 
+// The decode's input rows (compute_reconstruct_chunk step 1, leopard_gf16/decode.rs:858-893):
+// lanes 0..parity are the parity shards, lanes m..m+k the data shards; a
+// present shard's words, scaled by its lane's error-locator table, else zero.
+template <bool G16, int G>
+__device__ __forceinline__ void decode_row(const LeoArgs& a, uint32_t lane, uint64_t word, Word<G16> (&v)[G]) {
+  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
+  int shard = -1;
+  if (lane < a.parity) shard = (int)(a.k + lane);
+  else if (lane >= a.m && lane < a.m + a.k) shard = (int)(lane - a.m);
+  if (shard >= 0 && (shard_flags(a, shard) & 1)) {
+    gloadv<G16, G>(shard_ptr(a, shard), word, a.words, a.vec != 0, v);
+    const Tab<G16> te = load_tab<G16>(a.err_tabs + (size_t)lane * TW);
+#pragma unroll
+    for (int q = 0; q < G; ++q) v[q] = wmul<G16>(te, v[q]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < G; ++q) v[q] = Word<G16>{};
+  }
+}
+
+// Formal derivative (decode.rs:537-551, compute_formal_derivative16
+// leopard_gf16/decode.rs:1100-1116) in closed form: row j takes the XOR of the
+// rows j + 2^b for the clear bits b of j (j + 2^b < n), all read before any
+// update.  An item is the four rows 4g .. 4g+3 times G words: bits 0 and 1 stay
+// inside the group; bit b >= 2 adds group g + 2^(b-2) row for row when bit b-2
+// of g is clear.  A row depends only on itself and higher rows, so chunks of
+// items run in increasing g, each reading before any of its writes.
+template <bool G16, int G>
+__device__ __forceinline__ void formal_derivative(const LeoArgs& a, uint32_t* lds) {
+  const uint32_t C = a.C, ng = C / G, lg_ng = lg_groups<G>(a), groups = a.n >> 2;
+  for (uint32_t it0 = 0; it0 < groups * ng; it0 += blockDim.x) {
+    const uint32_t it = it0 + threadIdx.x;
+    const bool live = it < groups * ng;
+    const uint32_t g = it >> lg_ng, wq = (it & (ng - 1)) * G;
+    Word<G16> r0[G], r1[G], r2[G], r3[G];
+    if (live) {
+      lds_getv<G16, G>(lds, (4 * g) * C + wq, r0);
+      lds_getv<G16, G>(lds, (4 * g + 1) * C + wq, r1);
+      lds_getv<G16, G>(lds, (4 * g + 2) * C + wq, r2);
+      lds_getv<G16, G>(lds, (4 * g + 3) * C + wq, r3);
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        wxor(r0[q], r1[q]);
+        wxor(r0[q], r2[q]);
+        wxor(r1[q], r3[q]);
+        wxor(r2[q], r3[q]);
+      }
+      for (uint32_t step = 1; step < groups; step <<= 1) {
+        if ((g & step) || g + step >= groups) continue;
+        const uint32_t h = 4 * (g + step);
+        Word<G16> y[G];
+        lds_getv<G16, G>(lds, h * C + wq, y);
+#pragma unroll
+        for (int q = 0; q < G; ++q) wxor(r0[q], y[q]);
+        lds_getv<G16, G>(lds, (h + 1) * C + wq, y);
+#pragma unroll
+        for (int q = 0; q < G; ++q) wxor(r1[q], y[q]);
+        lds_getv<G16, G>(lds, (h + 2) * C + wq, y);
+#pragma unroll
+        for (int q = 0; q < G; ++q) wxor(r2[q], y[q]);
+        lds_getv<G16, G>(lds, (h + 3) * C + wq, y);
+#pragma unroll
+        for (int q = 0; q < G; ++q) wxor(r3[q], y[q]);
+      }
+    }
+    __syncthreads();
+    if (live) {
+      lds_putv<G16, G>(lds, (4 * g) * C + wq, r0);
+      lds_putv<G16, G>(lds, (4 * g + 1) * C + wq, r1);
+      lds_putv<G16, G>(lds, (4 * g + 2) * C + wq, r2);
+      lds_putv<G16, G>(lds, (4 * g + 3) * C + wq, r3);
+    }
+    __syncthreads();
+  }
+}
+
 // reconstruct_with_resources (leopard_gf8/decode.rs:319-429; leopard_gf16/decode.rs:816-905).
 template <bool G16, int G, bool PL>
 __device__ __forceinline__ void leo_decode_body(const LeoArgs& a, uint32_t* lds, const PlanPtrs& P) {
-  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;
+  constexpr int TW = G16 ? kLeoTabWords16 : kLeoTabWords8;  // inverse tables below
   const uint32_t C = a.C, n = a.n, m = a.m, ng = C / G, lg_ng = lg_groups<G>(a);
   const uint64_t w0 = (uint64_t)blockIdx.x * C;
-  // lanes: 0..parity parity shards, m..m+k data shards, scaled by the error locator
   for (uint32_t it = threadIdx.x; it < n * ng; it += blockDim.x) {
     const uint32_t lane = it >> lg_ng, wq = (it & (ng - 1)) * G;
     Word<G16> v[G];
-    int shard = -1;
-    if (lane < a.parity) shard = (int)(a.k + lane);
-    else if (lane >= m && lane < m + a.k) shard = (int)(lane - m);
-    if (shard >= 0 && (shard_flags(a, shard) & 1)) {
-      gloadv<G16, G>(shard_ptr(a, shard), w0 + wq, a.words, a.vec != 0, v);
-      const Tab<G16> te = load_tab<G16>(a.err_tabs + (size_t)lane * TW);
-#pragma unroll
-      for (int q = 0; q < G; ++q) v[q] = wmul<G16>(te, v[q]);
-    } else {
-#pragma unroll
-      for (int q = 0; q < G; ++q) v[q] = Word<G16>{};
-    }
+    decode_row<G16, G>(a, lane, w0 + wq, v);
     lds_putv<G16, G>(lds, lane * C + wq, v);
   }
   __syncthreads();
   run_layers<G16, G, PL>(lds, a.ifft_layer0, a.ifft_layers, a, P);
-  // formal derivative (decode.rs:537-551): in closed form every lane j takes
-  // lane j ^ XOR_{b : bit b of j clear, j + 2^b < n} lane (j + 2^b), all of
-  // them before the update.
-  for (uint32_t it0 = 0; it0 < n * ng; it0 += blockDim.x) {
-    const uint32_t it = it0 + threadIdx.x;
-    Word<G16> acc[G];
-    const bool live = it < n * ng;
-    uint32_t j = 0, wq = 0;
-    if (live) {
-      j = it >> lg_ng, wq = (it & (ng - 1)) * G;
-      lds_getv<G16, G>(lds, j * C + wq, acc);
-      for (uint32_t bit = 1; bit < n; bit <<= 1) {
-        if ((j & bit) || j + bit >= n) continue;
-        Word<G16> y[G];
-        lds_getv<G16, G>(lds, (j + bit) * C + wq, y);
+  if (n >= 4) {
+    formal_derivative<G16, G>(a, lds);
+  } else {
+    // formal derivative (decode.rs:537-551): in closed form every lane j takes
+    // lane j ^ XOR_{b : bit b of j clear, j + 2^b < n} lane (j + 2^b), all of
+    // them before the update.
+    for (uint32_t it0 = 0; it0 < n * ng; it0 += blockDim.x) {
+      const uint32_t it = it0 + threadIdx.x;
+      Word<G16> acc[G];
+      const bool live = it < n * ng;
+      uint32_t j = 0, wq = 0;
+      if (live) {
+        j = it >> lg_ng, wq = (it & (ng - 1)) * G;
+        lds_getv<G16, G>(lds, j * C + wq, acc);
+        for (uint32_t bit = 1; bit < n; bit <<= 1) {
+          if ((j & bit) || j + bit >= n) continue;
+          Word<G16> y[G];
+          lds_getv<G16, G>(lds, (j + bit) * C + wq, y);
 #pragma unroll
-        for (int q = 0; q < G; ++q) wxor(acc[q], y[q]);
+          for (int q = 0; q < G; ++q) wxor(acc[q], y[q]);
+        }
       }
+      __syncthreads();
+      if (live) lds_putv<G16, G>(lds, j * C + wq, acc);
+      __syncthreads();
     }
-    __syncthreads();
-    if (live) lds_putv<G16, G>(lds, j * C + wq, acc);
-    __syncthreads();
   }
   run_layers<G16, G, PL>(lds, a.fft_layer0, a.fft_layers, a, P);
   for (uint32_t it = threadIdx.x; it < (a.k + a.parity) * ng; it += blockDim.x) {

@@ -77,8 +77,8 @@ def test_encode_long_shards_match_oracle(bits, k, m, length):
     assert rs.verify(sh)
 
 
-@pytest.mark.parametrize("bits,k,m", [(8, 10, 4), (8, 12, 7), (8, 128, 64), (16, 10, 4),
-                                      (16, 33, 17), (16, 300, 45)])
+@pytest.mark.parametrize("bits,k,m", [(8, 1, 1), (16, 1, 1), (8, 2, 1), (8, 10, 4), (8, 12, 7),
+                                      (8, 128, 64), (16, 10, 4), (16, 33, 17), (16, 300, 45)])
 def test_reconstruct_random_patterns_match_oracle(bits, k, m):
     announce(f"leopard gf{bits} reconstruct {k}+{m}")
     n = 256
