@@ -216,7 +216,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
     }
   }
   if constexpr (MODE == kVerify) {
-    if (mismatch) verify_report(a.clear_ws, a.call_id, a.flag);
+    if (wave_leader(mismatch)) verify_report(a.clear_ws, a.call_id, a.flag);
   }
 }
 
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kDwordLanes) void apply_dword_kernel(const ApplyArg
     }
   }
   if constexpr (MODE == kVerify) {
-    if (mismatch) verify_report(a.clear_ws, a.call_id, a.flag);
+    if (wave_leader(mismatch)) verify_report(a.clear_ws, a.call_id, a.flag);
   }
 }
 

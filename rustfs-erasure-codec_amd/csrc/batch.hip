@@ -166,7 +166,7 @@ __device__ __forceinline__ void finish(const Cursor& c, uint32_t (&acc)[RMAX][NV
       }
     }
   }
-  if (mismatch) atomicOr(stripes_g[c.stripe].flag, 1);
+  if (wave_leader(mismatch)) raise_flag(stripes_g[c.stripe].flag);
 #pragma unroll
   for (int r = 0; r < RMAX; ++r)
 #pragma unroll

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void apply_bytes_kernel(const ByteArgs a) {
         if (a.tower) mismatch |= dst[1] != accc[r];
       }
     }
-    if (mismatch) atomicOr(a.flag, 1);
+    if (wave_leader(mismatch)) raise_flag(a.flag);
   }
 }
 
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void compare_rows_kernel(const CompareArgs 
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < c.len;
        i += (uint64_t)gridDim.x * kBlock)
     bad |= a[i] != b[i];
-  if (bad) atomicOr(c.flag, 1);
+  if (wave_leader(bad)) raise_flag(c.flag);
 }
 
 hipError_t launch_compare_rows(const CompareArgs& c, hipStream_t s) {

@@ -527,11 +527,12 @@ static int encode_sep_impl(const Codec& c, const rsgpu_cslice* data, size_t nd,
 }
 
 // ------------------------------------------------------------ verify tickets
-// Per (device, stream): the two-word device workspace of the in-kernel flag clear
-// and the stream's next call id.  Work on one stream runs in order, and the
-// slot's mutex is held from taking an id until the call's launches are enqueued,
-// so ids reach the device in increasing order.  Workspaces come from 16 KiB
-// device chunks zeroed once, kept for the life of the process.
+// Per (device, stream): the device workspace of the in-kernel flag protocol
+// (words: last id claimed, cleared, reported; one padding word) and the
+// stream's next call id.  Work on one stream runs in order, and the slot's mutex
+// is held from taking an id until the call's launches are enqueued, so ids reach
+// the device in increasing order.  Workspaces come from 32 KiB device chunks
+// zeroed once, kept for the life of the process.
 struct VerifySlot {
   std::mutex mu;
   uint64_t* ws = nullptr;
@@ -562,13 +563,13 @@ static VerifySlot* verify_slot(hipStream_t s) {
     if (std::get<0>(ch) == device && std::get<2>(ch) < kVerifyChunkSlots) chunk = &ch;
   if (!chunk) {
     void* p = nullptr;
-    if (hipMalloc(&p, kVerifyChunkSlots * 16) != hipSuccess) return nullptr;
-    if (hipMemset(p, 0, kVerifyChunkSlots * 16) != hipSuccess) return nullptr;  // once per chunk
+    if (hipMalloc(&p, kVerifyChunkSlots * 32) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, kVerifyChunkSlots * 32) != hipSuccess) return nullptr;  // once per chunk
     g_verify_chunks.emplace_back(device, (uint64_t*)p, 0);
     chunk = &g_verify_chunks.back();
   }
   auto slot = std::make_unique<VerifySlot>();
-  slot->ws = std::get<1>(*chunk) + 2 * std::get<2>(*chunk)++;
+  slot->ws = std::get<1>(*chunk) + 4 * std::get<2>(*chunk)++;  // words: claimed, cleared, reported
   VerifySlot* out = slot.get();
   g_verify_slots.emplace(key, std::move(slot));
   t_device = device, t_stream = s, t_slot = out;

@@ -47,13 +47,16 @@ __device__ __forceinline__ uint32_t gmul(const uint4& q, uint32_t t4, const Sel&
 
 // The verify flag of one call, cleared and reported by the same launch.  Every
 // launch of a verify call carries the call's id (ids grow per stream) and a
-// two-word workspace: ws[0] the last call claimed, ws[1] the last call whose
-// flag is cleared.  The lane that claims the id first clears the flag and
-// publishes the id.  Workgroup (0, 0) claims at its start, so a codeword's flag
-// ends cleared; a lane that finds a mismatch claims too if the id is not yet
-// published, then waits for it before ORing into the flag.  A waiting lane waits
+// workspace of words: ws[0] the last call claimed, ws[1] the last call whose
+// flag is cleared, ws[2] the last call whose flag a reporter has taken on.  The
+// lane that claims the id first clears the flag and publishes the id.
+// Workgroup (0, 0) claims at its start, so a codeword's flag ends cleared.  A
+// mismatch is reported once per call: the first reporter (the one that raises
+// ws[2] to the id; the others read it and stop) claims too if the id is not yet
+// published, waits for it, and then ORs into the flag.  A waiting lane waits
 // only on the lane that won the claim, which has started and finishes the clear
-// without waiting on anything.
+// without waiting on anything.  Only that first reporter pays for the acquire
+// (on gfx950 an agent-scope acquire invalidates the L2 of the XCD).
 __device__ __forceinline__ void verify_try_clear(uint64_t* ws, uint64_t id, int32_t* flag) {
   if (__hip_atomic_fetch_max(ws, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < id) {
     __hip_atomic_store(flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -63,8 +66,30 @@ __device__ __forceinline__ void verify_try_clear(uint64_t* ws, uint64_t id, int3
 __device__ __forceinline__ void verify_claim(uint64_t* ws, uint64_t id, int32_t* flag) {
   if (ws && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) verify_try_clear(ws, id, flag);
 }
+// True in one lane of the wave: the lowest active lane with pred set.  A failing
+// compare then reports once per wave, not once per lane.
+__device__ __forceinline__ bool wave_leader(bool pred) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(pred);
+  const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  return m != 0 && lane == (uint32_t)__builtin_ctzll(m);
+}
+// Sets a flag the caller cleared before the launch (stream order): a reporter
+// that reads it set leaves it, so a failing call makes few atomics on it.
+__device__ __forceinline__ void raise_flag(int32_t* flag) {
+  if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Raises the flag of the one-launch protocol (ws null: the caller cleared it).
+// ws[2] only grows, so a stale read of it is smaller and costs a fetch_max.
 __device__ __forceinline__ void verify_report(uint64_t* ws, uint64_t id, int32_t* flag) {
-  if (ws && __hip_atomic_load(ws + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < id) {
+  if (!ws) {
+    raise_flag(flag);
+    return;
+  }
+  if (__hip_atomic_load(ws + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= id ||
+      __hip_atomic_fetch_max(ws + 2, id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= id)
+    return;
+  if (__hip_atomic_load(ws + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < id) {
     verify_try_clear(ws, id, flag);
     while (__hip_atomic_load(ws + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < id) __builtin_amdgcn_s_sleep(2);
   }

@@ -885,7 +885,7 @@ __device__ __forceinline__ void leo_encode_body(const LeoArgs& a, uint32_t* lds,
         if (w0 + wq + q < a.words) mismatch |= !wequal<G16>(v[q], s[q]);
     }
   }
-  if (mismatch) atomicOr(a.flag, 1);
+  if (wave_leader(mismatch)) raise_flag(a.flag);
 }
 
 template <bool G16, int NT, int MINW = NT / 256>

@@ -87,6 +87,9 @@ class OpCase:
             self.args = (Slice * n)(*[Slice(p, shard_len) for p in ptrs])
         elif self.op == 1:
             self.args = (CSlice * n)(*[CSlice(p, shard_len) for p in ptrs])
+            # verify is timed on a codeword (the passing case a store mostly sees)
+            enc = (Slice * n)(*[Slice(p, shard_len) for p in ptrs])
+            _check(lib().rsgpu_encode(rs._h, enc, n, _stream(None)))
         else:
             self.args = (Slot * n)(*[Slot(p, shard_len, 1) for p in ptrs])
         self.n = n
@@ -110,8 +113,10 @@ class OpCase:
         return [v for v in range(variant_count()) if self.rs.variant_valid(self.op, e, v)]
 
     def time(self, variants, reps=None, rounds=2):
-        """Device microseconds per call for each variant (-1 = default tiling):
-        the mean over `reps` event-bracketed calls, minimum over interleaved rounds."""
+        """Device microseconds per call for each variant (-1 = default tiling): `reps`
+        calls back to back between two events, as a caller issuing one call per
+        stripe runs them (launch gaps included), per call; minimum over
+        interleaved rounds."""
         torch = self.torch
         if reps is None:
             self.rs.set_variant(-1)
@@ -124,19 +129,17 @@ class OpCase:
             est = max(a.elapsed_time(b) * 1000.0, 1.0)
             reps = int(min(40, max(3, 300.0 / est)))
         best = {v: float("inf") for v in variants}
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(reps)]
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         for _ in range(rounds):
             for v in variants:
                 self.rs.set_variant(v)
                 self.run()  # warm-up (decode-row cache, matrix-path codec)
-                for a, b in ev:
-                    a.record()
+                a.record()
+                for _ in range(reps):
                     self.run()
-                    b.record()
-                ev[-1][1].synchronize()
-                t = sum(a.elapsed_time(b) for a, b in ev) * 1000.0 / reps
-                best[v] = min(best[v], t)
+                b.record()
+                b.synchronize()
+                best[v] = min(best[v], a.elapsed_time(b) * 1000.0 / reps)
         self.rs.set_variant(-1)
         return best
 
