@@ -284,7 +284,12 @@ def leopard_section(torch, rsgpu, DSP, disp):
         rs = rsgpu.ReedSolomon(k, m, codec_family=fam)
         rows = {}
         for S in (4096, 65536, 262144):
-            for op, e in (("encode", 0), ("reconstruct", m)):
+            # m erasures, and on GF(2^16) at 256 KiB also 1 and 8 (few lost shards,
+            # where the decoder-matrix variants compete with the FFT)
+            extra = ((1, "reconstruct_1_erasure"), (8, "reconstruct_8_erasures")) \
+                if fam == "leopard_gf16" and S == 262144 else ()
+            for op, e, label in (("encode", 0, "encode"), ("reconstruct", m, "reconstruct"),
+                                 *[("reconstruct", x, lab) for x, lab in extra]):
                 case = DSP.OpCase(torch, rs, fam, k, m, S, op, e, seed=S + e)
                 rs.set_variant_table(None)
                 t_def = case.time([-1])[-1]
@@ -292,7 +297,7 @@ def leopard_section(torch, rsgpu, DSP, disp):
                 v = rs.variant_for(op, e, S)
                 t_dsp = case.time([-1])[-1]
                 rs.set_variant_table(None)
-                rows[f"{op}_{S // 1024}KiB"] = {
+                rows[f"{label}_{S // 1024}KiB"] = {
                     "default_gbs": round(k * S / t_def / 1e3, 2),
                     "dispatched_gbs": round(k * S / t_dsp / 1e3, 2),
                     "dispatched_variant": v, "dispatched_us": round(t_dsp, 2)}

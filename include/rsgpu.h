@@ -445,7 +445,10 @@ int rsgpu_batch_stats(const rsgpu_batch* batch, uint64_t* n_ops, uint64_t* bytes
  *           on a Leopard GF(2^8) codec this applies its generator matrix
  *           (encode, verify, reconstruct with exactly k shards present) or, for
  *           reconstruct with more present, the FFT decoder's own matrix over
- *           every present shard (cached per erasure pattern);
+ *           every present shard (cached per erasure pattern); on a Leopard
+ *           GF(2^16) codec, reconstruct of at most 32 erased shards only,
+ *           through the FFT decoder's own matrix (split-layout elements, 16-byte
+ *           aligned shards; others take the FFT);
  *   kind 1: Leopard FFT with a column tile of `leopard_tile_words` words.
  * A codec uses its forced variant if set, else its variant table, else the
  * default tiling; an entry that is not valid for a call falls back to the

@@ -33,7 +33,7 @@ N_TILES = 5
 def sources():
     units = [(f, f.replace(".hip", ".o"), []) for f in sorted(os.listdir(CSRC))
              if f.endswith(".hip") and f != INST]
-    for tower in (0, 1):
+    for tower in (0, 1, 2):
         for tile in range(N_TILES):
             units.append((INST, f"apply_t{tile}_w{tower}.o",
                           [f"-DAPPLY_TILE={tile}", f"-DAPPLY_TOWER={tower}"]))

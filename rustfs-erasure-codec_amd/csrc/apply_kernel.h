@@ -29,14 +29,19 @@ constexpr unsigned kWaitVm16 = (1u << 14) | (7u << 4) | (15u << 8);
 
 // LDS bytes of one launch: the 16-byte table chunks of every (input, output)
 // pair, rounded up to whole staging passes of LB chunks.
-template <int LB, int R, bool TOWER>
+template <int LB, int R, int ELEM>
 size_t apply_lds_bytes(int k) {
-  const size_t chunks = (size_t)k * R * (TOWER ? 4 * kTabWords : kTabWords) / 4;
+  const size_t chunks = (size_t)k * R * (ELEM != 0 ? 4 * kTabWords : kTabWords) / 4;
   return (chunks + LB - 1) / LB * LB * 16;
 }
 
-template <int LB, int R, int NV, int MODE, bool TOWER>
+// ELEM: 0 GF(2^8) bytes; 1 GF(2^16) tower elements as interleaved [x, c] byte
+// pairs; 2 GF(2^16) elements in the split layout of Leopard shards (each 64-byte
+// block holds the low bytes of 32 elements, then their high bytes), applied with
+// the same four byte maps per coefficient, low bytes as the first plane.
+template <int LB, int R, int NV, int MODE, int ELEM>
 __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
+  constexpr bool TOWER = ELEM != 0, SPLIT = ELEM == 2;
   constexpr int W = TOWER ? 4 * kTabWords : kTabWords;  // table dwords per coefficient
   // Inputs per load group: every lane issues KB*NV 128-bit loads back to back
   // before the arithmetic (64 VGPRs of input in flight for every NV).
@@ -85,7 +90,9 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
     ok[j] = u < a.n16;
     // a lane past the end loads the last vector instead (in bounds, unused), so
     // the loads need no per-lane branch
-    v[j] = (a.v_begin + (ok[j] ? u : a.n16 - 1)) * 16;  // byte offset
+    const uint64_t vi = a.v_begin + (ok[j] ? u : a.n16 - 1);
+    // byte offset of the vector (split: of its 8 low bytes, the high bytes 32 on)
+    v[j] = SPLIT ? ((vi >> 2) << 6) + ((vi & 3) << 3) : vi * 16;
   }
   uint32_t acc[R][NV][4];
 #pragma unroll
@@ -111,7 +118,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
 #pragma unroll
-      for (int j = 0; j < NV; ++j) x[b][j] = load16(p[b] + v[j]);
+      for (int j = 0; j < NV; ++j) x[b][j] = SPLIT ? load_split(p[b] + v[j]) : load16(p[b] + v[j]);
       __builtin_amdgcn_sched_barrier(0);  // keep the issue order
     }
   };
@@ -138,9 +145,9 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
         uint32_t w[4];
-        if constexpr (TOWER) {
+        if constexpr (TOWER && !SPLIT) {
           deinterleave(x[b][j], w);
-        } else {
+        } else {  // split vectors arrive as planes already
           w[0] = x[b][j].x;
           w[1] = x[b][j].y;
           w[2] = x[b][j].z;
@@ -187,7 +194,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
     uint4 o[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      if constexpr (TOWER) {
+      if constexpr (TOWER && !SPLIT) {
         o[r] = interleave(acc[r][j]);
       } else {
         o[r] = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
@@ -196,19 +203,26 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
     if constexpr (MODE == kWrite) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (r < rn) store16(a.out[r0 + r] + v[j], o[r]);
+        if (r < rn) {
+          if constexpr (SPLIT) store_split(a.out[r0 + r] + v[j], o[r]);
+          else store16(a.out[r0 + r] + v[j], o[r]);
+        }
     } else {
       // the stored outputs, all loaded before any is used (rows past rn re-read
       // row rn - 1, unused)
       uint4 p[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) p[r] = load16(a.out[r0 + (r < rn ? r : rn - 1)] + v[j]);
+      for (int r = 0; r < R; ++r) {
+        const uint8_t* src = a.out[r0 + (r < rn ? r : rn - 1)] + v[j];
+        p[r] = SPLIT ? load_split(src) : load16(src);
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (r >= rn) continue;
         if constexpr (MODE == kAccumulate) {
-          store16(a.out[r0 + r] + v[j], make_uint4(p[r].x ^ o[r].x, p[r].y ^ o[r].y, p[r].z ^ o[r].z,
-                                                  p[r].w ^ o[r].w));
+          const uint4 q = make_uint4(p[r].x ^ o[r].x, p[r].y ^ o[r].y, p[r].z ^ o[r].z, p[r].w ^ o[r].w);
+          if constexpr (SPLIT) store_split(a.out[r0 + r] + v[j], q);
+          else store16(a.out[r0 + r] + v[j], q);
         } else {
           mismatch |= ((p[r].x ^ o[r].x) | (p[r].y ^ o[r].y) | (p[r].z ^ o[r].z) | (p[r].w ^ o[r].w)) != 0;
         }
@@ -336,7 +350,7 @@ static hipError_t launch_apply_dword_t(const ApplyArgs& a, hipStream_t s) {
   const uint64_t n4 = a.n16 * 4;
   const uint64_t tiles = (n4 + kDwordLanes - 1) / kDwordLanes;
   const unsigned gy = (unsigned)((a.r + R - 1) / R);
-  const size_t lds = apply_lds_bytes<kDwordLanes, R, false>(a.k);
+  const size_t lds = apply_lds_bytes<kDwordLanes, R, 0>(a.k);
   for (uint64_t t0 = 0; t0 < tiles; t0 += kMaxTiles) {
     ApplyArgs part = a;  // v_begin and n16 stay in 16-byte vectors: kMaxTiles * 256 is a multiple of 4
     part.v_begin = a.v_begin + t0 * kDwordLanes / 4;
@@ -347,45 +361,45 @@ static hipError_t launch_apply_dword_t(const ApplyArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int LB, int R, int NV, int MODE, bool TOWER>
+template <int LB, int R, int NV, int MODE, int ELEM>
 static hipError_t launch_apply_t(const ApplyArgs& a, hipStream_t s) {
   // one workgroup per tile; launches of at most 2^30 tiles, each shifted by v_begin
   constexpr uint64_t kMaxTiles = 1ull << 30;
   const uint64_t tile = (uint64_t)LB * NV;
   const uint64_t tiles = (a.n16 + tile - 1) / tile;
   const unsigned gy = (unsigned)((a.r + R - 1) / R);
-  const size_t lds = apply_lds_bytes<LB, R, TOWER>(a.k);
+  const size_t lds = apply_lds_bytes<LB, R, ELEM>(a.k);
   for (uint64_t t0 = 0; t0 < tiles; t0 += kMaxTiles) {
     ApplyArgs part = a;
     part.v_begin = a.v_begin + t0 * tile;
     part.n16 = std::min<uint64_t>(a.n16 - t0 * tile, kMaxTiles * tile);
     const unsigned gx = (unsigned)std::min<uint64_t>(tiles - t0, kMaxTiles);
-    hipLaunchKernelGGL((apply_kernel<LB, R, NV, MODE, TOWER>), dim3(gx, gy), dim3(LB), lds, s, part);
+    hipLaunchKernelGGL((apply_kernel<LB, R, NV, MODE, ELEM>), dim3(gx, gy), dim3(LB), lds, s, part);
   }
   return hipGetLastError();
 }
 
-template <int LB, int NV, int MODE, bool TOWER>
+template <int LB, int NV, int MODE, int ELEM>
 static hipError_t launch_apply_rows(const ApplyArgs& a, int rows, hipStream_t s) {
   switch (rows) {
-    case 1: return launch_apply_t<LB, 1, NV, MODE, TOWER>(a, s);
-    case 2: return launch_apply_t<LB, 2, NV, MODE, TOWER>(a, s);
-    case 3: return launch_apply_t<LB, 3, NV, MODE, TOWER>(a, s);
-    case 4: return launch_apply_t<LB, 4, NV, MODE, TOWER>(a, s);
-    case 5: return launch_apply_t<LB, 5, NV, MODE, TOWER>(a, s);
-    case 6: return launch_apply_t<LB, 6, NV, MODE, TOWER>(a, s);
-    case 7: return launch_apply_t<LB, 7, NV, MODE, TOWER>(a, s);
-    case 8: return launch_apply_t<LB, 8, NV, MODE, TOWER>(a, s);
+    case 1: return launch_apply_t<LB, 1, NV, MODE, ELEM>(a, s);
+    case 2: return launch_apply_t<LB, 2, NV, MODE, ELEM>(a, s);
+    case 3: return launch_apply_t<LB, 3, NV, MODE, ELEM>(a, s);
+    case 4: return launch_apply_t<LB, 4, NV, MODE, ELEM>(a, s);
+    case 5: return launch_apply_t<LB, 5, NV, MODE, ELEM>(a, s);
+    case 6: return launch_apply_t<LB, 6, NV, MODE, ELEM>(a, s);
+    case 7: return launch_apply_t<LB, 7, NV, MODE, ELEM>(a, s);
+    case 8: return launch_apply_t<LB, 8, NV, MODE, ELEM>(a, s);
   }
   return hipErrorInvalidValue;
 }
 
-template <int LB, int NV, bool TOWER>
+template <int LB, int NV, int ELEM>
 static hipError_t launch_apply_mode(const ApplyArgs& a, int mode, int rows, hipStream_t s) {
   switch (mode) {
-    case kWrite: return launch_apply_rows<LB, NV, kWrite, TOWER>(a, rows, s);
-    case kAccumulate: return launch_apply_rows<LB, NV, kAccumulate, TOWER>(a, rows, s);
-    case kVerify: return launch_apply_rows<LB, NV, kVerify, TOWER>(a, rows, s);
+    case kWrite: return launch_apply_rows<LB, NV, kWrite, ELEM>(a, rows, s);
+    case kAccumulate: return launch_apply_rows<LB, NV, kAccumulate, ELEM>(a, rows, s);
+    case kVerify: return launch_apply_rows<LB, NV, kVerify, ELEM>(a, rows, s);
   }
   return hipErrorInvalidValue;
 }

@@ -65,25 +65,28 @@ __global__ void fill_i32_kernel(int32_t* p, int32_t v, int n) {
   hipError_t launch_apply_t##t##_w##w(const ApplyArgs& a, int mode, int rows, hipStream_t s);
 RSGPU_DECL(0, 0) RSGPU_DECL(1, 0) RSGPU_DECL(2, 0) RSGPU_DECL(3, 0) RSGPU_DECL(4, 0)
 RSGPU_DECL(0, 1) RSGPU_DECL(1, 1) RSGPU_DECL(2, 1) RSGPU_DECL(3, 1) RSGPU_DECL(4, 1)
+RSGPU_DECL(0, 2) RSGPU_DECL(1, 2) RSGPU_DECL(2, 2) RSGPU_DECL(3, 2) RSGPU_DECL(4, 2)
 #undef RSGPU_DECL
 
 hipError_t launch_apply_dword(const ApplyArgs& a, int mode, int rows, hipStream_t s);
 
 using ApplyFn = hipError_t (*)(const ApplyArgs&, int, int, hipStream_t);
-static const ApplyFn kApplyFns[2][kNumTiles] = {
+static const ApplyFn kApplyFns[3][kNumTiles] = {
     {launch_apply_t0_w0, launch_apply_t1_w0, launch_apply_t2_w0, launch_apply_t3_w0, launch_apply_t4_w0},
-    {launch_apply_t0_w1, launch_apply_t1_w1, launch_apply_t2_w1, launch_apply_t3_w1, launch_apply_t4_w1}};
+    {launch_apply_t0_w1, launch_apply_t1_w1, launch_apply_t2_w1, launch_apply_t3_w1, launch_apply_t4_w1},
+    {launch_apply_t0_w2, launch_apply_t1_w2, launch_apply_t2_w2, launch_apply_t3_w2, launch_apply_t4_w2}};
 
 hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s) {
   if (a.n16 == 0) return hipSuccess;
   const bool dword = tile == kTileDword && !tower;
-  if ((!dword && (tile < 0 || tile >= kNumTiles)) || a.r < 1 || a.r > kMaxOut || a.k < 1 || a.k > kMaxIn)
+  if ((!dword && (tile < 0 || tile >= kNumTiles)) || tower < 0 || tower > 2 || a.r < 1 || a.r > kMaxOut ||
+      a.k < 1 || a.k > kMaxIn)
     return hipErrorInvalidValue;
   // the kernel stages its tables into LDS with 16-byte direct loads
   if (((uintptr_t)a.tables & 15) != 0 || (a.tab_row_stride & 3) != 0) return hipErrorInvalidValue;
   if (rows < 1 || rows > a.r) rows = a.r;
   if (dword) return launch_apply_dword(a, mode, rows, s);
-  return kApplyFns[tower ? 1 : 0][tile](a, mode, rows, s);
+  return kApplyFns[tower][tile](a, mode, rows, s);
 }
 
 hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s) {

@@ -296,12 +296,14 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
                      const uint8_t* const* in, uint8_t* const* out,
                      size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1,
                      int table_cols = -1, int col0 = 0, const uint32_t* h_tables = nullptr,
-                     const VerifyClear* vc = nullptr) {
+                     const VerifyClear* vc = nullptr, bool leo16 = false) {
   if (table_cols < 0) table_cols = k_total;
   const int W = words_per_coef(tower);
   bool vec_ok = true;
   for (int i = 0; i < k_total; ++i) vec_ok &= aligned16(in[i]);
   for (int j = 0; j < r_total; ++j) vec_ok &= aligned16(out[j]);
+  // Leopard GF(2^16) split layout (tower tables): vector tiles only, no byte tail
+  if (leo16 && (!tower || !vec_ok || len % 16)) return RSGPU_ERR_UNSUPPORTED;
   if (mode == kVerify && k_total > kMaxIn) return RSGPU_ERR_UNSUPPORTED;  // handled by caller
   const uint64_t n16 = vec_ok ? len / 16 : 0;
   const size_t tail_begin = (size_t)n16 * 16;
@@ -352,7 +354,7 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
           a.tab_row_stride = table_cols * W;
           a.tab_col0 = col0 + k0;
         }
-        HIP_TRY(launch_apply(a, m, tile, split ? std::min(split, rn) : rn, tower ? 1 : 0, s));
+        HIP_TRY(launch_apply(a, m, tile, split ? std::min(split, rn) : rn, leo16 ? 2 : (tower ? 1 : 0), s));
       }
       if (tail_begin < len) {
         ByteArgs b{};
@@ -414,6 +416,7 @@ static int32_t* thread_flag(int device) {
 
 // ------------------------------------------------------------ variant dispatch
 constexpr int kPick = -2;  // impl argument: take the variant from the codec's dispatch state
+constexpr size_t kLeo16MatrixErasures = 32;
 
 static bool variant_valid(const Codec& c, int op, size_t erasures, int v) {
   if (v < 0 || v >= kNumVariants) return false;
@@ -421,8 +424,12 @@ static bool variant_valid(const Codec& c, int op, size_t erasures, int v) {
   if (v >= kNumApplyVariants) return true;  // Leopard FFT column tiles
   // Leopard GF(2^8): matrix variants for every operation; reconstruct applies
   // the generator-matrix solve (exactly k shards present) or the FFT decoder's
-  // own matrix (more), both bit-identical to the FFT decoder.
-  return c.family == RSGPU_FAMILY_LEOPARD_GF8;
+  // own matrix (more), both bit-identical to the FFT decoder.  Leopard GF(2^16):
+  // reconstruct of at most kLeo16MatrixErasures shards, through the FFT decoder's
+  // own matrix (its apply grows with erasures x present shards, the FFT does not:
+  // past a few dozen erasures the FFT is faster at every shape).
+  return c.family == RSGPU_FAMILY_LEOPARD_GF8 ||
+         (c.family == RSGPU_FAMILY_LEOPARD_GF16 && op == kOpReconstruct && erasures <= kLeo16MatrixErasures);
 }
 
 static int size_bucket(size_t len) {
@@ -1399,23 +1406,32 @@ int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int
 // The FFT decoder is linear over the field (constant multiplies, XORs and the
 // transforms), so for one (present, target) pattern its output is M * (present
 // shards).  M is read off one decode of unit columns: present shard p holds the
-// field's identity at byte p, so target t comes back with M[t][p] at byte p.
+// field's identity at element p, so target t comes back with M[t][p] at element p.
 // Applying M then gives the decoder's bytes for any input, codewords or not.
+// GF(2^16) elements sit in the split layout (element p of a shard: low byte at
+// (p / 32) * 64 + p % 32, high byte 32 bytes on).
 static int build_decoder_matrix(const Codec& c, const uint8_t* flags, hipStream_t s,
                                 std::shared_ptr<const DecoderMatrix>& out) {
   const size_t total = c.k + c.m;
+  const bool g16 = c.family == RSGPU_FAMILY_LEOPARD_GF16;
   auto dm = std::make_shared<DecoderMatrix>();
   for (size_t i = 0; i < total; ++i) {
     if (flags[i] & 1) dm->inputs.push_back((int)i);
     else if (flags[i] & 2) dm->targets.push_back((int)i);
   }
   const size_t np = dm->inputs.size(), nt = dm->targets.size();
-  const size_t L = std::max<size_t>(64, (np + 63) / 64 * 64);
-  std::shared_ptr<const FieldTables> ft = leopard8_field_tables();
+  const size_t L = g16 ? std::max<size_t>(64, (np + 31) / 32 * 64) : std::max<size_t>(64, (np + 63) / 64 * 64);
+  auto at = [&](size_t p) { return g16 ? (p / 32) * 64 + p % 32 : p; };  // low (or only) byte of element p
+  std::shared_ptr<const FieldTables> ft = g16 ? nullptr : leopard8_field_tables();
+  const uint16_t one = g16 ? leopard16_one() : ft->one;
   DeviceBuffer buf;
   HIP_TRY(buf.alloc(total * L));
   std::vector<uint8_t> host(total * L, 0);
-  for (size_t p = 0; p < np; ++p) host[(size_t)dm->inputs[p] * L + p] = ft->one;
+  for (size_t p = 0; p < np; ++p) {
+    uint8_t* e = host.data() + (size_t)dm->inputs[p] * L + at(p);
+    e[0] = (uint8_t)one;
+    if (g16) e[32] = (uint8_t)(one >> 8);
+  }
   HIP_TRY(hipMemcpyAsync(buf.p, host.data(), host.size(), hipMemcpyHostToDevice, s));
   std::vector<rsgpu_slot> sl(total);
   std::vector<uint8_t> req(total);
@@ -1429,9 +1445,17 @@ static int build_decoder_matrix(const Codec& c, const uint8_t* flags, hipStream_
   HIP_TRY(hipStreamSynchronize(s));
   std::vector<uint16_t> rows(nt * np);
   for (size_t t = 0; t < nt; ++t)
-    for (size_t p = 0; p < np; ++p) rows[t * np + p] = host[(size_t)dm->targets[t] * L + p];
+    for (size_t p = 0; p < np; ++p) {
+      const uint8_t* e = host.data() + (size_t)dm->targets[t] * L + at(p);
+      rows[t * np + p] = g16 ? (uint16_t)(e[0] | e[32] << 8) : e[0];
+    }
   std::vector<uint32_t> ht;
-  build_tables_host(rows.data(), (int)nt, (int)np, false, ht, ft.get());
+  if (g16) {
+    ht.resize(rows.size() * 4 * kTabWords);
+    leopard16_apply_tables(rows.data(), rows.size(), ht.data());
+  } else {
+    build_tables_host(rows.data(), (int)nt, (int)np, false, ht, ft.get());
+  }
   HIP_TRY(dm->tables.alloc(ht.size() * 4));
   HIP_TRY(hipMemcpy(dm->tables.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice));
   out = dm;
@@ -1459,8 +1483,9 @@ int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_
   std::vector<uint8_t*> out;
   for (int i : dm->inputs) in.push_back(shards[i].ptr);
   for (int t : dm->targets) out.push_back(shards[t].ptr);
-  int rc = run_apply((const uint32_t*)dm->tables.p, (int)in.size(), (int)out.size(), false, in, out, len,
-                     kWrite, nullptr, s, variant);
+  const bool g16 = c.family == RSGPU_FAMILY_LEOPARD_GF16;
+  int rc = run_apply((const uint32_t*)dm->tables.p, (int)in.size(), (int)out.size(), g16, in.data(), out.data(),
+                     len, kWrite, nullptr, s, variant, -1, 0, nullptr, nullptr, g16);
   if (rc) return rc;
   for (int t : dm->targets) shards[t].present = 1;
   return RSGPU_OK;

@@ -93,9 +93,9 @@ constexpr int kNumBuckets = 50;
 enum DispatchOp : int { kOpEncode = 0, kOpVerify = 1, kOpReconstruct = 2, kNumOps = 3 };
 
 // Launchers (classic_kernels.hip; the apply kernels are compiled from apply_inst.hip).  `tile` indexes
-// kTileShapes, `rows` is the number of outputs per workgroup.  `tower` selects
-// GF((2^8)^2) element pairs; tables are then the 2x2-expanded GF(2^8)
-// coefficients [row][input][4][kTabWords].
+// kTileShapes, `rows` is the number of outputs per workgroup.  `tower` 1 selects
+// GF((2^8)^2) element pairs, 2 the split-layout GF(2^16) elements of Leopard
+// shards; tables are then four byte maps per coefficient [row][input][4][kTabWords].
 hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s);
 hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t s);

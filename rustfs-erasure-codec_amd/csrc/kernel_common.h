@@ -108,6 +108,20 @@ __device__ __forceinline__ void store16(uint8_t* p, uint4 v) {
   v4u_t w = {v.x, v.y, v.z, v.w};
   __builtin_nontemporal_store(w, reinterpret_cast<v4u_t*>(p));
 }
+// Eight GF(2^16) elements of a split-layout shard (a 64-byte block holds the low
+// bytes of 32 elements, then their high bytes): 8 low bytes at p and their 8
+// high bytes at p + 32, as (low plane, high plane) dwords.
+typedef unsigned int v2u_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint4 load_split(const uint8_t* p) {
+  v2u_t lo = __builtin_nontemporal_load(reinterpret_cast<const v2u_t*>(p));
+  v2u_t hi = __builtin_nontemporal_load(reinterpret_cast<const v2u_t*>(p + 32));
+  return make_uint4(lo.x, lo.y, hi.x, hi.y);
+}
+__device__ __forceinline__ void store_split(uint8_t* p, uint4 v) {
+  v2u_t lo = {v.x, v.y}, hi = {v.z, v.w};
+  __builtin_nontemporal_store(lo, reinterpret_cast<v2u_t*>(p));
+  __builtin_nontemporal_store(hi, reinterpret_cast<v2u_t*>(p + 32));
+}
 
 // Tower elements [x, c] interleaved in a 16-byte vector -> planes (X0, X1, C0, C1).
 __device__ __forceinline__ void deinterleave(uint4 v, uint32_t* w) {

@@ -14,7 +14,8 @@ int leopard_verify(const Codec& c, const rsgpu_cslice* shards, size_t len, int32
                    hipStream_t s, int variant);
 // goal: 0 all, 1 data only, 2 required flags.  A matrix-apply variant on a GF(2^8)
 // codec goes to leopard_matrix_reconstruct (exactly k shards present) or to
-// leopard_decoder_matrix_reconstruct (more).
+// leopard_decoder_matrix_reconstruct (more); on a GF(2^16) codec to
+// leopard_decoder_matrix_reconstruct.
 int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
                         const uint8_t* required, size_t n_required, hipStream_t s, int variant);
 
@@ -27,10 +28,18 @@ std::shared_ptr<const FieldTables> leopard8_field_tables();
 int leopard_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
                                const uint8_t* required, size_t n_required, hipStream_t s, int variant);
 
-// Reconstruct with more than k shards present as one matrix apply over them,
-// bit-identical to the FFT decoder (codec.hip).  flags[i]: bit 0 present, bit 1
-// a target; every shard is len bytes.
+// Reconstruct as one matrix apply over the present shards, bit-identical to the
+// FFT decoder (codec.hip): GF(2^8) codecs with more than k shards present, GF(2^16)
+// codecs with any.  flags[i]: bit 0 present, bit 1 a target; every shard is len
+// bytes.
 int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, const uint8_t* flags,
                                        size_t len, hipStream_t s, int variant);
+
+// Leopard GF(2^16) field helpers of that path: the multiplicative identity, and
+// the matrix-apply tables of n coefficients for the split shard layout (four byte
+// maps each, in the kernel's plane order: low->low, high->low, low->high,
+// high->high; a zero coefficient gets zero maps).
+uint16_t leopard16_one();
+void leopard16_apply_tables(const uint16_t* coef, size_t n, uint32_t* out);
 
 }  // namespace rsgpu

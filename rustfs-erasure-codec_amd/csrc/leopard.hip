@@ -150,6 +150,32 @@ static void leo_table(const LeoField& F, uint32_t log_m, bool twiddle, uint32_t*
   pack_table(hh, out + 3 * kTabWords);
 }
 
+uint16_t leopard16_one() { return (uint16_t)field16().exp[0]; }
+
+// pack_table needs f at 20 arguments only: 0..7, 8*i, 64*i.
+void leopard16_apply_tables(const uint16_t* coef, size_t n, uint32_t* out) {
+  const LeoField& F = field16();
+  static const int kArgs[] = {1, 2, 3, 4, 5, 6, 7, 8, 16, 24, 32, 40, 48, 56, 64, 128, 192};
+  for (size_t t = 0; t < n; ++t) {
+    uint32_t* dst = out + t * kLeoTabWords16;
+    if (coef[t] == 0) {
+      std::fill(dst, dst + kLeoTabWords16, 0u);
+      continue;
+    }
+    const uint32_t lg = F.log[coef[t]];
+    uint8_t ll[256] = {}, lh[256] = {}, hl[256] = {}, hh[256] = {};
+    for (int v : kArgs) {
+      const uint32_t lo = F.mul_log((uint32_t)v, lg), hi = F.mul_log((uint32_t)v << 8, lg);
+      ll[v] = lo & 0xff, lh[v] = lo >> 8, hl[v] = hi & 0xff, hh[v] = hi >> 8;
+    }
+    // the kernel's planes: low bytes first, so x' = A x ^ B c is low = ll(low) ^ hl(high)
+    pack_table(ll, dst);
+    pack_table(hl, dst + kTabWords);
+    pack_table(lh, dst + 2 * kTabWords);
+    pack_table(hh, dst + 3 * kTabWords);
+  }
+}
+
 // ------------------------------------------------------------ transform plans
 // A butterfly block: lanes r .. r + radix * cnt, with the twiddle logs of its
 // three multiplies (radix 2: one twiddle, stored three times).  Host only: the
@@ -1644,6 +1670,14 @@ int leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
     if (present == c.k)
       return leopard_matrix_reconstruct(c, shards, n, goal, required, n_required, s, variant);
     return leopard_decoder_matrix_reconstruct(c, shards, n, flags.data(), len, s, variant);
+  }
+  // GF(2^16): the decoder's own matrix over the present shards, on the vector
+  // tiles of the split layout (16-byte aligned shards; else the FFT below)
+  if (variant >= 0 && variant < kNumApplyVariants && st.gf16) {
+    bool aligned = true;
+    for (size_t i = 0; i < total; ++i)
+      if (flags[i] & 3) aligned = aligned && ((uintptr_t)shards[i].ptr & 15) == 0;
+    if (aligned) return leopard_decoder_matrix_reconstruct(c, shards, n, flags.data(), len, s, variant);
   }
   std::vector<uint8_t> pres(total);
   for (size_t i = 0; i < total; ++i) pres[i] = shards[i].present ? 1 : 0;

@@ -184,7 +184,15 @@ def sample_specs(n, seed=0x5EED_D15C):
             op, e = "verify", 0
         else:
             op = "reconstruct"
-            e = m if (kind.startswith("leopard") and rng.random() < 0.5) else int(rng.integers(1, m + 1))
+            r = rng.random()
+            if kind.startswith("leopard") and r < 0.4:
+                e = m  # exactly k present
+            elif kind == "leopard_gf16" and r < 0.7:
+                # few erasures: where the decoder-matrix variants compete with the FFT
+                e = int(round(math.exp(rng.uniform(0.0, math.log(min(m, 32) + 0.5)))))
+                e = min(max(e, 1), m)
+            else:
+                e = int(rng.integers(1, m + 1))
         out.append((kind, k, m, size, op, e))
     return out
 

@@ -183,6 +183,40 @@ def test_decoder_matrix_reconstruct_matches_oracle(k, m, n):
                 assert work[i] is None
 
 
+@pytest.mark.parametrize("k,m,n,counts", [(10, 4, 192, None), (64, 16, 4096, None),
+                                           (300, 45, 640, (1, 2, 9, 44, 45)),
+                                           (1024, 256, 4096, (1, 3, 256))])
+def test_gf16_decoder_matrix_reconstruct_matches_oracle(k, m, n, counts):
+    """Leopard GF(2^16) matrix variants apply the FFT decoder's own matrix over
+    every present shard on the split-layout vector tiles: on non-codeword inputs
+    the bytes equal the oracle's decoder for each erasure count (up to m, i.e.
+    exactly k present), for reconstruct_data, and again from the cached matrix."""
+    announce(f"leopard gf16 decoder matrix {k}+{m} len={n}")
+    rng = np.random.default_rng(k * 100 + m + 16)
+    shards = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
+    rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf16")
+    assert rs.variant_valid("reconstruct", 1, 5) and not rs.variant_valid("encode", 0, 5)
+    for e in (counts or range(1, m + 1)):
+        rs.set_variant(5 if e % 2 else 0)  # two matrix-apply tiles
+        missing = sorted(rng.choice(k + m, e, replace=False).tolist())
+        present = [0 if i in missing else 1 for i in range(k + m)]
+        rc, exp = O.leo_reconstruct(16, k, m, [s.copy() for s in shards], present)
+        assert rc == 0
+        for rep in range(2):  # the second call uses the cached matrix
+            work = [None if i in missing else to_dev(shards[i]) for i in range(k + m)]
+            rs.reconstruct(work)
+            for i in missing:
+                assert (to_host(work[i]) == exp[i]).all(), (e, missing, i, rep)
+        work = [None if i in missing else to_dev(shards[i]) for i in range(k + m)]
+        rs.reconstruct_data(work)
+        for i in missing:
+            if i < k:
+                assert (to_host(work[i]) == exp[i]).all(), (e, missing, i)
+            else:
+                assert work[i] is None
+    rs.set_variant(-1)
+
+
 def test_reconstruct_data_some_and_errors():
     announce("leopard reconstruct_data / errors")
     k, m, n = 8, 4, 128
