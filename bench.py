@@ -61,6 +61,11 @@ def parse():
     ap.add_argument("--leopard-case", nargs=4, metavar=("FAMILY", "K", "M", "SHARD"),
                     help="one Leopard shape (e.g. leopard_gf8 10 4 1048576): encode and "
                          "reconstruct with m erasures, default kernel choice")
+    ap.add_argument("--leopard-erasures", type=int, default=None, metavar="E",
+                    help="with --leopard-case: reconstruct with E erasures instead of m, "
+                         "also with the matrix-apply variant (--leopard-variant)")
+    ap.add_argument("--leopard-variant", type=int, default=-1,
+                    help="with --leopard-case: the forced kernel variant of the timed calls")
     ap.add_argument("--mfma-only", action="store_true",
                     help="only the int8 matrix-core vs VALU encode comparison")
     ap.add_argument("--host-only", action="store_true",
@@ -488,10 +493,14 @@ def main():
         torch.cuda.init()
         fam, k, m, S = args.leopard_case[0], *map(int, args.leopard_case[1:])
         rs = rsgpu.ReedSolomon(k, m, codec_family=fam)
+        e_rec = m if args.leopard_erasures is None else args.leopard_erasures
+        v = args.leopard_variant
         out = {}
-        for op, e in (("encode", 0), ("reconstruct", m)):
-            t = DSP.OpCase(torch, rs, fam, k, m, S, op, e, seed=1).time([-1])[-1]
-            out[op] = {"us": round(t, 1), "data_gbs": round(k * S / t / 1e3, 1)}
+        for op, e in (("encode", 0), ("reconstruct", e_rec)):
+            vv = v if rs.variant_valid(op, e, v) else -1
+            t = DSP.OpCase(torch, rs, fam, k, m, S, op, e, seed=1).time([vv])[vv]
+            out[op] = {"us": round(t, 1), "data_gbs": round(k * S / t / 1e3, 1), "erasures": e,
+                       "variant": vv}
         print(json.dumps({"leopard_case": {"family": fam, "k": k, "m": m, "shard": S, **out}}))
         return
     if args.host_only:

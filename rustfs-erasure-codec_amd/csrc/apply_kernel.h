@@ -54,6 +54,10 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   const int rn = a.r - r0 < R ? a.r - r0 : R;
   const uint64_t tile = (uint64_t)LB * NV;
   if constexpr (MODE == kVerify) verify_claim(a.clear_ws, a.call_id, a.flag);
+  // inputs of this workgroup: all of a.k, or chunk blockIdx.z of a split over inputs
+  const bool splitk = a.in_dev != nullptr;
+  const int kz = splitk ? min(a.k, a.k_total - (int)blockIdx.z * a.k) : a.k;
+  const int col0 = a.tab_col0 + (splitk ? (int)blockIdx.z * a.k : 0);
 
   // Table staging: 16-byte chunks go straight from global memory into LDS
   // (global_load_lds, no registers), issued before the shard loads so that both
@@ -62,7 +66,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   // output re-read row rn - 1, so every address stays inside the tables.
   {
     constexpr int CW = W / 4;  // chunks per coefficient table
-    const int n_chunks = a.k * R * CW;
+    const int n_chunks = kz * R * CW;
     // tables in device memory, or inline in this launch's kernel arguments
     const uint32_t* tab = a.tables ? a.tables
                                    : (const uint32_t*)((const char*)__builtin_amdgcn_kernarg_segment_ptr() +
@@ -74,7 +78,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
       const int blk = c / CW, part = c % CW, i = blk / R, r = blk % R;
       const int rr = r < rn ? r : rn - 1;
       const uint32_t* src =
-          tab + (size_t)(r0 + rr) * a.tab_row_stride + (size_t)(a.tab_col0 + i) * W + part * 4;
+          tab + (size_t)(r0 + rr) * a.tab_row_stride + (size_t)(col0 + i) * W + part * 4;
       __builtin_amdgcn_global_load_lds(src, s_tab + (size_t)(c0 + wave * 64) * 4, 16, 0, 0);
     }
   }
@@ -110,11 +114,17 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   // b's arithmetic waits for its own loads only while the later ones fly.
   auto load_group = [&](int i0) __attribute__((always_inline)) {
     const uint8_t* p[KB];
-    const uint8_t* last = a.in[a.k - 1];
+    if (splitk) {  // this chunk's pointers from the device array, past its end the last
+      const uint8_t* const* ip = a.in_dev + (size_t)blockIdx.z * a.k;
 #pragma unroll
-    for (int b = 0; b < KB; ++b) p[b] = a.in[i0 + b];  // i0 + b < kMaxIn: inside ApplyArgs::in
+      for (int b = 0; b < KB; ++b) p[b] = ip[i0 + b < kz ? i0 + b : kz - 1];
+    } else {
+      const uint8_t* last = a.in[a.k - 1];
 #pragma unroll
-    for (int b = 0; b < KB; ++b) p[b] = i0 + b < a.k ? p[b] : last;
+      for (int b = 0; b < KB; ++b) p[b] = a.in[i0 + b];  // i0 + b < kMaxIn: inside ApplyArgs::in
+#pragma unroll
+      for (int b = 0; b < KB; ++b) p[b] = i0 + b < a.k ? p[b] : last;
+    }
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
 #pragma unroll
@@ -130,7 +140,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   auto compute_group = [&](int i0) __attribute__((always_inline)) {
 #pragma unroll
     for (int b = 0; b < KB; ++b) {
-      if (i0 + b >= a.k) continue;  // uniform; `continue` keeps the loop fully unrollable
+      if (i0 + b >= kz) continue;  // uniform; `continue` keeps the loop fully unrollable
       const uint32_t* tcol = s_tab + (size_t)(i0 + b) * R * W;
       uint4 tq[R][NT];
       uint32_t t4[R][NT];
@@ -182,12 +192,19 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_sched_barrier(0);
   compute_group(0);
-  for (int i0 = KB; i0 < a.k; i0 += KB) {
+  for (int i0 = KB; i0 < kz; i0 += KB) {
     load_group(i0);
     compute_group(i0);
   }
 
   bool mismatch = false;
+  // a split over inputs writes partial rows (write mode only)
+  uint8_t* outp[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int row = r0 + (r < rn ? r : rn - 1);
+    outp[r] = splitk ? a.part + ((size_t)blockIdx.z * a.r + row) * a.part_pitch : a.out[row];
+  }
 #pragma unroll
   for (int j = 0; j < NV; ++j) {
     if (!ok[j]) continue;
@@ -204,8 +221,8 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (r < rn) {
-          if constexpr (SPLIT) store_split(a.out[r0 + r] + v[j], o[r]);
-          else store16(a.out[r0 + r] + v[j], o[r]);
+          if constexpr (SPLIT) store_split(outp[r] + v[j], o[r]);
+          else store16(outp[r] + v[j], o[r]);
         }
     } else {
       // the stored outputs, all loaded before any is used (rows past rn re-read
@@ -213,7 +230,7 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
       uint4 p[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const uint8_t* src = a.out[r0 + (r < rn ? r : rn - 1)] + v[j];
+        const uint8_t* src = outp[r] + v[j];
         p[r] = SPLIT ? load_split(src) : load16(src);
       }
 #pragma unroll
@@ -221,8 +238,8 @@ __global__ __launch_bounds__(LB) void apply_kernel(const ApplyArgs a) {
         if (r >= rn) continue;
         if constexpr (MODE == kAccumulate) {
           const uint4 q = make_uint4(p[r].x ^ o[r].x, p[r].y ^ o[r].y, p[r].z ^ o[r].z, p[r].w ^ o[r].w);
-          if constexpr (SPLIT) store_split(a.out[r0 + r] + v[j], q);
-          else store16(a.out[r0 + r] + v[j], q);
+          if constexpr (SPLIT) store_split(outp[r] + v[j], q);
+          else store16(outp[r] + v[j], q);
         } else {
           mismatch |= ((p[r].x ^ o[r].x) | (p[r].y ^ o[r].y) | (p[r].z ^ o[r].z) | (p[r].w ^ o[r].w)) != 0;
         }
@@ -374,7 +391,8 @@ static hipError_t launch_apply_t(const ApplyArgs& a, hipStream_t s) {
     part.v_begin = a.v_begin + t0 * tile;
     part.n16 = std::min<uint64_t>(a.n16 - t0 * tile, kMaxTiles * tile);
     const unsigned gx = (unsigned)std::min<uint64_t>(tiles - t0, kMaxTiles);
-    hipLaunchKernelGGL((apply_kernel<LB, R, NV, MODE, ELEM>), dim3(gx, gy), dim3(LB), lds, s, part);
+    const unsigned gz = a.in_dev ? (unsigned)((a.k_total + a.k - 1) / a.k) : 1u;
+    hipLaunchKernelGGL((apply_kernel<LB, R, NV, MODE, ELEM>), dim3(gx, gy, gz), dim3(LB), lds, s, part);
   }
   return hipGetLastError();
 }

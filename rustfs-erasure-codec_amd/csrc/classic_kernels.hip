@@ -98,6 +98,50 @@ hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The fold of a split over inputs: out[row] (^)= XOR_z part[z][row], 16 bytes
+// per lane per step, grid-stride over rows x len.
+struct XorPartials {
+  const uint8_t* part;
+  uint64_t pitch, n16;
+  uint8_t* out[kMaxOut];
+  int32_t chunks, rows, accumulate;
+};
+
+__global__ __launch_bounds__(kBlock) void xor_partials_kernel(const XorPartials x) {
+  const uint64_t total = x.n16 * (uint64_t)x.rows;
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (uint64_t)gridDim.x * kBlock) {
+    const int row = (int)(t / x.n16);
+    const uint64_t off = (t % x.n16) * 16;
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    for (int z = 0; z < x.chunks; ++z) {
+      const uint4 q = load16(x.part + ((size_t)z * x.rows + row) * x.pitch + off);
+      acc.x ^= q.x, acc.y ^= q.y, acc.z ^= q.z, acc.w ^= q.w;
+    }
+    uint8_t* dst = x.out[row] + off;
+    if (x.accumulate) {
+      const uint4 o = load16(dst);
+      acc.x ^= o.x, acc.y ^= o.y, acc.z ^= o.z, acc.w ^= o.w;
+    }
+    store16(dst, acc);
+  }
+}
+
+hipError_t launch_xor_partials(const uint8_t* part, uint64_t pitch, int chunks, int rows,
+                               uint8_t* const* out, uint64_t len, bool accumulate, hipStream_t s) {
+  if (rows < 1 || rows > kMaxOut || chunks < 1 || len % 16) return hipErrorInvalidValue;
+  XorPartials x{};
+  x.part = part;
+  x.pitch = pitch;
+  x.n16 = len / 16;
+  for (int r = 0; r < rows; ++r) x.out[r] = out[r];
+  x.chunks = chunks;
+  x.rows = rows;
+  x.accumulate = accumulate ? 1 : 0;
+  const uint64_t blocks = (x.n16 * rows + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(xor_partials_kernel, dim3((unsigned)std::min<uint64_t>(blocks, 8192)), dim3(kBlock), 0, s, x);
+  return hipGetLastError();
+}
+
 __global__ void fill_u8_kernel(uint8_t* p, uint8_t v, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;

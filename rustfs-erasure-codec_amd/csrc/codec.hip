@@ -377,6 +377,85 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
   return RSGPU_OK;
 }
 
+// Per (device, stream): the partial-sum rows of split-over-inputs applies.  Calls
+// on one stream reuse it in stream order (the slot's mutex is held until a call's
+// launches are enqueued); it only grows, after waiting for its stream, the only
+// one that uses it.  Kept for the life of the process.
+struct PartSlot {
+  std::mutex mu;
+  DeviceBuffer buf;
+};
+static std::mutex g_part_mu;
+static std::map<std::pair<int, hipStream_t>, std::unique_ptr<PartSlot>>* g_part_slots =
+    new std::map<std::pair<int, hipStream_t>, std::unique_ptr<PartSlot>>();
+
+static PartSlot* part_slot(hipStream_t s) {
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(g_part_mu);
+  auto& slot = (*g_part_slots)[std::make_pair(device, s)];
+  if (!slot) slot = std::make_unique<PartSlot>();
+  return slot.get();
+}
+
+// out (r_total rows) = M (r_total x k_total, tables as run_apply) applied to
+// k_total > kMaxIn inputs: one launch per row group in which workgroup column z
+// takes inputs [z * kMaxIn, z * kMaxIn + kMaxIn) from a staged pointer array and
+// writes partial rows, then xor_partials folds the chunks into the outputs.  A
+// long, narrow apply (thousands of inputs, a few outputs) then fills the chip
+// instead of running kMaxIn-input launches one after another.  Vector tiles only:
+// 16-byte aligned buffers, len a multiple of 16.  elem: 0 GF(2^8), 1 tower, 2
+// split-layout GF(2^16).
+static int run_apply_splitk(const uint32_t* d_tables, int k_total, int r_total, int elem,
+                            const uint8_t* const* in, uint8_t* const* out, size_t len, hipStream_t s,
+                            int variant) {
+  const int W = words_per_coef(elem != 0);
+  const int chunks = (k_total + kMaxIn - 1) / kMaxIn;
+  int tile = 0, split = 0;
+  if (variant >= 0 && variant < kNumApplyVariants) {
+    tile = variant % kNumTiles;
+    split = kRowSplit[variant / kNumTiles];
+  }
+  PartSlot* ps = part_slot(s);
+  if (!ps) return RSGPU_ERR_HIP;
+  std::lock_guard<std::mutex> g(ps->mu);
+  const size_t pitch = (len + 255) / 256 * 256;
+  const size_t need = (size_t)chunks * std::min(kMaxOut, r_total) * pitch;
+  if (ps->buf.bytes < need) {
+    HIP_TRY(hipStreamSynchronize(s));  // the old rows may still be read on s
+    if (ps->buf.p) {
+      (void)hipFree(ps->buf.p);
+      ps->buf.p = nullptr;
+      ps->buf.bytes = 0;
+    }
+    HIP_TRY(ps->buf.alloc(need));
+  }
+  std::vector<const uint8_t*> ptrs(in, in + k_total);
+  const uint8_t* const* d_in = nullptr;
+  size_t off = 0, bytes = 0;
+  int rc = stage_pointer_table(ptrs, s, &d_in, &off, &bytes);
+  if (rc) return rc;
+  for (int r0 = 0; r0 < r_total; r0 += kMaxOut) {
+    const int rn = std::min(kMaxOut, r_total - r0);
+    ApplyArgs a;
+    std::memset(&a, 0, offsetof(ApplyArgs, tab_inline));
+    for (int j = 0; j < rn; ++j) a.out[j] = out[r0 + j];
+    a.in[0] = in[0];
+    a.n16 = len / 16;
+    a.k = kMaxIn;
+    a.r = rn;
+    a.tables = d_tables + (size_t)r0 * k_total * W;
+    a.tab_row_stride = k_total * W;
+    a.in_dev = d_in;
+    a.part = (uint8_t*)ps->buf.p;
+    a.part_pitch = pitch;
+    a.k_total = k_total;
+    HIP_TRY(launch_apply(a, kWrite, tile, split ? std::min(split, rn) : rn, elem, s));
+    HIP_TRY(launch_xor_partials((const uint8_t*)ps->buf.p, pitch, chunks, rn, out + r0, len, false, s));
+  }
+  return release_pointer_table(off, bytes, s);
+}
+
 static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool tower,
                      const std::vector<const uint8_t*>& in, const std::vector<uint8_t*>& out,
                      size_t len, int mode, int32_t* flag, hipStream_t s, int variant = -1,
@@ -1484,8 +1563,15 @@ int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_
   for (int i : dm->inputs) in.push_back(shards[i].ptr);
   for (int t : dm->targets) out.push_back(shards[t].ptr);
   const bool g16 = c.family == RSGPU_FAMILY_LEOPARD_GF16;
-  int rc = run_apply((const uint32_t*)dm->tables.p, (int)in.size(), (int)out.size(), g16, in.data(), out.data(),
-                     len, kWrite, nullptr, s, variant, -1, 0, nullptr, nullptr, g16);
+  bool vec = len % 16 == 0;
+  for (auto p : in) vec = vec && aligned16(p);
+  for (auto p : out) vec = vec && aligned16(p);
+  // split over inputs from three input chunks on (two chunks run as fast in turn)
+  int rc = (in.size() > (size_t)2 * kMaxIn && vec)
+               ? run_apply_splitk((const uint32_t*)dm->tables.p, (int)in.size(), (int)out.size(), g16 ? 2 : 0,
+                                  in.data(), out.data(), len, s, variant)
+               : run_apply((const uint32_t*)dm->tables.p, (int)in.size(), (int)out.size(), g16, in.data(),
+                           out.data(), len, kWrite, nullptr, s, variant, -1, 0, nullptr, nullptr, g16);
   if (rc) return rc;
   for (int t : dm->targets) shards[t].present = 1;
   return RSGPU_OK;

@@ -48,6 +48,15 @@ struct ApplyArgs {
   // null: the caller cleared the flag
   uint64_t* clear_ws;
   uint64_t call_id;
+  // Split over inputs (in_dev set, write mode): blockIdx.z takes inputs
+  // [z * k, z * k + k) of the k_total device pointers in_dev (the last chunk
+  // fewer) and writes its partial sums to rows z * r + row of `part`, part_pitch
+  // bytes apart; xor_partials then folds the chunks into the outputs.
+  const uint8_t* const* in_dev;
+  uint8_t* part;
+  uint64_t part_pitch;
+  int32_t k_total;
+  int32_t pad_;
   // the tables when `tables` is null (same layout, rows tab_row_stride apart);
   // the kernel reads them straight from its kernel-argument segment
   uint32_t tab_inline[kInlineTabWords];
@@ -98,6 +107,10 @@ enum DispatchOp : int { kOpEncode = 0, kOpVerify = 1, kOpReconstruct = 2, kNumOp
 // shards; tables are then four byte maps per coefficient [row][input][4][kTabWords].
 hipError_t launch_apply(const ApplyArgs& a, int mode, int tile, int rows, int tower, hipStream_t s);
 hipError_t launch_apply_bytes(const ByteArgs& a, hipStream_t s);
+// out[row] (^)= XOR over z < chunks of part[(z * rows + row) * pitch ..], len bytes
+// (a multiple of 16) per row; accumulate XORs into the outputs instead of writing.
+hipError_t launch_xor_partials(const uint8_t* part, uint64_t pitch, int chunks, int rows,
+                               uint8_t* const* out, uint64_t len, bool accumulate, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, uint8_t v, uint64_t n, hipStream_t s);
 

@@ -40,6 +40,12 @@ int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_
 // maps each, in the kernel's plane order: low->low, high->low, low->high,
 // high->high; a zero coefficient gets zero maps).
 uint16_t leopard16_one();
+// A table of device pointers staged on stream s through the calling thread's
+// pinned ring (*d its device address); release it after the launches that read
+// it are enqueued.
+int stage_pointer_table(const std::vector<const uint8_t*>& ptrs, hipStream_t s, const uint8_t* const** d,
+                        size_t* off, size_t* bytes);
+int release_pointer_table(size_t off, size_t bytes, hipStream_t s);
 void leopard16_apply_tables(const uint16_t* coef, size_t n, uint32_t* out);
 
 }  // namespace rsgpu

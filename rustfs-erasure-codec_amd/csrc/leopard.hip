@@ -1376,6 +1376,12 @@ static int release_args(size_t off, size_t bytes, hipStream_t s) {
   return RSGPU_OK;
 }
 
+int stage_pointer_table(const std::vector<const uint8_t*>& ptrs, hipStream_t s, const uint8_t* const** d,
+                        size_t* off, size_t* bytes) {
+  return stage_args(ptrs, nullptr, s, d, nullptr, off, bytes);
+}
+int release_pointer_table(size_t off, size_t bytes, hipStream_t s) { return release_args(off, bytes, s); }
+
 int init_leopard(Codec& c) {
   auto st = std::make_shared<LeoState>();
   st->gf16 = c.family == RSGPU_FAMILY_LEOPARD_GF16;

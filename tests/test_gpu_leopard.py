@@ -153,7 +153,8 @@ def test_inconsistent_input_matches_oracle():
                 assert (to_host(work[i]) == exp[i]).all(), (bits, missing, i)
 
 
-@pytest.mark.parametrize("k,m,n", [(6, 4, 192), (10, 4, (64 << 10) + 64), (12, 7, 4096), (32, 16, 640)])
+@pytest.mark.parametrize("k,m,n", [(6, 4, 192), (10, 4, (64 << 10) + 64), (12, 7, 4096), (32, 16, 640),
+                                   (128, 64, 1024)])
 def test_decoder_matrix_reconstruct_matches_oracle(k, m, n):
     """Matrix variants with more than k shards present apply the FFT decoder's own
     matrix over all of them: on non-codeword inputs the bytes equal the oracle's
@@ -164,7 +165,7 @@ def test_decoder_matrix_reconstruct_matches_oracle(k, m, n):
     shards = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
     rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf8")
     rs.set_variant(5)  # a matrix-apply variant
-    for e in range(1, m):
+    for e in (range(1, m) if m <= 16 else (1, 2, 17, m - 1)):  # > 64 present: split over inputs
         missing = sorted(rng.choice(k + m, e, replace=False).tolist())
         present = [0 if i in missing else 1 for i in range(k + m)]
         rc, exp = O.leo_reconstruct(8, k, m, [s.copy() for s in shards], present)
