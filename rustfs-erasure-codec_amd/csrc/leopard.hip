@@ -953,23 +953,25 @@ __device__ __forceinline__ void decode_row(const LeoArgs& a, uint32_t lane, uint
 // Formal derivative (decode.rs:537-551, compute_formal_derivative16
 // leopard_gf16/decode.rs:1100-1116) in closed form: row j takes the XOR of the
 // rows j + 2^b for the clear bits b of j (j + 2^b < n), all read before any
-// update.  An item is the four rows 4g .. 4g+3 times G words: bits 0 and 1 stay
-// inside the group; bit b >= 2 adds group g + 2^(b-2) row for row when bit b-2
-// of g is clear.  A row depends only on itself and higher rows, so chunks of
-// items run in increasing g, each reading before any of its writes.
+// update.  With Q = n / 4, an item is the four rows g, g + Q, g + 2Q, g + 3Q
+// (g < Q) times G words: the two top bits stay inside the item, and a clear bit
+// b of g adds item g + 2^b row for row.  Consecutive lanes take consecutive g,
+// so every LDS access is a run of consecutive rows (conflict free).  A row
+// depends only on itself and higher rows, so chunks of items run in increasing
+// g, each reading before any of its writes.
 template <bool G16, int G>
 __device__ __forceinline__ void formal_derivative(const LeoArgs& a, uint32_t* lds) {
-  const uint32_t C = a.C, ng = C / G, lg_ng = lg_groups<G>(a), groups = a.n >> 2;
-  for (uint32_t it0 = 0; it0 < groups * ng; it0 += blockDim.x) {
+  const uint32_t C = a.C, ng = C / G, lg_ng = lg_groups<G>(a), Q = a.n >> 2;
+  for (uint32_t it0 = 0; it0 < Q * ng; it0 += blockDim.x) {
     const uint32_t it = it0 + threadIdx.x;
-    const bool live = it < groups * ng;
+    const bool live = it < Q * ng;
     const uint32_t g = it >> lg_ng, wq = (it & (ng - 1)) * G;
     Word<G16> r0[G], r1[G], r2[G], r3[G];
     if (live) {
-      lds_getv<G16, G>(lds, (4 * g) * C + wq, r0);
-      lds_getv<G16, G>(lds, (4 * g + 1) * C + wq, r1);
-      lds_getv<G16, G>(lds, (4 * g + 2) * C + wq, r2);
-      lds_getv<G16, G>(lds, (4 * g + 3) * C + wq, r3);
+      lds_getv<G16, G>(lds, g * C + wq, r0);
+      lds_getv<G16, G>(lds, (g + Q) * C + wq, r1);
+      lds_getv<G16, G>(lds, (g + 2 * Q) * C + wq, r2);
+      lds_getv<G16, G>(lds, (g + 3 * Q) * C + wq, r3);
 #pragma unroll
       for (int q = 0; q < G; ++q) {
         wxor(r0[q], r1[q]);
@@ -977,30 +979,30 @@ __device__ __forceinline__ void formal_derivative(const LeoArgs& a, uint32_t* ld
         wxor(r1[q], r3[q]);
         wxor(r2[q], r3[q]);
       }
-      for (uint32_t step = 1; step < groups; step <<= 1) {
-        if ((g & step) || g + step >= groups) continue;
-        const uint32_t h = 4 * (g + step);
+      for (uint32_t step = 1; step < Q; step <<= 1) {
+        if ((g & step) || g + step >= Q) continue;
+        const uint32_t h = g + step;
         Word<G16> y[G];
         lds_getv<G16, G>(lds, h * C + wq, y);
 #pragma unroll
         for (int q = 0; q < G; ++q) wxor(r0[q], y[q]);
-        lds_getv<G16, G>(lds, (h + 1) * C + wq, y);
+        lds_getv<G16, G>(lds, (h + Q) * C + wq, y);
 #pragma unroll
         for (int q = 0; q < G; ++q) wxor(r1[q], y[q]);
-        lds_getv<G16, G>(lds, (h + 2) * C + wq, y);
+        lds_getv<G16, G>(lds, (h + 2 * Q) * C + wq, y);
 #pragma unroll
         for (int q = 0; q < G; ++q) wxor(r2[q], y[q]);
-        lds_getv<G16, G>(lds, (h + 3) * C + wq, y);
+        lds_getv<G16, G>(lds, (h + 3 * Q) * C + wq, y);
 #pragma unroll
         for (int q = 0; q < G; ++q) wxor(r3[q], y[q]);
       }
     }
     __syncthreads();
     if (live) {
-      lds_putv<G16, G>(lds, (4 * g) * C + wq, r0);
-      lds_putv<G16, G>(lds, (4 * g + 1) * C + wq, r1);
-      lds_putv<G16, G>(lds, (4 * g + 2) * C + wq, r2);
-      lds_putv<G16, G>(lds, (4 * g + 3) * C + wq, r3);
+      lds_putv<G16, G>(lds, g * C + wq, r0);
+      lds_putv<G16, G>(lds, (g + Q) * C + wq, r1);
+      lds_putv<G16, G>(lds, (g + 2 * Q) * C + wq, r2);
+      lds_putv<G16, G>(lds, (g + 3 * Q) * C + wq, r3);
     }
     __syncthreads();
   }
