@@ -68,6 +68,8 @@ def parse():
                     help="with --leopard-case: the forced kernel variant of the timed calls")
     ap.add_argument("--mfma-only", action="store_true",
                     help="only the int8 matrix-core vs VALU encode comparison")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving section (request server latency and capacity)")
     ap.add_argument("--host-only", action="store_true",
                     help="only the host-boundary section (host-pointer and block-stream rates)")
     ap.add_argument("--roofline-only", action="store_true",
@@ -502,6 +504,12 @@ def main():
             out[op] = {"us": round(t, 1), "data_gbs": round(k * S / t / 1e3, 1), "erasures": e,
                        "variant": vv}
         print(json.dumps({"leopard_case": {"family": fam, "k": k, "m": m, "shard": S, **out}}))
+        return
+    if args.serving_only:
+        import torch
+
+        torch.cuda.init()
+        print(json.dumps({"serving": serving_section(torch, [int(r) for r in args.rates.split(",")])}))
         return
     if args.host_only:
         import torch

@@ -229,6 +229,39 @@ int launch(rsgpu_server* S, uint64_t t) {
       taken.push_back(S->pending.front());
       S->pending.pop_front();
     }
+    // A lone request runs as its own per-stripe call on the slot's stream: no
+    // descriptor upload and no batch kernel, so a lightly loaded server answers
+    // at the single-call latency.
+    if (taken.size() == 1) {
+      Req& q = S->reqs.at(taken[0]);
+      const rsgpu_codec* c = q.r.codec;
+      if (q.r.op == 0) {
+        rc = rsgpu_encode(c, q.sl.data(), q.sl.size(), B.stream);
+      } else if (q.r.op == 1) {
+        q.flag_idx = n_flags++;
+        rc = rsgpu_verify_async(c, q.csl.data(), q.csl.size(), (int32_t*)B.d_flags.p + q.flag_idx, B.stream);
+      } else {
+        rc = rsgpu_reconstruct(c, q.slots.data(), q.slots.size(), B.stream);
+      }
+      q.launch = t;
+      if (rc) {  // validation error: reported per request, never launched
+        q.status = rc;
+        complete(S, q, t);
+        continue;
+      }
+      q.slot = (int)si;
+      B.tags.push_back(q.r.tag);
+      if (n_flags && hipMemcpyAsync(B.h_flags, B.d_flags.p, 4, hipMemcpyDeviceToHost, B.stream) != hipSuccess)
+        return RSGPU_ERR_HIP;
+      if (hipEventRecord(B.ev, B.stream) != hipSuccess) return RSGPU_ERR_HIP;
+      B.busy = true;
+      S->st.batches++;
+      S->st.launched_ops += 1;
+      uint64_t inflight = 0;
+      for (auto& o : S->slots) inflight += o->busy ? 1 : 0;
+      S->st.max_inflight_batches = std::max<uint64_t>(S->st.max_inflight_batches, inflight);
+      continue;
+    }
     // the new erasure patterns among them, solved together ahead of the adds
     std::vector<PresolveCall> calls;
     for (uint64_t tag : taken) {
