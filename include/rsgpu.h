@@ -409,7 +409,10 @@ int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k,
  * ops run in one batch kernel, tower-field and Leopard ops through their
  * own kernels on the same stream.  Reconstruct ops enqueue any decode-row solve
  * on `stream` when added and mark their targets present.  Verify ops OR a
- * nonzero value into `device_flag` on mismatch (the caller zeroes it). ---- */
+ * nonzero value into `device_flag` on mismatch (the caller zeroes it).  A batch
+ * refers to each op's codec (its tables, or the codec itself for ops run through
+ * their own kernels) and shards without copying them: both must stay alive until
+ * the batch is cleared or freed. ---- */
 typedef struct rsgpu_batch rsgpu_batch;
 int rsgpu_batch_new(rsgpu_batch** out);
 void rsgpu_batch_free(rsgpu_batch* batch);

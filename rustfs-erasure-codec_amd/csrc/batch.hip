@@ -280,8 +280,9 @@ struct rsgpu_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
-  // ops the batch kernel does not take (tower field, Leopard): kept and run
-  // through their own entry points on the batch's stream.
+  // ops the batch kernel does not take (tower field, Leopard GF(2^16), Leopard
+  // GF(2^8) when the FFT serves them): kept and run through their own entry
+  // points on the batch's stream.
   struct Deferred {
     const rsgpu_codec* codec;
     int op;  // 0 encode, 1 verify, 2 reconstruct
@@ -316,6 +317,13 @@ int batch_verify_stripe(const Codec& c, const rsgpu_cslice* shards, size_t n, in
                         BatchPlan& plan);
 int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
                              hipStream_t s, BatchPlan& plan);
+// Leopard GF(2^8) codecs: the matrix forms in the batch kernel, or
+// RSGPU_ERR_UNSUPPORTED when the op runs as its own FFT launch
+int batch_leopard8_encode(const Codec& c, const rsgpu_slice* shards, size_t n, BatchPlan& plan);
+int batch_leopard8_verify(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* flag,
+                          BatchPlan& plan);
+int batch_leopard8_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
+                               BatchPlan& plan);
 }  // namespace rsgpu
 
 static void invalidate_graph(rsgpu_batch* b) {
@@ -483,6 +491,13 @@ static bool batch_kernel_ok(const rsgpu_codec* c) {
 
 int rsgpu_batch_add_encode(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_slice* shards, size_t n) {
   if (!b || !c || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (c->c.family == RSGPU_FAMILY_LEOPARD_GF8) {
+    const int rc = batch_leopard8_encode(c->c, shards, n, b->plan);
+    if (rc != RSGPU_ERR_UNSUPPORTED) {
+      if (!rc) b->dirty = true;
+      return rc;
+    }
+  }
   if (!batch_kernel_ok(c)) {
     rsgpu_batch::Deferred d{c, 0, std::vector<rsgpu_slice>(shards, shards + n), {}, {}, nullptr, 0, {}};
     b->deferred.push_back(std::move(d));
@@ -498,6 +513,13 @@ int rsgpu_batch_add_encode(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_sli
 int rsgpu_batch_add_verify(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_cslice* shards,
                            size_t n, int32_t* device_flag) {
   if (!b || !c || !shards || !device_flag) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (c->c.family == RSGPU_FAMILY_LEOPARD_GF8) {
+    const int rc = batch_leopard8_verify(c->c, shards, n, device_flag, b->plan);
+    if (rc != RSGPU_ERR_UNSUPPORTED) {
+      if (!rc) b->dirty = true;
+      return rc;
+    }
+  }
   if (!batch_kernel_ok(c)) {
     rsgpu_batch::Deferred d{c, 1, {}, std::vector<rsgpu_cslice>(shards, shards + n), {}, device_flag, 0, {}};
     b->deferred.push_back(std::move(d));
@@ -513,10 +535,26 @@ int rsgpu_batch_add_verify(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_csl
 int rsgpu_batch_add_reconstruct(rsgpu_batch* b, const rsgpu_codec* c, rsgpu_slot* shards, size_t n,
                                 int32_t data_only, void* stream) {
   if (!b || !c || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
+  if (c->c.family == RSGPU_FAMILY_LEOPARD_GF8) {
+    const int rc = batch_leopard8_reconstruct(c->c, shards, n, data_only ? 1 : 0, (hipStream_t)stream, b->plan);
+    if (rc != RSGPU_ERR_UNSUPPORTED) {
+      if (!rc) b->dirty = true;
+      return rc;
+    }
+  }
   if (!batch_kernel_ok(c)) {
     rsgpu_batch::Deferred d{c, 2, {}, {}, std::vector<rsgpu_slot>(shards, shards + n), nullptr,
                             data_only ? 1 : 0, {}};
-    for (size_t i = 0; i < n; ++i) d.present0.push_back(shards[i].present);
+    size_t present = 0;
+    for (size_t i = 0; i < n; ++i) {
+      d.present0.push_back(shards[i].present);
+      present += shards[i].present ? 1 : 0;
+    }
+    // the targets read as present from here on, as after an op in the kernel;
+    // the run restores the recorded presence before the call
+    if (n == c->c.k + c->c.m && present >= c->c.k)
+      for (size_t i = 0; i < n; ++i)
+        if (!shards[i].present && (!data_only || i < c->c.k)) shards[i].present = 1;
     b->deferred.push_back(std::move(d));
     b->plan.n_ops++;
     invalidate_graph(b);

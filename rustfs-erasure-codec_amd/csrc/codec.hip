@@ -1684,6 +1684,107 @@ int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int g
   return RSGPU_OK;
 }
 
+// ------------------------------------------- Leopard GF(2^8) in the batch kernel
+// The matrix forms of a Leopard GF(2^8) codec are GF(2^8) byte applies with
+// tables in Leopard's field, bit-identical to the FFT: encode and verify over
+// [I; G], reconstruct with exactly k shards present as the [I; G] solve, with
+// more as the decoder's own matrix over every present shard.  A batch takes them
+// into its kernel when the codec's dispatch picks a matrix variant for the call,
+// or, with no dispatch state, for k <= 32, where the matrix form measured about
+// twice the FFT's rate per call; otherwise RSGPU_ERR_UNSUPPORTED tells the batch
+// to run the FFT through its own launch.
+static bool leo8_batch_matrix(const Codec& c, int op, size_t erasures, size_t len) {
+  if (c.family != RSGPU_FAMILY_LEOPARD_GF8) return false;
+  const bool no_state = c.forced_variant.load(std::memory_order_relaxed) < 0 &&
+                        !std::atomic_load(&c.variant_table);
+  if (no_state) return c.k <= 32;
+  return is_matrix_variant(c, pick_variant(c, op, erasures, len));
+}
+
+int batch_leopard8_encode(const Codec& c, const rsgpu_slice* shards, size_t n, BatchPlan& plan) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  if (shards[0].len % 64) return RSGPU_INCORRECT_SHARD_SIZE;  // leopard.rs:50-56
+  if (!leo8_batch_matrix(c, kOpEncode, 0, shards[0].len)) return RSGPU_ERR_UNSUPPORTED;
+  std::shared_ptr<Codec> mc;
+  if ((rc = leo_matrix_codec(c, nullptr, mc))) return rc;
+  return batch_encode_stripe(*mc, shards, n, plan);
+}
+
+int batch_leopard8_verify(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* flag,
+                          BatchPlan& plan) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  int rc = check_slices(shards, n);
+  if (rc) return rc;
+  if (shards[0].len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
+  if (!leo8_batch_matrix(c, kOpVerify, 0, shards[0].len)) return RSGPU_ERR_UNSUPPORTED;
+  std::shared_ptr<Codec> mc;
+  if ((rc = leo_matrix_codec(c, nullptr, mc))) return rc;
+  return batch_verify_stripe(*mc, shards, n, flag, plan);
+}
+
+// Validation as leopard_reconstruct (reconstruct.rs:795-1040) for goals 0 (all)
+// and 1 (data only).
+int batch_leopard8_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
+                               BatchPlan& plan) {
+  const size_t total = c.k + c.m;
+  if (n < total) return RSGPU_TOO_FEW_SHARDS;
+  if (n > total) return RSGPU_TOO_MANY_SHARDS;
+  size_t len = 0, present = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (!shards[i].present) continue;
+    if (shards[i].len == 0) return RSGPU_EMPTY_SHARD;
+    if ((len && shards[i].len != len) || shards[i].len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
+    len = shards[i].len;
+    ++present;
+  }
+  if (present < c.k) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  std::vector<uint8_t> flags(total, 0);
+  bool need = false;
+  for (size_t i = 0; i < total; ++i) {
+    const bool req = goal == 0 || i < c.k;
+    flags[i] = (uint8_t)((shards[i].present ? 1 : 0) | (!shards[i].present && req ? 2 : 0));
+    if (!shards[i].present && req) {
+      need = true;
+      if (!shards[i].ptr || shards[i].len != len) return RSGPU_INCORRECT_SHARD_SIZE;
+    }
+  }
+  if (!need) return RSGPU_OK;
+  if (total - present > c.m) return RSGPU_TOO_FEW_SHARDS_PRESENT;
+  if (!leo8_batch_matrix(c, kOpReconstruct, total - present, len)) return RSGPU_ERR_UNSUPPORTED;
+  int rc;
+  if (present == c.k) {
+    std::shared_ptr<Codec> mc;
+    if ((rc = leo_matrix_codec(c, s, mc))) return rc;
+    return batch_reconstruct_stripe(*mc, shards, n, goal, s, plan);
+  }
+  const std::string key((const char*)flags.data(), total);
+  std::shared_ptr<const DecoderMatrix> dm;
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    dm = c.dec_matrix.get(key);
+  }
+  if (!dm) {
+    if ((rc = build_decoder_matrix(c, flags.data(), s, dm))) return rc;
+    std::vector<std::shared_ptr<const DecoderMatrix>> evicted;  // freed after the lock
+    std::lock_guard<std::mutex> g(c.mu);
+    c.dec_matrix.put(key, dm, evicted);
+  }
+  std::vector<const uint8_t*> in;
+  std::vector<uint8_t*> out;
+  for (int i : dm->inputs) in.push_back(shards[i].ptr);
+  for (int t : dm->targets) out.push_back(shards[t].ptr);
+  if ((rc = add_stripe(plan, in, out, (const uint32_t*)dm->tables.p, len, kWrite, nullptr))) return rc;
+  plan.keep.push_back(std::const_pointer_cast<DecoderMatrix>(dm));
+  for (int t : dm->targets) shards[t].present = 1;
+  return RSGPU_OK;
+}
+
 // ---------------------------------------------- allocation-free reconstruct
 // Device scratch of rsgpu_reconstruct_noalloc: one decode solve's status word,
 // coefficient tables and rows.  Each device holds its own copy (for the life of

@@ -965,7 +965,8 @@ _lib.rsgpu_batch_stats.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
 
 
 class Batch:
-    """Many stripe operations executed by one batch-kernel launch."""
+    """Many stripe operations executed by one batch-kernel launch.  The batch holds
+    each op's codec and shards until it is cleared or dropped."""
 
     def __init__(self):
         h = C.c_void_p()
@@ -985,12 +986,12 @@ class Batch:
     def add_encode(self, rs, shards):
         arr = (Slice * len(shards))(*[Slice(*_ptr_len(s)) for s in shards])
         _check(_lib.rsgpu_batch_add_encode(self._h, rs._h, arr, len(shards)))
-        self._keep.append(shards)
+        self._keep.append((rs, shards))
 
     def add_verify(self, rs, shards, device_flag):
         arr = (CSlice * len(shards))(*[CSlice(*_ptr_len(s)) for s in shards])
         _check(_lib.rsgpu_batch_add_verify(self._h, rs._h, arr, len(shards), device_flag.data_ptr()))
-        self._keep.append((shards, device_flag))
+        self._keep.append((rs, shards, device_flag))
 
     def add_reconstruct(self, rs, slots, data_only=False, stream=None):
         """slots: list of ShardSlot (buffers for missing shards must be allocated)."""
@@ -999,7 +1000,7 @@ class Batch:
                                                 _stream(stream)))
         for s, a in zip(slots, arr):
             s.present = bool(a.present)
-        self._keep.append(slots)
+        self._keep.append((rs, slots))
 
     def set_tune(self, inputs_per_group, vectors_per_lane, blocks_per_cu):
         _check(_lib.rsgpu_batch_set_tune(self._h, inputs_per_group, vectors_per_lane, blocks_per_cu))

@@ -153,6 +153,66 @@ def test_batch_with_tower_and_leopard_codecs_defers():
     assert all((to_host(sh[k + j]) == exp[j]).all() for j in range(m))
 
 
+def test_leopard_gf8_ops_in_the_batch_kernel_match_the_fft():
+    """Leopard GF(2^8) codecs with k <= 32 and no dispatch state put their matrix forms
+    into the batch kernel (encode and verify over [I; G], reconstruct with exactly k
+    present as the [I; G] solve, with more as the decoder's own matrix); larger ones
+    run their FFT as deferred launches.  Every output equals the oracle's FFT codec,
+    on non-codeword inputs for the decoder-matrix reconstruct, and a corrupted stripe
+    fails its verify."""
+    announce("batch leopard gf8 matrix forms")
+    rng = np.random.default_rng(88)
+    b = rsgpu.Batch()
+    checks = []
+    for k, m, n in [(10, 4, 4096), (32, 16, 1024), (128, 64, 640)]:
+        # rs is rebound on each pass: the batch holds every op's codec until it runs
+        rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf8")
+        data = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k)]
+        rc, par = O.leo_encode(8, k, m, data)
+        assert rc == 0
+        # encode
+        sh = [torch.from_numpy(d).cuda() for d in data] + [torch.zeros(n, dtype=torch.uint8, device="cuda")
+                                                           for _ in range(m)]
+        b.add_encode(rs, sh)
+        checks.append(("encode", sh, [(k + j, par[j]) for j in range(m)]))
+        # verify: a codeword and a corrupted copy
+        full = data + list(par)
+        good = [torch.from_numpy(x.copy()).cuda() for x in full]
+        bad = [torch.from_numpy(x.copy()).cuda() for x in full]
+        bad[k + m - 1][5] ^= 1
+        flags = torch.zeros(2, dtype=torch.int32, device="cuda")
+        b.add_verify(rs, good, flags[0:1])
+        b.add_verify(rs, bad, flags[1:2])
+        checks.append(("verify", flags, None))
+        # reconstruct, exactly k present (a codeword)
+        miss = sorted(rng.choice(k + m, m, replace=False).tolist())
+        slots = [rsgpu.ShardSlot(torch.zeros(n, dtype=torch.uint8, device="cuda"), False) if i in miss
+                 else rsgpu.ShardSlot(torch.from_numpy(full[i].copy()).cuda()) for i in range(k + m)]
+        b.add_reconstruct(rs, slots)
+        checks.append(("rec", slots, [(i, full[i]) for i in miss]))
+        # reconstruct, more than k present, non-codeword inputs (the decoder's bytes)
+        junk = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
+        miss = sorted(rng.choice(k + m, 2, replace=False).tolist())
+        present = [0 if i in miss else 1 for i in range(k + m)]
+        rc, exp = O.leo_reconstruct(8, k, m, [x.copy() for x in junk], present)
+        assert rc == 0
+        slots = [rsgpu.ShardSlot(torch.zeros(n, dtype=torch.uint8, device="cuda"), False) if i in miss
+                 else rsgpu.ShardSlot(torch.from_numpy(junk[i].copy()).cuda()) for i in range(k + m)]
+        b.add_reconstruct(rs, slots)
+        checks.append(("rec", slots, [(i, exp[i]) for i in miss]))
+    b.run()
+    torch.cuda.synchronize()
+    for kind, obj, want in checks:
+        if kind == "encode":
+            for i, w in want:
+                assert (to_host(obj[i]) == w).all(), (kind, i)
+        elif kind == "verify":
+            assert int(obj[0]) == 0 and int(obj[1]) != 0
+        else:
+            for i, w in want:
+                assert obj[i].present and (to_host(obj[i].data) == w).all(), (kind, i)
+
+
 def test_ragged_workload_plain_equals_batched():
     """The ragged mix (mixed shapes, 4 KiB-256 KiB shards, encode/verify/reconstruct with
     0..m erasures): the batch kernel writes the same bytes as per-stripe launches."""

@@ -101,7 +101,8 @@ def test_training_on_device_fits_a_smooth_function():
 def test_dispatcher_trained_on_device_regret_and_gain():
     """Train on the committed timing set (80%) with the device kernels; on the
     held-out 20% the network's picks must be within 10% of the exhaustively
-    measured best on average and beat the default tiling."""
+    measured best on average and keep most of the best's gain over the default
+    tiling."""
     announce("dispatcher held-out regret")
     data = D.load(D.DEFAULT_TIMINGS)
     assert int(np.isfinite(data["T"]).sum()) > 40000
@@ -117,8 +118,14 @@ def test_dispatcher_trained_on_device_regret_and_gain():
     print(f"held-out regret {r}", flush=True)
     assert r["mean"] <= 0.10, r
     assert r["total"] <= 0.05, r
-    assert r["gain_vs_default_geomean"] >= 1.15, r   # per call, over the default tiling
-    assert r["gain_vs_default_total"] >= 1.05, r     # and in total time
+    # over the default tiling, per call and in total time: at least 80% of the log
+    # gain that the exhaustively measured best (default included) reaches on this set
+    T, Dt = data["T"][te], data["D"][te]
+    best = np.minimum(np.where(np.isfinite(T), T, np.inf).min(1), Dt)
+    ceil_geo, ceil_tot = float(np.exp(np.log(Dt / best).mean())), float(Dt.sum() / best.sum())
+    print(f"gain ceiling geomean {ceil_geo:.3f} total {ceil_tot:.3f}", flush=True)
+    assert r["gain_vs_default_geomean"] >= ceil_geo ** 0.8, r
+    assert r["gain_vs_default_total"] >= ceil_tot ** 0.8, r
     assert r["mean"] < r["default_mean"] / 4, r
 
 
