@@ -68,6 +68,8 @@ def parse():
                     help="with --leopard-case: the forced kernel variant of the timed calls")
     ap.add_argument("--mfma-only", action="store_true",
                     help="only the int8 matrix-core vs VALU encode comparison")
+    ap.add_argument("--gf16-batch-only", action="store_true",
+                    help="only the GF(2^16) batched-vs-plain section")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving section (request server latency and capacity)")
     ap.add_argument("--host-only", action="store_true",
@@ -110,14 +112,14 @@ class Stripes:
     k+m streams of a column on the same HBM channel bits; independent allocations
     would not, so the default pitch adds 64 KiB + 256 B (both pitches are reported)."""
 
-    def __init__(self, rsgpu, torch, k, m, n, shard, seed, pad=65536 + 256):
+    def __init__(self, rsgpu, torch, k, m, n, shard, seed, pad=65536 + 256, codec_kw=None):
         self.k, self.m, self.n, self.shard = k, m, n, shard
         self.pitch = shard + pad
         g = torch.Generator(device="cuda").manual_seed(seed)
         self.store = torch.empty((n, k + m, self.pitch), dtype=torch.uint8, device="cuda")
         self.buf = self.store[:, :, :shard]
         self.buf[:, :k].random_(0, 256, generator=g)
-        self.rs = rsgpu.ReedSolomon(k, m)
+        self.rs = rsgpu.ReedSolomon(k, m, **(codec_kw or {}))
         self.lib = rsgpu.lib()
         self.rsgpu = rsgpu
         self.enc, self.ver = [], []
@@ -172,6 +174,71 @@ class Stripes:
                 self._ok(lib.rsgpu_batch_add_reconstruct(b._h, self.rs._h, self.slots(s, erasures[s]),
                                                          self.k + self.m, 0, stream))
         return b
+
+
+def gf16_batch_section(torch, rsgpu, steps):
+    """GF(2^16) stripes in the batch kernels against one call per stripe.  Classic
+    (tower) 10+4 x 256 KiB over 256 stripes: encode, and reconstruct of 1..4 erasures;
+    Leopard GF(2^16) 64+16 x 64 KiB over 64 stripes: reconstruct of 1..4 erasures as
+    the decoder's own matrix (matrix variant forced; the batch puts it in the
+    split-layout kernel).  The stripes are codewords (encoded first), rebuilt shards
+    are overwritten with a pattern before each timed run and compared with the
+    codeword after it."""
+    import ctypes as C
+
+    import numpy as np
+
+    sp = torch.cuda.current_stream().cuda_stream
+    rate = lambda nbytes, t: round(nbytes / t / 1e9, 2)
+    out = {}
+    cases = [("classic_gf16_10x4_256KiB", {"field": "gf16"}, 10, 4, 256, 256 << 10),
+             ("leopard_gf16_64x16_64KiB", {"codec_family": "leopard_gf16"}, 64, 16, 64, 64 << 10)]
+    for name, kw, k, m, n, S in cases:
+        st = Stripes(rsgpu, torch, k, m, n, S, seed=7, codec_kw=kw)
+        row = {}
+        enc_plain = lambda: [st.encode(s, sp) for s in range(n)]
+        t_plain = timed(torch, enc_plain, steps, 1)
+        ref = st.buf.clone()
+        if name.startswith("classic"):
+            b = st.batch("encode", sp)
+            st.buf[:, k:].fill_(0xA5)
+            t_batch = timed(torch, b.graph_launch, steps, 1)
+            assert torch.equal(st.buf, ref), name + " encode: batched differs from plain"
+            row["encode"] = {"plain_gbs": rate(n * k * S, t_plain), "batched_gbs": rate(n * k * S, t_batch),
+                             "batched_vs_plain": round(t_plain / t_batch, 2)}
+            del b
+        else:
+            st.rs.set_variant(0)
+        rng = np.random.default_rng(5)
+        er = [set(rng.choice(k + m, int(rng.integers(1, 5)), replace=False).tolist()) for _ in range(n)]
+        layouts = [st.slots(s, er[s]) for s in range(n)]
+        work = [st.slots(s, er[s]) for s in range(n)]
+        nbytes = C.sizeof(layouts[0])
+
+        def rec_plain():
+            for s in range(n):
+                C.memmove(work[s], layouts[s], nbytes)  # presence as laid out (a call marks targets)
+                st.reconstruct(work[s], sp)
+
+        def poison():
+            for s in range(n):
+                for i in er[s]:
+                    st.buf[s, i].fill_(0x5A)
+
+        poison()
+        t_plain = timed(torch, rec_plain, steps, 1)
+        assert torch.equal(st.buf, ref), name + " reconstruct: plain differs from the codeword"
+        b = st.batch("reconstruct", sp, erasures=er)
+        poison()
+        t_batch = timed(torch, b.graph_launch, steps, 1)
+        assert torch.equal(st.buf, ref), name + " reconstruct: batched differs from the codeword"
+        row["reconstruct_1_4_erasures"] = {"plain_gbs": rate(n * k * S, t_plain),
+                                           "batched_gbs": rate(n * k * S, t_batch),
+                                           "batched_vs_plain": round(t_plain / t_batch, 2)}
+        row["batch_tiles"] = b.stats()["tiles"]
+        out[name] = row
+        del b, ref, st
+    return out
 
 
 def timed(torch, fn, steps, warmup):
@@ -519,6 +586,14 @@ def main():
         torch.cuda.init()
         print(json.dumps({"host_boundary": host_section(rsgpu)}))
         return
+    if args.gf16_batch_only:
+        import torch
+
+        import rsgpu
+
+        torch.cuda.init()
+        print(json.dumps({"gf16_batch": gf16_batch_section(torch, rsgpu, 10)}))
+        return
     if args.mfma_only:
         import torch
 
@@ -710,6 +785,7 @@ def main():
         del b_rg, rg
 
     leopard = leopard_section(torch, rsgpu, DSP, disp) if args.leopard else None
+    gf16_batch = gf16_batch_section(torch, rsgpu, 10)
     gf2_mfma = gf2_mfma_section(torch, rsgpu)
     dispatch_live = None
     if args.dispatch_live:
@@ -756,6 +832,7 @@ def main():
         "ragged": ragged,
         "dispatch_live": dispatch_live,
         "leopard": leopard,
+        "gf16_batch": gf16_batch,
         "gf2_mfma": gf2_mfma,
         "serving": serving,
         "host_boundary": host,

@@ -405,11 +405,16 @@ int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k,
  * (encode_sep_par encode.rs:1177, execute_classic_plan_parallel
  * reconstruct.rs:229, ParallelPolicy parallel.rs:65-264); a batch is that
  * parallel policy for many stripes at once.  Each add_* validates exactly like
- * the single-stripe call of the same name and records the op; classic GF(2^8)
- * ops run in one batch kernel, tower-field and Leopard ops through their
- * own kernels on the same stream.  Reconstruct ops enqueue any decode-row solve
- * on `stream` when added and mark their targets present.  Verify ops OR a
- * nonzero value into `device_flag` on mismatch (the caller zeroes it).  A batch
+ * the single-stripe call of the same name and records the op.  Classic ops run
+ * in the batch kernels (one launch per element type present: GF(2^8) bytes,
+ * GF(2^16) tower pairs), and so do the matrix forms of Leopard ops: GF(2^8)
+ * encode, verify and reconstruct when the codec's dispatch picks a matrix
+ * variant (with no dispatch state: k <= 32), GF(2^16) reconstruct when it picks
+ * one (split-layout kernel, 16-byte aligned shards).  Other Leopard ops run
+ * their FFT kernels after the batch kernels, on the same stream.  Reconstruct
+ * ops enqueue any decode-row solve on `stream` when added and mark their targets
+ * present.  Verify ops OR a nonzero value into `device_flag` on mismatch (the
+ * caller zeroes it).  A batch
  * refers to each op's codec (its tables, or the codec itself for ops run through
  * their own kernels) and shards without copying them: both must stay alive until
  * the batch is cleared or freed. ---- */

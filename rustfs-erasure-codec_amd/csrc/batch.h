@@ -1,5 +1,5 @@
-// batch.h -- many classic stripes (mixed shapes, shard sizes and operations)
-// executed by one kernel launch.
+// batch.h -- many stripe operations (mixed shapes, shard sizes and operations)
+// executed by one kernel launch per element type present.
 #pragma once
 #include "internal.h"
 
@@ -14,8 +14,18 @@ struct BatchStripe {
   uint32_t ptr0;           // index of in[0] in the pointer table; outputs follow the k inputs
   uint32_t tab_row_stride; // dwords between table rows
   uint16_t k, r;
-  uint16_t mode, tower;
+  uint16_t mode, elem;     // elem: the stripe's element type (kElemByte, kElemTower, kElemSplit)
 };
+
+// Element types of the batch kernels, one kernel instance each:
+//   kElemByte  GF(2^8) bytes (classic GF(2^8), Leopard GF(2^8) matrix forms);
+//   kElemTower GF(2^16) tower elements as interleaved [x, c] byte pairs (classic GF(2^16));
+//   kElemSplit GF(2^16) elements in the split layout of Leopard shards (each 64-byte
+//              block: the low bytes of 32 elements, then their high bytes).
+constexpr int kElemByte = 0, kElemTower = 1, kElemSplit = 2, kNumElems = 3;
+// Outputs per descriptor of the GF(2^16) kernels (their accumulators are compiled
+// for four rows).
+constexpr int kMaxOut16 = 4;
 
 // Shape of the batch kernel: inputs per pipelined load group (kb), 16-byte
 // vectors per lane (nv; tiles are kBlock * nv vectors per shard), resident
@@ -30,18 +40,23 @@ struct BatchTune {
 };
 
 struct BatchPlan {
-  std::vector<BatchStripe> stripes;
+  std::vector<BatchStripe> stripes[kNumElems];  // by element type
   std::vector<const void*> ptrs;
   std::vector<std::shared_ptr<void>> keep;  // decode entries referenced by the stripes
   // tails (length % 16) and unaligned stripes run through the byte kernel
   std::vector<ByteArgs> byte_jobs;
-  uint64_t total_tiles = 0;
+  uint64_t total_tiles[kNumElems] = {0, 0, 0};
   uint64_t bytes_moved = 0;  // algorithmic bytes (read k, write/compare r)
   uint64_t data_bytes = 0;   // k * shard bytes
   int n_ops = 0;
 };
 
-hipError_t launch_batch(const BatchStripe* d_stripes, int n_stripes, const void* const* d_ptrs,
+// One batch kernel over the descriptors of one element type.  The GF(2^16)
+// kernels take a fixed shape (8 inputs per load group, one vector per lane, four
+// rows) and the tune's grid.
+hipError_t launch_batch(int elem, const BatchStripe* d_stripes, int n_stripes, const void* const* d_ptrs,
                         uint64_t total_tiles, const BatchTune& tune, int n_cu, hipStream_t s);
+// 16-byte vectors per lane of a kernel's tiles.
+inline int batch_tile_nv(int elem, const BatchTune& t) { return elem == kElemByte ? t.nv : 1; }
 
 }  // namespace rsgpu

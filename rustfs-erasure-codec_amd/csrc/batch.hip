@@ -1,18 +1,19 @@
-// batch.hip -- the ragged-batch kernel and its host object.
+// batch.hip -- the ragged-batch kernels and their host object.
 //
-// Thousands of classic GF(2^8) stripe operations (encode, verify, reconstruct;
-// any mix of k, r and shard length) are flattened into one table of stripe
-// descriptors plus one table of shard pointers in device memory.  Every stripe
-// is cut into tiles of kBlock 16-byte vectors per shard; the tiles of all
-// stripes form one global index space; the grid strides through it (one
-// workgroup per tile by default, or a persistent grid of a few per CU).
-// A workgroup maps its tile to the stripe by a wave-uniform binary search over
-// the stripes' first-tile offsets (scalar loads), then runs the same
-// register-accumulator matrix apply as the single-stripe kernel with runtime
-// k and r (r <= 8 outputs per descriptor), software-pipelined so that the
-// loads of the next input group are in flight while the current one computes.  The launch sequence (decode solves
-// are enqueued when an op is added; the run is a single kernel) can be captured
-// in a hipGraph and replayed.
+// Thousands of stripe operations (encode, verify, reconstruct; any mix of k, r
+// and shard length) are flattened into one table of stripe descriptors plus one
+// table of shard pointers in device memory.  Every stripe is cut into tiles of
+// kBlock 16-byte vectors per shard; the tiles of all stripes of one element type
+// form one global index space; the grid strides through it (one workgroup per
+// tile by default, or a persistent grid of a few per CU).  A workgroup maps its
+// tile to the stripe by a wave-uniform binary search over the stripes'
+// first-tile offsets (scalar loads), then runs the same register-accumulator
+// matrix apply as the single-stripe kernel with runtime k and r, software-
+// pipelined so that the loads of the next input group are in flight while the
+// current one computes.  One kernel instance per element type: GF(2^8) bytes,
+// GF(2^16) tower pairs, GF(2^16) in Leopard's split layout.  The launch
+// sequence (decode solves are enqueued when an op is added; the run is one
+// kernel per element type present) can be captured in a hipGraph and replayed.
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -71,7 +72,26 @@ __device__ __forceinline__ bool cursor_next(Cursor& c, cstripe_t stripes, int n_
   return cursor_at<NV>(c, c.g + gridDim.x, stripes, n_stripes, total_tiles);
 }
 
-template <int KB, int NV>
+// Byte offset of 16-byte vector v of a shard: contiguous, or in the split layout
+// 8 bytes of low plane at this offset and 8 of high plane 32 bytes on.
+template <int ELEM>
+__device__ __forceinline__ uint64_t vec_offset(uint64_t v) {
+  return ELEM == kElemSplit ? ((v >> 2) << 6) + ((v & 3) << 3) : v * 16;
+}
+
+template <int ELEM>
+__device__ __forceinline__ uint4 load_vec(const uint8_t* p) {
+  if constexpr (ELEM == kElemSplit) return load_split(p);
+  else return load16(p);
+}
+
+template <int ELEM>
+__device__ __forceinline__ void store_vec(uint8_t* p, uint4 v) {
+  if constexpr (ELEM == kElemSplit) store_split(p, v);
+  else store16(p, v);
+}
+
+template <int KB, int NV, int ELEM>
 __device__ __forceinline__ void issue(uint4 (&x)[KB][NV], const Cursor& c, cptr_t ptrs) {
 #pragma unroll
   for (int b = 0; b < KB; ++b) {
@@ -80,7 +100,7 @@ __device__ __forceinline__ void issue(uint4 (&x)[KB][NV], const Cursor& c, cptr_
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
       const uint64_t v = c.vtile + j * kBlock + threadIdx.x;
-      x[b][j] = (i < c.k && v < c.n16) ? load16(src + v * 16) : make_uint4(0, 0, 0, 0);
+      x[b][j] = (i < c.k && v < c.n16) ? load_vec<ELEM>(src + vec_offset<ELEM>(v)) : make_uint4(0, 0, 0, 0);
     }
   }
 }
@@ -89,11 +109,56 @@ __device__ __forceinline__ uint32_t word(const uint4& v, int d) {
   return d == 0 ? v.x : d == 1 ? v.y : d == 2 ? v.z : v.w;
 }
 
+// GF(2^16): each coefficient is four byte maps [A B; C D] on the element planes
+// (x' = A x ^ B c, c' = C x ^ D c; tower pairs de-interleaved into planes, split
+// vectors arrive as planes), 4 * kTabWords table words per coefficient.
+template <int KB, int RMAX, int NV, int ELEM>
+__device__ __forceinline__ void compute16(const uint4 (&x)[KB][NV], const Cursor& c,
+                                          uint32_t (&acc)[RMAX][NV][4]) {
+  constexpr int W = 4 * kTabWords;
+#pragma unroll
+  for (int b = 0; b < KB; ++b) {
+    if (c.i0 + b >= c.k) continue;  // uniform
+    Sel s[NV][4];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      uint32_t w[4];
+      if constexpr (ELEM == kElemTower) {
+        deinterleave(x[b][j], w);
+      } else {
+        w[0] = x[b][j].x;
+        w[1] = x[b][j].y;
+        w[2] = x[b][j].z;
+        w[3] = x[b][j].w;
+      }
+#pragma unroll
+      for (int d = 0; d < 4; ++d) s[j][d] = selectors(w[d]);
+    }
+    ctab_t col = c.tables + (size_t)(c.i0 + b) * W;
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+      if (r >= c.R) continue;
+      ctab_t t = col + (size_t)r * c.stride;
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          acc[r][j][d] ^= gmul(t, s[j][d]) ^ gmul(t + kTabWords, s[j][d + 2]);
+          acc[r][j][d + 2] ^= gmul(t + 2 * kTabWords, s[j][d]) ^ gmul(t + 3 * kTabWords, s[j][d + 2]);
+        }
+    }
+  }
+}
+
 // Inputs are taken in pairs so that the six byte lookups of two coefficients
 // fold into the accumulator with three 3-input XORs.
-template <int KB, int RMAX, int NV>
+template <int KB, int RMAX, int NV, int ELEM>
 __device__ __forceinline__ void compute(const uint4 (&x)[KB][NV], const Cursor& c,
                                         uint32_t (&acc)[RMAX][NV][4]) {
+  if constexpr (ELEM != kElemByte) {
+    compute16<KB, RMAX, NV, ELEM>(x, c, acc);
+    return;
+  }
 #pragma unroll
   for (int b = 0; b < KB; b += 2) {
     const bool has0 = c.i0 + b < c.k, has1 = c.i0 + b + 1 < c.k;  // uniform
@@ -143,7 +208,7 @@ __device__ __forceinline__ void compute(const uint4 (&x)[KB][NV], const Cursor& 
 }
 
 // Writes (or compares) the accumulators when the chunk closes its tile.
-template <int KB, int RMAX, int NV>
+template <int KB, int RMAX, int NV, int ELEM>
 __device__ __forceinline__ void finish(const Cursor& c, uint32_t (&acc)[RMAX][NV][4], cptr_t ptrs,
                                        const BatchStripe* stripes_g) {
   if (c.i0 + KB < c.k) return;
@@ -156,12 +221,14 @@ __device__ __forceinline__ void finish(const Cursor& c, uint32_t (&acc)[RMAX][NV
     for (int j = 0; j < NV; ++j) {
       const uint64_t v = c.vtile + j * kBlock + threadIdx.x;
       if (v >= c.n16) continue;
-      uint8_t* dst = base + v * 16;
-      uint4 o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
+      uint8_t* dst = base + vec_offset<ELEM>(v);
+      uint4 o;
+      if constexpr (ELEM == kElemTower) o = interleave(acc[r][j]);
+      else o = make_uint4(acc[r][j][0], acc[r][j][1], acc[r][j][2], acc[r][j][3]);
       if (c.mode == kWrite) {
-        store16(dst, o);
+        store_vec<ELEM>(dst, o);
       } else {
-        uint4 p = load16(dst);
+        uint4 p = load_vec<ELEM>(dst);
         mismatch |= ((p.x ^ o.x) | (p.y ^ o.y) | (p.z ^ o.z) | (p.w ^ o.w)) != 0;
       }
     }
@@ -179,7 +246,7 @@ __device__ __forceinline__ void finish(const Cursor& c, uint32_t (&acc)[RMAX][NV
 // inputs of its current tile) the 128-bit loads of its next chunk -- the next
 // input group, or the first group of its next tile, possibly in another
 // stripe -- are already in flight in the second register buffer.
-template <int KB, int RMAX, int NV>
+template <int KB, int RMAX, int NV, int ELEM>
 __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchStripe* stripes_g, int n_stripes,
                                                        const void* const* ptrs_g,
                                                        uint64_t total_tiles) {
@@ -195,20 +262,20 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchStripe* stripe
 #pragma unroll
       for (int d = 0; d < 4; ++d) acc[r][j][d] = 0;
   uint4 xa[KB][NV], xb[KB][NV];
-  issue<KB, NV>(xa, cur, ptrs);
+  issue<KB, NV, ELEM>(xa, cur, ptrs);
   while (true) {
     Cursor nxt = cur;
     bool more = cursor_next<KB, NV>(nxt, stripes, n_stripes, total_tiles);
-    if (more) issue<KB, NV>(xb, nxt, ptrs);
-    compute<KB, RMAX, NV>(xa, cur, acc);
-    finish<KB, RMAX, NV>(cur, acc, ptrs, stripes_g);
+    if (more) issue<KB, NV, ELEM>(xb, nxt, ptrs);
+    compute<KB, RMAX, NV, ELEM>(xa, cur, acc);
+    finish<KB, RMAX, NV, ELEM>(cur, acc, ptrs, stripes_g);
     if (!more) break;
     cur = nxt;
     nxt = cur;
     more = cursor_next<KB, NV>(nxt, stripes, n_stripes, total_tiles);
-    if (more) issue<KB, NV>(xa, nxt, ptrs);
-    compute<KB, RMAX, NV>(xb, cur, acc);
-    finish<KB, RMAX, NV>(cur, acc, ptrs, stripes_g);
+    if (more) issue<KB, NV, ELEM>(xa, nxt, ptrs);
+    compute<KB, RMAX, NV, ELEM>(xb, cur, acc);
+    finish<KB, RMAX, NV, ELEM>(cur, acc, ptrs, stripes_g);
     if (!more) break;
     cur = nxt;
   }
@@ -218,10 +285,10 @@ template <int KB, int RMAX>
 static void launch_nv(int nv, unsigned grid, hipStream_t s, const BatchStripe* d_stripes,
                       int n_stripes, const void* const* d_ptrs, uint64_t total_tiles) {
   if (nv == 2)
-    hipLaunchKernelGGL((batch_kernel<KB, RMAX, 2>), dim3(grid), dim3(kBlock), 0, s, d_stripes,
+    hipLaunchKernelGGL((batch_kernel<KB, RMAX, 2, kElemByte>), dim3(grid), dim3(kBlock), 0, s, d_stripes,
                        n_stripes, d_ptrs, total_tiles);
   else
-    hipLaunchKernelGGL((batch_kernel<KB, RMAX, 1>), dim3(grid), dim3(kBlock), 0, s, d_stripes,
+    hipLaunchKernelGGL((batch_kernel<KB, RMAX, 1, kElemByte>), dim3(grid), dim3(kBlock), 0, s, d_stripes,
                        n_stripes, d_ptrs, total_tiles);
 }
 
@@ -236,7 +303,7 @@ static void launch_kb(int kb, int nv, unsigned grid, hipStream_t s, const BatchS
   }
 }
 
-hipError_t launch_batch(const BatchStripe* d_stripes, int n_stripes, const void* const* d_ptrs,
+hipError_t launch_batch(int elem, const BatchStripe* d_stripes, int n_stripes, const void* const* d_ptrs,
                         uint64_t total_tiles, const BatchTune& tune, int n_cu, hipStream_t s) {
   if (total_tiles == 0 || n_stripes == 0) return hipSuccess;
   // blocks_per_cu == 0: one workgroup per tile, handed out in tile order by the
@@ -247,7 +314,13 @@ hipError_t launch_batch(const BatchStripe* d_stripes, int n_stripes, const void*
   if (grid > total_tiles) grid = total_tiles;
   const uint64_t max_grid = (UINT64_C(1) << 32) / kBlock - 1;
   if (grid > max_grid) grid = max_grid;
-  if (tune.rmax <= 4)
+  if (elem == kElemTower)
+    hipLaunchKernelGGL((batch_kernel<8, kMaxOut16, 1, kElemTower>), dim3((unsigned)grid), dim3(kBlock), 0, s,
+                       d_stripes, n_stripes, d_ptrs, total_tiles);
+  else if (elem == kElemSplit)
+    hipLaunchKernelGGL((batch_kernel<8, kMaxOut16, 1, kElemSplit>), dim3((unsigned)grid), dim3(kBlock), 0, s,
+                       d_stripes, n_stripes, d_ptrs, total_tiles);
+  else if (tune.rmax <= 4)
     launch_kb<4>(tune.kb, tune.nv, (unsigned)grid, s, d_stripes, n_stripes, d_ptrs, total_tiles);
   else
     launch_kb<8>(tune.kb, tune.nv, (unsigned)grid, s, d_stripes, n_stripes, d_ptrs, total_tiles);
@@ -280,9 +353,8 @@ struct rsgpu_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
-  // ops the batch kernel does not take (tower field, Leopard GF(2^16), Leopard
-  // GF(2^8) when the FFT serves them): kept and run through their own entry
-  // points on the batch's stream.
+  // ops the batch kernels do not take (Leopard ops that their FFT serves): kept
+  // and run through their own entry points on the batch's stream.
   struct Deferred {
     const rsgpu_codec* codec;
     int op;  // 0 encode, 1 verify, 2 reconstruct
@@ -317,13 +389,13 @@ int batch_verify_stripe(const Codec& c, const rsgpu_cslice* shards, size_t n, in
                         BatchPlan& plan);
 int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int goal,
                              hipStream_t s, BatchPlan& plan);
-// Leopard GF(2^8) codecs: the matrix forms in the batch kernel, or
+// Leopard codecs: the matrix forms in the batch kernels, or
 // RSGPU_ERR_UNSUPPORTED when the op runs as its own FFT launch
 int batch_leopard8_encode(const Codec& c, const rsgpu_slice* shards, size_t n, BatchPlan& plan);
 int batch_leopard8_verify(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* flag,
                           BatchPlan& plan);
-int batch_leopard8_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
-                               BatchPlan& plan);
+int batch_leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
+                              BatchPlan& plan);
 }  // namespace rsgpu
 
 static void invalidate_graph(rsgpu_batch* b) {
@@ -335,16 +407,24 @@ static void invalidate_graph(rsgpu_batch* b) {
 
 static void retile(rsgpu_batch* b) {
   if (b->tune.kb == 12) b->tune.nv = 1;  // the 12-input group is built for one vector per lane
-  const uint64_t tile = (uint64_t)kBlock * b->tune.nv;
-  uint64_t t0 = 0;
   int rmax = 1;
-  for (auto& s : b->plan.stripes) {
-    s.tile0 = t0;
-    t0 += (s.n16 + tile - 1) / tile;
-    rmax = std::max<int>(rmax, s.r);
+  for (int e = 0; e < kNumElems; ++e) {
+    const uint64_t tile = (uint64_t)kBlock * batch_tile_nv(e, b->tune);
+    uint64_t t0 = 0;
+    for (auto& s : b->plan.stripes[e]) {
+      s.tile0 = t0;
+      t0 += (s.n16 + tile - 1) / tile;
+      if (e == kElemByte) rmax = std::max<int>(rmax, s.r);
+    }
+    b->plan.total_tiles[e] = t0;
   }
-  b->plan.total_tiles = t0;
   b->tune.rmax = rmax <= 4 ? 4 : 8;
+}
+
+static size_t n_descriptors(const BatchPlan& p) {
+  size_t n = 0;
+  for (auto& v : p.stripes) n += v.size();
+  return n;
 }
 
 namespace rsgpu {
@@ -375,7 +455,8 @@ static int upload(rsgpu_batch* b, hipStream_t s) {
   }
   retile(b);
   auto& p = b->plan;
-  const size_t sb = std::max<size_t>(p.stripes.size(), 1) * sizeof(BatchStripe);
+  const size_t nd = n_descriptors(p);
+  const size_t sb = std::max<size_t>(nd, 1) * sizeof(BatchStripe);
   const size_t pb = std::max<size_t>(p.ptrs.size(), 1) * sizeof(void*);
   // grow geometrically; an outgrown buffer is parked, not freed
   auto grow = [&](DeviceBuffer& d, size_t need) -> bool {
@@ -404,11 +485,15 @@ static int upload(rsgpu_batch* b, hipStream_t s) {
       return RSGPU_ERR_OUT_OF_MEMORY;
     b->h_stage_bytes = want;
   }
-  std::memcpy(b->h_stage, p.stripes.data(), p.stripes.size() * sizeof(BatchStripe));
+  // descriptors of the element types back to back, in kernel order
+  size_t at = 0;
+  for (auto& v : p.stripes) {
+    std::memcpy(b->h_stage + at * sizeof(BatchStripe), v.data(), v.size() * sizeof(BatchStripe));
+    at += v.size();
+  }
   std::memcpy(b->h_stage + sb, p.ptrs.data(), p.ptrs.size() * sizeof(void*));
-  if (!p.stripes.empty() &&
-      hipMemcpyAsync(b->d_stripes.p, b->h_stage, p.stripes.size() * sizeof(BatchStripe),
-                     hipMemcpyHostToDevice, s) != hipSuccess)
+  if (nd && hipMemcpyAsync(b->d_stripes.p, b->h_stage, nd * sizeof(BatchStripe), hipMemcpyHostToDevice,
+                           s) != hipSuccess)
     return RSGPU_ERR_HIP;
   if (!p.ptrs.empty() && hipMemcpyAsync(b->d_ptrs.p, b->h_stage + sb, p.ptrs.size() * sizeof(void*),
                                         hipMemcpyHostToDevice, s) != hipSuccess)
@@ -444,11 +529,17 @@ static int mark_ran(rsgpu_batch* b, hipStream_t s) {
 
 static int launch_main(rsgpu_batch* b, hipStream_t s) {
   auto& p = b->plan;
-  hipError_t e = launch_batch((const BatchStripe*)b->d_stripes.p, (int)p.stripes.size(),
-                              (const void* const*)b->d_ptrs.p, p.total_tiles, b->tune, b->n_cu, s);
-  if (e != hipSuccess) {
-    set_hip_error(e, "batch_kernel");
-    return RSGPU_ERR_HIP;
+  hipError_t e = hipSuccess;
+  size_t at = 0;
+  for (int el = 0; el < kNumElems; ++el) {
+    const size_t n = p.stripes[el].size();
+    if (n && (e = launch_batch(el, (const BatchStripe*)b->d_stripes.p + at, (int)n,
+                               (const void* const*)b->d_ptrs.p, p.total_tiles[el], b->tune, b->n_cu, s)) !=
+                 hipSuccess) {
+      set_hip_error(e, "batch_kernel");
+      return RSGPU_ERR_HIP;
+    }
+    at += n;
   }
   for (auto& j : p.byte_jobs) {
     e = launch_apply_bytes(j, s);
@@ -485,9 +576,7 @@ int rsgpu_batch_clear(rsgpu_batch* b) {
   return RSGPU_OK;
 }
 
-static bool batch_kernel_ok(const rsgpu_codec* c) {
-  return c->c.family == RSGPU_FAMILY_CLASSIC && c->c.field == RSGPU_FIELD_GF8;
-}
+static bool batch_kernel_ok(const rsgpu_codec* c) { return c->c.family == RSGPU_FAMILY_CLASSIC; }
 
 int rsgpu_batch_add_encode(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_slice* shards, size_t n) {
   if (!b || !c || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
@@ -535,8 +624,8 @@ int rsgpu_batch_add_verify(rsgpu_batch* b, const rsgpu_codec* c, const rsgpu_csl
 int rsgpu_batch_add_reconstruct(rsgpu_batch* b, const rsgpu_codec* c, rsgpu_slot* shards, size_t n,
                                 int32_t data_only, void* stream) {
   if (!b || !c || !shards) return RSGPU_ERR_INVALID_ARGUMENT;
-  if (c->c.family == RSGPU_FAMILY_LEOPARD_GF8) {
-    const int rc = batch_leopard8_reconstruct(c->c, shards, n, data_only ? 1 : 0, (hipStream_t)stream, b->plan);
+  if (c->c.family != RSGPU_FAMILY_CLASSIC) {
+    const int rc = batch_leopard_reconstruct(c->c, shards, n, data_only ? 1 : 0, (hipStream_t)stream, b->plan);
     if (rc != RSGPU_ERR_UNSUPPORTED) {
       if (!rc) b->dirty = true;
       return rc;
@@ -638,9 +727,11 @@ int rsgpu_batch_stats(const rsgpu_batch* b, uint64_t* n_ops, uint64_t* bytes_mov
   if (bytes_moved) *bytes_moved = b->plan.bytes_moved;
   if (data_bytes) *data_bytes = b->plan.data_bytes;
   if (tiles) {
-    const uint64_t tile = (uint64_t)kBlock * b->tune.nv;
     uint64_t t = 0;
-    for (auto& s : b->plan.stripes) t += (s.n16 + tile - 1) / tile;
+    for (int e = 0; e < kNumElems; ++e) {
+      const uint64_t tile = (uint64_t)kBlock * batch_tile_nv(e, b->tune);
+      for (auto& s : b->plan.stripes[e]) t += (s.n16 + tile - 1) / tile;
+    }
     *tiles = t;
   }
   return RSGPU_OK;

@@ -312,13 +312,24 @@ static int run_apply(const uint32_t* d_tables, int k_total, int r_total, bool to
   if (!inline_tabs && !d_tables) return RSGPU_ERR_UNSUPPORTED;
   // the byte kernel alone does not clear a verify flag
   if (mode == kVerify && vc && !n16) HIP_TRY(launch_fill_i32(flag, 0, 1, s));
-  // Default tiling: GF(2^8) one dword per lane (apply_dword_kernel); tower
-  // elements 256-lane workgroups with every output, two vectors per lane only
-  // when that still leaves >= 8 workgroups per CU.
+  // Default tiling: GF(2^8) one dword per lane (apply_dword_kernel).  GF(2^16)
+  // elements take four byte maps per coefficient, so a workgroup holding every
+  // output runs short of registers (two waves per SIMD at 4 outputs, one at 8) and
+  // of workgroups on short shards: one output per workgroup up to 256 KiB
+  // (64-lane tiles up to 64 KiB), two above; two vectors per lane from 16 MiB.
+  // Measured per call (back to back) on MI355X against every output per
+  // workgroup: 20+8 encode at 16 KiB 6.5 us against 23.6, 10+4 at 256 KiB 4.8
+  // against 8.5, 10+4 at 1 MiB 8.1 against 9.2; at 4 MiB within the run-to-run
+  // spread (about 10%) of the fastest tiling.
   int tile, split;
   if (variant < 0 || variant >= kNumApplyVariants) {
-    tile = tower ? (n16 >= (uint64_t)kBlock * 2 * 2048 ? 1 : 0) : kTileDword;
-    split = 0;
+    if (tower) {
+      tile = len <= ((size_t)64 << 10) ? 2 : (len >= ((size_t)16 << 20) ? 1 : 0);
+      split = len <= ((size_t)256 << 10) ? 1 : 2;
+    } else {
+      tile = kTileDword;
+      split = 0;
+    }
   } else {
     tile = variant % kNumTiles;
     split = kRowSplit[variant / kNumTiles];
@@ -1578,36 +1589,40 @@ int leopard_decoder_matrix_reconstruct(const Codec& c, rsgpu_slot* shards, size_
 }
 
 // ------------------------------------------------------------ batch stripes
-// Adds one stripe operation to a batch: descriptors of at
-// most kMaxOut outputs each, plus byte-kernel jobs for unaligned views and for
-// the length % 16 tail.
+// Adds one stripe operation to a batch: descriptors of at most kMaxOut
+// (GF(2^16): kMaxOut16) outputs each, plus byte-kernel jobs for unaligned views
+// and for the length % 16 tail.  The split layout takes whole aligned 64-byte
+// blocks only (Leopard shard lengths are multiples of 64).
 static int add_stripe(BatchPlan& plan, const std::vector<const uint8_t*>& in,
                       const std::vector<uint8_t*>& out, const uint32_t* tables, size_t len, int mode,
-                      int32_t* flag) {
+                      int32_t* flag, int elem = kElemByte) {
   const int k = (int)in.size(), r = (int)out.size();
+  const int W = words_per_coef(elem != kElemByte);
+  const int rstep = elem == kElemByte ? kMaxOut : kMaxOut16;
   bool vec_ok = true;
   for (auto p : in) vec_ok &= aligned16(p);
   for (auto p : out) vec_ok &= aligned16(p);
+  if (elem == kElemSplit && (!vec_ok || len % 64)) return RSGPU_ERR_UNSUPPORTED;
   const uint64_t n16 = vec_ok ? len / 16 : 0;
   const size_t tail = (size_t)n16 * 16;
   // a verify tail compares the complete sum, so its inputs cannot be split
   if (tail < len && k > kMaxIn && mode == kVerify) return RSGPU_ERR_UNSUPPORTED;
-  for (int r0 = 0; r0 < r; r0 += kMaxOut) {
-    const int rn = std::min(kMaxOut, r - r0);
+  for (int r0 = 0; r0 < r; r0 += rstep) {
+    const int rn = std::min(rstep, r - r0);
     if (n16) {
       BatchStripe s{};
       s.n16 = n16;
-      s.tables = tables + (size_t)r0 * k * kTabWords;
+      s.tables = tables + (size_t)r0 * k * W;
       s.flag = flag;
       s.ptr0 = (uint32_t)plan.ptrs.size();
-      s.tab_row_stride = (uint32_t)(k * kTabWords);
+      s.tab_row_stride = (uint32_t)(k * W);
       s.k = (uint16_t)k;
       s.r = (uint16_t)rn;
       s.mode = (uint16_t)mode;
-      s.tower = 0;
+      s.elem = (uint16_t)elem;
       for (auto p : in) plan.ptrs.push_back(p);
       for (int j = 0; j < rn; ++j) plan.ptrs.push_back(out[r0 + j]);
-      plan.stripes.push_back(s);
+      plan.stripes[elem].push_back(s);
     }
     // the len % 16 tail (or all of an unaligned stripe) in byte jobs of at most
     // kMaxIn inputs; after the first, each accumulates into the outputs
@@ -1616,16 +1631,16 @@ static int add_stripe(BatchPlan& plan, const std::vector<const uint8_t*>& in,
       const int kn = std::min(kMaxIn, k - k0);
       for (int i = 0; i < kn; ++i) b.in[i] = in[k0 + i];
       for (int j = 0; j < rn; ++j) b.out[j] = out[r0 + j];
-      b.tables = tables + (size_t)r0 * k * kTabWords;
+      b.tables = tables + (size_t)r0 * k * W;
       b.flag = flag;
       b.begin = tail;
       b.end = len;
       b.k = kn;
-      b.tab_row_stride = k * kTabWords;
+      b.tab_row_stride = k * W;
       b.tab_col0 = k0;
       b.r = rn;
       b.mode = (k0 == 0 || mode == kVerify) ? mode : kAccumulate;
-      b.tower = 0;
+      b.tower = elem == kElemTower ? 1 : 0;
       plan.byte_jobs.push_back(b);
     }
   }
@@ -1634,6 +1649,9 @@ static int add_stripe(BatchPlan& plan, const std::vector<const uint8_t*>& in,
   plan.n_ops++;
   return RSGPU_OK;
 }
+
+// A classic codec's element type in the batch kernels.
+static int batch_elem(const Codec& c) { return c.field == RSGPU_FIELD_GF16 ? kElemTower : kElemByte; }
 
 int batch_encode_stripe(const Codec& c, const rsgpu_slice* shards, size_t n, BatchPlan& plan) {
   const size_t total = c.k + c.m;
@@ -1647,7 +1665,9 @@ int batch_encode_stripe(const Codec& c, const rsgpu_slice* shards, size_t n, Bat
   for (size_t j = 0; j < c.m; ++j) out[j] = shards[c.k + j].ptr;
   for (size_t i = 0; i < n; ++i)
     if (!shards[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
-  return add_stripe(plan, in, out, (const uint32_t*)c.d_parity_tables.p, shards[0].len, kWrite, nullptr);
+  if ((rc = check_elem_len(c, shards[0].len))) return rc;
+  return add_stripe(plan, in, out, (const uint32_t*)c.d_parity_tables.p, shards[0].len, kWrite, nullptr,
+                    batch_elem(c));
 }
 
 int batch_verify_stripe(const Codec& c, const rsgpu_cslice* shards, size_t n, int32_t* flag,
@@ -1663,7 +1683,9 @@ int batch_verify_stripe(const Codec& c, const rsgpu_cslice* shards, size_t n, in
   for (size_t j = 0; j < c.m; ++j) out[j] = const_cast<uint8_t*>(shards[c.k + j].ptr);
   for (size_t i = 0; i < n; ++i)
     if (!shards[i].ptr) return RSGPU_ERR_INVALID_ARGUMENT;
-  return add_stripe(plan, in, out, (const uint32_t*)c.d_parity_tables.p, shards[0].len, kVerify, flag);
+  if ((rc = check_elem_len(c, shards[0].len))) return rc;
+  return add_stripe(plan, in, out, (const uint32_t*)c.d_parity_tables.p, shards[0].len, kVerify, flag,
+                    batch_elem(c));
 }
 
 int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
@@ -1677,23 +1699,29 @@ int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int g
   std::vector<uint8_t*> out(p.targets.size());
   for (size_t i = 0; i < c.k; ++i) in[i] = shards[p.valid[i]].ptr;
   for (size_t t = 0; t < p.targets.size(); ++t) out[t] = shards[p.targets[t]].ptr;
-  if ((rc = add_stripe(plan, in, out, e->tables, p.shard_len, kWrite, nullptr)))
+  if ((rc = add_stripe(plan, in, out, e->tables, p.shard_len, kWrite, nullptr, batch_elem(c))))
     return rc;
   plan.keep.push_back(e);
   for (int t : p.targets) shards[t].present = 1;
   return RSGPU_OK;
 }
 
-// ------------------------------------------- Leopard GF(2^8) in the batch kernel
+// ------------------------------------------- Leopard codecs in the batch kernels
 // The matrix forms of a Leopard GF(2^8) codec are GF(2^8) byte applies with
 // tables in Leopard's field, bit-identical to the FFT: encode and verify over
 // [I; G], reconstruct with exactly k shards present as the [I; G] solve, with
 // more as the decoder's own matrix over every present shard.  A batch takes them
 // into its kernel when the codec's dispatch picks a matrix variant for the call,
 // or, with no dispatch state, for k <= 32, where the matrix form measured about
-// twice the FFT's rate per call; otherwise RSGPU_ERR_UNSUPPORTED tells the batch
-// to run the FFT through its own launch.
-static bool leo8_batch_matrix(const Codec& c, int op, size_t erasures, size_t len) {
+// twice the FFT's rate per call.  A Leopard GF(2^16) reconstruct goes into the
+// split-layout kernel as the decoder's own matrix when the dispatch picks a
+// matrix variant for it (at most 32 erasures).  Otherwise RSGPU_ERR_UNSUPPORTED
+// tells the batch to run the FFT through its own launch.
+static bool leo_batch_matrix(const Codec& c, int op, size_t erasures, size_t len) {
+  if (c.family == RSGPU_FAMILY_LEOPARD_GF16) {
+    const int v = pick_variant(c, op, erasures, len);
+    return v >= 0 && v < kNumApplyVariants;
+  }
   if (c.family != RSGPU_FAMILY_LEOPARD_GF8) return false;
   const bool no_state = c.forced_variant.load(std::memory_order_relaxed) < 0 &&
                         !std::atomic_load(&c.variant_table);
@@ -1708,7 +1736,7 @@ int batch_leopard8_encode(const Codec& c, const rsgpu_slice* shards, size_t n, B
   int rc = check_slices(shards, n);
   if (rc) return rc;
   if (shards[0].len % 64) return RSGPU_INCORRECT_SHARD_SIZE;  // leopard.rs:50-56
-  if (!leo8_batch_matrix(c, kOpEncode, 0, shards[0].len)) return RSGPU_ERR_UNSUPPORTED;
+  if (!leo_batch_matrix(c, kOpEncode, 0, shards[0].len)) return RSGPU_ERR_UNSUPPORTED;
   std::shared_ptr<Codec> mc;
   if ((rc = leo_matrix_codec(c, nullptr, mc))) return rc;
   return batch_encode_stripe(*mc, shards, n, plan);
@@ -1722,16 +1750,16 @@ int batch_leopard8_verify(const Codec& c, const rsgpu_cslice* shards, size_t n, 
   int rc = check_slices(shards, n);
   if (rc) return rc;
   if (shards[0].len % 64) return RSGPU_INCORRECT_SHARD_SIZE;
-  if (!leo8_batch_matrix(c, kOpVerify, 0, shards[0].len)) return RSGPU_ERR_UNSUPPORTED;
+  if (!leo_batch_matrix(c, kOpVerify, 0, shards[0].len)) return RSGPU_ERR_UNSUPPORTED;
   std::shared_ptr<Codec> mc;
   if ((rc = leo_matrix_codec(c, nullptr, mc))) return rc;
   return batch_verify_stripe(*mc, shards, n, flag, plan);
 }
 
 // Validation as leopard_reconstruct (reconstruct.rs:795-1040) for goals 0 (all)
-// and 1 (data only).
-int batch_leopard8_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
-                               BatchPlan& plan) {
+// and 1 (data only); both Leopard fields.
+int batch_leopard_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int goal, hipStream_t s,
+                              BatchPlan& plan) {
   const size_t total = c.k + c.m;
   if (n < total) return RSGPU_TOO_FEW_SHARDS;
   if (n > total) return RSGPU_TOO_MANY_SHARDS;
@@ -1756,9 +1784,14 @@ int batch_leopard8_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int
   }
   if (!need) return RSGPU_OK;
   if (total - present > c.m) return RSGPU_TOO_FEW_SHARDS_PRESENT;
-  if (!leo8_batch_matrix(c, kOpReconstruct, total - present, len)) return RSGPU_ERR_UNSUPPORTED;
+  if (!leo_batch_matrix(c, kOpReconstruct, total - present, len)) return RSGPU_ERR_UNSUPPORTED;
+  const bool g16 = c.family == RSGPU_FAMILY_LEOPARD_GF16;
+  if (g16) {  // the split-layout kernel takes aligned shards only
+    for (size_t i = 0; i < total; ++i)
+      if ((flags[i] & 3) && !aligned16(shards[i].ptr)) return RSGPU_ERR_UNSUPPORTED;
+  }
   int rc;
-  if (present == c.k) {
+  if (present == c.k && !g16) {
     std::shared_ptr<Codec> mc;
     if ((rc = leo_matrix_codec(c, s, mc))) return rc;
     return batch_reconstruct_stripe(*mc, shards, n, goal, s, plan);
@@ -1779,7 +1812,9 @@ int batch_leopard8_reconstruct(const Codec& c, rsgpu_slot* shards, size_t n, int
   std::vector<uint8_t*> out;
   for (int i : dm->inputs) in.push_back(shards[i].ptr);
   for (int t : dm->targets) out.push_back(shards[t].ptr);
-  if ((rc = add_stripe(plan, in, out, (const uint32_t*)dm->tables.p, len, kWrite, nullptr))) return rc;
+  if ((rc = add_stripe(plan, in, out, (const uint32_t*)dm->tables.p, len, kWrite, nullptr,
+                       g16 ? kElemSplit : kElemByte)))
+    return rc;
   plan.keep.push_back(std::const_pointer_cast<DecoderMatrix>(dm));
   for (int t : dm->targets) shards[t].present = 1;
   return RSGPU_OK;

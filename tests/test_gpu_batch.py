@@ -136,8 +136,8 @@ def test_every_launch_shape_is_bit_exact(kb, nv, bpc):
             assert (to_host(sh[it["k"] + j]) == it["parity"][j]).all(), (it["k"], it["m"], it["length"], j)
 
 
-def test_batch_with_tower_and_leopard_codecs_defers():
-    announce("batch deferred tower op")
+def test_batch_tower_encode_small():
+    announce("batch tower encode 5+3")
     k, m, n = 5, 3, 256
     rs16 = rsgpu.ReedSolomon(k, m, field="gf16")
     _, omat = O.build_matrix(O.GF16, O.VANDERMONDE, k, m)
@@ -211,6 +211,102 @@ def test_leopard_gf8_ops_in_the_batch_kernel_match_the_fft():
         else:
             for i, w in want:
                 assert obj[i].present and (to_host(obj[i].data) == w).all(), (kind, i)
+
+
+def _dev_view(host, off):
+    """A device copy of `host` starting `off` bytes into its allocation."""
+    buf = torch.zeros(host.size + off, dtype=torch.uint8, device="cuda")
+    v = buf[off:off + host.size]
+    v.copy_(torch.from_numpy(host))
+    return v
+
+
+def test_classic_gf16_ops_in_the_tower_batch_kernel_match_the_oracle():
+    """Classic GF(2^16) encode, verify and reconstruct (all and data only) run in
+    the tower batch kernel beside GF(2^8) stripes: more outputs than one
+    descriptor holds, more inputs than one apply launch takes, 16-byte tails and
+    unaligned views through the byte kernel.  Reconstructs take non-codeword
+    inputs, where the survivors chosen decide the bytes.  Every output equals the
+    oracle's, and a corrupted stripe fails its verify."""
+    announce("batch classic gf16 tower kernel")
+    rng = np.random.default_rng(16)
+    b = rsgpu.Batch()
+    checks = []
+    for k, m, n, off in [(10, 4, 4096, 0), (5, 3, 2006, 0), (12, 6, 2048, 2), (20, 8, 65538, 0),
+                         (70, 6, 4096, 0), (3, 5, 256, 0), (8, 3, 1 << 20, 0)]:
+        for field in ("gf16", "gf8") if k == 10 else ("gf16",):
+            F = O.GF16 if field == "gf16" else O.GF8
+            rs = rsgpu.ReedSolomon(k, m, field=field)
+            mat = O.build_matrix(F, O.VANDERMONDE, k, m)[1]
+            data = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k)]
+            par = O.encode(F, mat, k, m, data)
+            sh = [_dev_view(d, off) for d in data] + [_dev_view(np.zeros(n, np.uint8), off) for _ in range(m)]
+            b.add_encode(rs, sh)
+            checks.append(("encode", sh, [(k + j, par[j]) for j in range(m)]))
+            full = data + list(par)
+            flags = torch.zeros(2, dtype=torch.int32, device="cuda")
+            good = [_dev_view(x, off) for x in full]
+            bad = [_dev_view(x, off) for x in full]
+            bad[rng.integers(k + m)][int(rng.integers(n))] ^= 0x40
+            b.add_verify(rs, good, flags[0:1])
+            b.add_verify(rs, bad, flags[1:2])
+            checks.append(("verify", flags, None))
+            for data_only in (False, True):
+                junk = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
+                e = int(rng.integers(1, m + 1))
+                miss = sorted(rng.choice(k + m, e, replace=False).tolist())
+                present = [0 if i in miss else 1 for i in range(k + m)]
+                rc, exp = O.reconstruct(F, mat, k, m, [x.copy() for x in junk], present, data_only)
+                assert rc == 0
+                slots = [rsgpu.ShardSlot(_dev_view(np.zeros(n, np.uint8), off), False) if i in miss
+                         else rsgpu.ShardSlot(_dev_view(junk[i], off)) for i in range(k + m)]
+                b.add_reconstruct(rs, slots, data_only=data_only)
+                want = [i for i in miss if i < k or not data_only]
+                checks.append(("rec", slots, [(i, exp[i]) for i in want]))
+    assert b.stats()["tiles"] > 0
+    b.run()
+    torch.cuda.synchronize()
+    for kind, obj, want in checks:
+        if kind == "encode":
+            for i, w in want:
+                assert (to_host(obj[i]) == w).all(), (kind, i)
+        elif kind == "verify":
+            assert int(obj[0]) == 0 and int(obj[1]) != 0
+        else:
+            for i, w in want:
+                assert obj[i].present and (to_host(obj[i].data) == w).all(), (kind, i)
+
+
+def test_leopard_gf16_reconstruct_in_the_split_batch_kernel_matches_the_oracle():
+    """A Leopard GF(2^16) reconstruct whose dispatch picks a matrix variant runs
+    as the decoder's own matrix in the split-layout batch kernel (more targets than
+    one descriptor holds, thousands of inputs); one with more erasures than the
+    matrix form takes runs its FFT after the kernels.  Inputs are not codewords;
+    every output equals the oracle's FFT decoder."""
+    announce("batch leopard gf16 split kernel")
+    rng = np.random.default_rng(61)
+    b = rsgpu.Batch()
+    checks = []
+    for k, m, n, e in [(10, 4, 4096, 1), (10, 4, 640, 3), (64, 16, 4096, 16), (300, 45, 640, 9),
+                       (1024, 256, 4096, 2), (300, 45, 640, 40)]:
+        rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf16")
+        rs.set_variant(0)  # a matrix variant: taken by reconstructs of at most 32 erasures
+        junk = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
+        miss = sorted(rng.choice(k + m, e, replace=False).tolist())
+        present = [0 if i in miss else 1 for i in range(k + m)]
+        rc, exp = O.leo_reconstruct(16, k, m, [x.copy() for x in junk], present)
+        assert rc == 0
+        slots = [rsgpu.ShardSlot(torch.zeros(n, dtype=torch.uint8, device="cuda"), False) if i in miss
+                 else rsgpu.ShardSlot(torch.from_numpy(junk[i].copy()).cuda()) for i in range(k + m)]
+        b.add_reconstruct(rs, slots)
+        assert all(slots[i].present for i in miss)
+        checks.append((slots, [(i, exp[i]) for i in miss]))
+    assert b.stats()["tiles"] > 0
+    b.run()
+    torch.cuda.synchronize()
+    for slots, want in checks:
+        for i, w in want:
+            assert (to_host(slots[i].data) == w).all(), i
 
 
 def test_ragged_workload_plain_equals_batched():
