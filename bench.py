@@ -241,6 +241,76 @@ def gf16_batch_section(torch, rsgpu, steps):
     return out
 
 
+def config1_section(torch, rsgpu, steps):
+    """64 stripes of RS 10+4 x 64 KiB: encode, verify, and reconstruct of 4 erasures
+    per stripe (positions uniform over the 14 shards), batched (one batch-kernel launch
+    per step, hipGraph replay) and plain (one call per stripe from Python).  Encoded
+    shards and rebuilt shards are overwritten with a pattern before each timed run and
+    compared with the codeword after it; verify runs on codewords and must pass."""
+    import ctypes as C
+
+    import numpy as np
+
+    sp = torch.cuda.current_stream().cuda_stream
+    k, m, n, S = 10, 4, 64, 64 << 10
+    data = n * k * S
+    rate = lambda t: round(data / t / 1e9, 2)
+    st = Stripes(rsgpu, torch, k, m, n, S, seed=0x00C0F161)
+    out = {"stripes": n, "shard_bytes": S}
+    st.buf[:, k:].fill_(0xA5)
+    t_plain = timed(torch, lambda: [st.encode(s, sp) for s in range(n)], steps, 2)
+    ref = st.buf.clone()
+    b = st.batch("encode", sp)
+    b.autotune(reps=3)
+    st.buf[:, k:].fill_(0xA5)
+    t_batch = timed(torch, b.graph_launch, steps, 2)
+    assert torch.equal(st.buf, ref), "config 1 encode: batched differs from plain"
+    out["encode"] = {"plain_gbs": rate(t_plain), "batched_gbs": rate(t_batch)}
+    del b
+    flags = torch.zeros(n, dtype=torch.int32, device="cuda")
+    t_plain = timed(torch, lambda: [st.verify_async(s, flags.data_ptr() + 4 * s, sp) for s in range(n)],
+                    steps, 2)
+    assert int(flags.sum()) == 0, "config 1 verify (plain) reported a mismatch"
+    b = st.batch("verify", sp, flags=flags)
+    b.autotune(reps=3)
+
+    def ver_batch():
+        flags.zero_()
+        b.graph_launch()
+
+    t_batch = timed(torch, ver_batch, steps, 2)
+    assert int(flags.sum()) == 0, "config 1 verify (batched) reported a mismatch"
+    out["verify"] = {"plain_gbs": rate(t_plain), "batched_gbs": rate(t_batch)}
+    del b
+    rng = np.random.default_rng(0xC1)
+    er = [set(rng.choice(k + m, 4, replace=False).tolist()) for _ in range(n)]
+    layouts = [st.slots(s, er[s]) for s in range(n)]
+    work = [st.slots(s, er[s]) for s in range(n)]
+    nbytes = C.sizeof(layouts[0])
+
+    def rec_plain():
+        for s in range(n):
+            C.memmove(work[s], layouts[s], nbytes)  # presence as laid out (a call marks targets)
+            st.reconstruct(work[s], sp)
+
+    def poison():
+        for s in range(n):
+            for i in er[s]:
+                st.buf[s, i].fill_(0x5A)
+
+    poison()
+    t_plain = timed(torch, rec_plain, steps, 2)
+    assert torch.equal(st.buf, ref), "config 1 reconstruct: plain differs from the codeword"
+    b = st.batch("reconstruct", sp, erasures=er)
+    b.autotune(reps=3)
+    poison()
+    t_batch = timed(torch, b.graph_launch, steps, 2)
+    assert torch.equal(st.buf, ref), "config 1 reconstruct: batched differs from the codeword"
+    out["reconstruct_4_erasures"] = {"plain_gbs": rate(t_plain), "batched_gbs": rate(t_batch)}
+    del b, ref, st
+    return out
+
+
 def timed(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -592,7 +662,8 @@ def main():
         import rsgpu
 
         torch.cuda.init()
-        print(json.dumps({"gf16_batch": gf16_batch_section(torch, rsgpu, 10)}))
+        print(json.dumps({"gf16_batch": gf16_batch_section(torch, rsgpu, 10),
+                          "config1_64x10+4x64KiB": config1_section(torch, rsgpu, args.steps)}))
         return
     if args.mfma_only:
         import torch
@@ -786,6 +857,7 @@ def main():
 
     leopard = leopard_section(torch, rsgpu, DSP, disp) if args.leopard else None
     gf16_batch = gf16_batch_section(torch, rsgpu, 10)
+    config1 = config1_section(torch, rsgpu, args.steps)
     gf2_mfma = gf2_mfma_section(torch, rsgpu)
     dispatch_live = None
     if args.dispatch_live:
@@ -829,6 +901,7 @@ def main():
         "reconstruct_batched_bytes_moved_gbs": gbs(rec_moved, t_rec_batch),
         "encode_8x3_batched_gbs": gbs(N * 8 * S, t83_batch),
         "encode_8x3_plain_gbs": gbs(N * 8 * S, t83_plain),
+        "config1_64x10+4x64KiB": config1,
         "ragged": ragged,
         "dispatch_live": dispatch_live,
         "leopard": leopard,

@@ -437,7 +437,10 @@ int rsgpu_batch_set_tune(rsgpu_batch* batch, int32_t inputs_per_group, int32_t v
 int rsgpu_batch_autotune(rsgpu_batch* batch, void* stream, int32_t reps, int32_t* inputs_per_group,
                          int32_t* vectors_per_lane, int32_t* blocks_per_cu, float* ms);
 int rsgpu_batch_run(rsgpu_batch* batch, void* stream);
-/* Same work, captured once into a hipGraph and replayed on later calls. */
+/* Same work, captured once into a hipGraph and replayed on later calls.  A batch
+ * that is a single kernel launch is launched directly (a replay only adds its
+ * start-up), and one holding ops run through their own kernels runs as
+ * rsgpu_batch_run. */
 int rsgpu_batch_graph_launch(rsgpu_batch* batch, void* stream);
 int rsgpu_batch_stats(const rsgpu_batch* batch, uint64_t* n_ops, uint64_t* bytes_moved,
                       uint64_t* data_bytes, uint64_t* tiles);

@@ -683,6 +683,14 @@ int rsgpu_batch_graph_launch(rsgpu_batch* b, void* stream) {
   int rc = upload(b, s);
   if (rc) return rc;
   if (!b->deferred.empty()) return rsgpu_batch_run(b, stream);  // deferred ops sync on the host
+  // A batch that is one kernel launch runs it directly: a graph replay adds its own
+  // start-up (64 x 10+4 x 64 KiB encode: 27 us per step replayed, 18.7 direct).
+  size_t launches = b->plan.byte_jobs.size();
+  for (auto& v : b->plan.stripes) launches += v.empty() ? 0 : 1;
+  if (launches <= 1) {
+    if ((rc = launch_main(b, s))) return rc;
+    return mark_ran(b, s);
+  }
   if (!b->exec) {
     // Capture on the batch's own stream (the legacy default stream cannot be
     // captured); the instantiated graph is then launched on the caller's stream.
