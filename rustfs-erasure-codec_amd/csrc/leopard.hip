@@ -1188,18 +1188,23 @@ static uint32_t tile_words(const LeoState& s, uint32_t rows, uint64_t words, uin
 
 // Threads per workgroup.  The GF8 kernels take 72 VGPRs, so two 512-thread
 // workgroups share a CU at 4 waves per SIMD.  The GF16 kernels take about 137
-// (3 waves per SIMD at most).  GF16 encode runs 256 threads while two workgroups
-// fit the LDS and 512 when one fills it.  GF16 decode is built to 128 VGPRs (a
-// small spill) for 4 waves per SIMD: 512 threads while two workgroups fit, 1024
-// when one fills the LDS.  Measured on MI355X against 256 threads everywhere:
-// GF8 128+64 encode / reconstruct 672 -> 787 / 261 -> 357 GB/s, GF16 300+45
-// reconstruct 268 -> 315, GF16 1024+256 reconstruct 140 -> 207, GF16 4096+1024
-// encode 418 -> 505.
+// (3 waves per SIMD at most) unless built to 128 (a small spill) for 4 waves per
+// SIMD.  GF16 encode runs 512 threads: built to 128 VGPRs while two workgroups
+// fit the LDS, at its natural count when one fills it.  GF16 decode is built to
+// 128 VGPRs: 512 threads while two workgroups fit, 1024 when one fills the LDS.
+// Measured on MI355X against 256 threads everywhere: GF8 128+64 encode /
+// reconstruct 672 -> 787 / 261 -> 357 GB/s, GF16 300+45 reconstruct 268 -> 315,
+// GF16 1024+256 reconstruct 140 -> 207, GF16 4096+1024 encode 418 -> 505; GF16
+// encode at 512 threads and 128 VGPRs against 256 threads at 136 (two
+// workgroups per CU): 1024+256 x 256 KiB 302 -> 289 us, 300+45 x 256 KiB 81.5 ->
+// 76.4, 64+16 x 256 KiB 18.1 -> 16.6, 128+64 x 256 KiB 37.7 -> 36.4.  (Half the
+// threads of a 1024+256 tile idle in its layers; two-word items to occupy them
+// ran 16% slower: each item loads its twiddle tables for half the words.)
 static int leo_threads(const LeoState& s, size_t lds, bool decode) {
   if (!s.gf16) return 512;
   const bool one_fills = 2 * lds > s.lds_limit;
   if (decode) return one_fills ? 1024 : 512;
-  return one_fills ? 512 : 256;
+  return 512;
 }
 
 // Bytes of LDS a plan takes when copied there, or 0 for a plan kept in global
@@ -1507,8 +1512,8 @@ static int launch_encode(const Codec& c, const std::vector<const uint8_t*>& ptrs
   const unsigned grid = (unsigned)((words + C - 1) / C);
   const int nt = leo_threads(st, lds, false);
   if (!st.gf16) hipLaunchKernelGGL((leo_encode_kernel<false, 512>), dim3(grid), dim3(nt), lds, s, a);
-  else if (nt == 512) hipLaunchKernelGGL((leo_encode_kernel<true, 512>), dim3(grid), dim3(nt), lds, s, a);
-  else hipLaunchKernelGGL((leo_encode_kernel<true, 256>), dim3(grid), dim3(nt), lds, s, a);
+  else if (2 * lds > st.lds_limit) hipLaunchKernelGGL((leo_encode_kernel<true, 512>), dim3(grid), dim3(nt), lds, s, a);
+  else hipLaunchKernelGGL((leo_encode_kernel<true, 512, 4>), dim3(grid), dim3(nt), lds, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_hip_error(e, "leo_encode_kernel");
