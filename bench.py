@@ -717,8 +717,7 @@ def main():
     assert torch.equal(st.buf[:, k:], ref_parity), "batched encode differs from plain encode"
     t_enc_batch = timed(torch, b_enc.graph_launch, args.steps, args.warmup)
     # roofline: the kernel's own duration, HIP events around 10 direct launches of the
-    # same tuned batch (an event pair around a graph launch also holds the graph's
-    # start-up on the device, which a kernel trace leaves out)
+    # same tuned batch, on the stream it is launched on
     d_batch = event_durations(torch, lambda i: b_enc.run(), 10)
     bytes_per_launch = N * (k + m) * S
     achieved = bytes_per_launch / (float(d_batch.mean()) * 1e-3) / 1e9
@@ -883,7 +882,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"RS 10+4 GF(2^8) encode of {N} stripes x {S // 1024} KiB shards, "
                                "device-resident, batched: one batch-kernel launch per step "
-                               "(hipGraph replay); plain per-stripe launches reported alongside",
+                               "(rsgpu_batch_graph_launch, which launches a one-kernel batch "
+                               "directly); plain per-stripe launches reported alongside",
                    "stripes": N, "shard_bytes": S, "data_shards": k, "parity_shards": m,
                    "shard_pitch_bytes": pitch},
         "encode_batched_gbs": gbs(data_bytes, t_enc_batch),
