@@ -25,6 +25,7 @@ Prints ONE JSON line.
 """
 import argparse
 import ctypes as C
+import gc
 import json
 import os
 import sys
@@ -854,9 +855,15 @@ def main():
                   "tune": rg_tune}
         del b_rg, rg
 
+    # the per-call sections below are host-bound at a few microseconds a call: start
+    # each without the ragged section's Python objects pending collection
+    gc.collect()
     leopard = leopard_section(torch, rsgpu, DSP, disp) if args.leopard else None
+    gc.collect()
     gf16_batch = gf16_batch_section(torch, rsgpu, 10)
+    gc.collect()
     config1 = config1_section(torch, rsgpu, args.steps)
+    gc.collect()
     gf2_mfma = gf2_mfma_section(torch, rsgpu)
     dispatch_live = None
     if args.dispatch_live:
