@@ -309,6 +309,46 @@ def test_leopard_gf16_reconstruct_in_the_split_batch_kernel_matches_the_oracle()
             assert (to_host(slots[i].data) == w).all(), i
 
 
+def test_graph_replays_every_element_type():
+    """One batch holding GF(2^8), tower GF(2^16) and split-layout Leopard GF(2^16)
+    stripes is three kernel launches, so graph_launch captures and replays them.
+    The outputs are overwritten between replays (eager work on the same stream)
+    and every replay must rebuild the oracle's bytes."""
+    announce("batch graph replay, three element types")
+    rng = np.random.default_rng(77)
+    b = rsgpu.Batch()
+    outs = []
+    for field, k, m, n in [("gf8", 10, 4, 8192), ("gf16", 6, 3, 4096 + 2), ("gf16", 12, 4, 16384)]:
+        F = O.GF16 if field == "gf16" else O.GF8
+        rs = rsgpu.ReedSolomon(k, m, field=field)
+        mat = O.build_matrix(F, O.VANDERMONDE, k, m)[1]
+        data = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k)]
+        par = O.encode(F, mat, k, m, data)
+        sh = [torch.from_numpy(d).cuda() for d in data] + [torch.zeros(n, dtype=torch.uint8, device="cuda")
+                                                           for _ in range(m)]
+        b.add_encode(rs, sh)
+        outs += [(sh[k + j], par[j]) for j in range(m)]
+    k, m, n = 64, 16, 4096
+    rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf16")
+    rs.set_variant(0)
+    junk = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
+    miss = [3, 40, 70]
+    rc, exp = O.leo_reconstruct(16, k, m, [x.copy() for x in junk], [0 if i in miss else 1 for i in range(k + m)])
+    assert rc == 0
+    slots = [rsgpu.ShardSlot(torch.zeros(n, dtype=torch.uint8, device="cuda"), False) if i in miss
+             else rsgpu.ShardSlot(torch.from_numpy(junk[i].copy()).cuda()) for i in range(k + m)]
+    b.add_reconstruct(rs, slots)
+    outs += [(slots[i].data, exp[i]) for i in miss]
+    for rep in range(3):
+        for t, _ in outs:
+            t.fill_(0xA5 ^ rep)
+        torch.cuda.synchronize()
+        b.graph_launch()
+        torch.cuda.synchronize()
+        for t, w in outs:
+            assert (to_host(t) == w).all(), rep
+
+
 def test_ragged_workload_plain_equals_batched():
     """The ragged mix (mixed shapes, 4 KiB-256 KiB shards, encode/verify/reconstruct with
     0..m erasures): the batch kernel writes the same bytes as per-stripe launches."""
