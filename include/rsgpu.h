@@ -410,7 +410,8 @@ int rsgpu_apply_rows(int32_t field, const uint8_t* rows, size_t r, size_t k,
  * GF(2^16) tower pairs), and so do the matrix forms of Leopard ops: GF(2^8)
  * encode, verify and reconstruct when the codec's dispatch picks a matrix
  * variant (with no dispatch state: k <= 32), GF(2^16) reconstruct when it picks
- * one (split-layout kernel, 16-byte aligned shards).  Other Leopard ops run
+ * one (with no dispatch state: at most 4 erasures; split-layout kernel, 16-byte
+ * aligned shards).  Other Leopard ops run
  * their FFT kernels after the batch kernels, on the same stream.  Reconstruct
  * ops enqueue any decode-row solve on `stream` when added and mark their targets
  * present.  Verify ops OR a nonzero value into `device_flag` on mismatch (the

@@ -1717,16 +1717,21 @@ int batch_reconstruct_stripe(const Codec& c, rsgpu_slot* shards, size_t n, int g
 // matrix; reconstruct of 1 or 4 erasures 19 us FFT, 4.8 and 6.4 matrix), while the
 // matrix forms share the batch's one launch.  A Leopard GF(2^16) reconstruct goes into the
 // split-layout kernel as the decoder's own matrix when the dispatch picks a
-// matrix variant for it (at most 32 erasures).  Otherwise RSGPU_ERR_UNSUPPORTED
-// tells the batch to run the FFT through its own launch.
+// matrix variant for it (at most 32 erasures), or, with no dispatch state, for
+// at most 4 erasures: the FFT costs the same for one erasure as for m, and per
+// call the decoder matrix measured faster at every shape up to 4 (1024+256 x
+// 256 KiB: 1 / 4 erasures 92 / 161 us against 930-1000 FFT; 4096+1024 x 64 KiB:
+// 133 / 201 against 1750; 300+45 x 64 KiB: 31 / 40 against 62-67).  Otherwise
+// RSGPU_ERR_UNSUPPORTED tells the batch to run the FFT through its own launch.
 static bool leo_batch_matrix(const Codec& c, int op, size_t erasures, size_t len) {
+  const bool no_state = c.forced_variant.load(std::memory_order_relaxed) < 0 &&
+                        !std::atomic_load(&c.variant_table);
   if (c.family == RSGPU_FAMILY_LEOPARD_GF16) {
+    if (no_state) return op == kOpReconstruct && erasures <= 4;
     const int v = pick_variant(c, op, erasures, len);
     return v >= 0 && v < kNumApplyVariants;
   }
   if (c.family != RSGPU_FAMILY_LEOPARD_GF8) return false;
-  const bool no_state = c.forced_variant.load(std::memory_order_relaxed) < 0 &&
-                        !std::atomic_load(&c.variant_table);
   if (no_state) return c.k <= 32;
   return is_matrix_variant(c, pick_variant(c, op, erasures, len));
 }

@@ -309,6 +309,31 @@ def test_leopard_gf16_reconstruct_in_the_split_batch_kernel_matches_the_oracle()
             assert (to_host(slots[i].data) == w).all(), i
 
 
+def test_leopard_gf16_few_erasures_enter_the_batch_without_dispatch_state():
+    """With no dispatch state, a Leopard GF(2^16) reconstruct of at most 4 erasures
+    runs as the decoder's own matrix in the split-layout kernel (the batch gains
+    tiles); 5 or more run the FFT after the kernels.  Bytes equal the oracle's."""
+    announce("batch leopard gf16 default matrix rule")
+    rng = np.random.default_rng(62)
+    k, m, n = 100, 20, 1024
+    rs = rsgpu.ReedSolomon(k, m, codec_family="leopard_gf16")
+    for e, in_kernel in ((1, True), (4, True), (5, False)):
+        b = rsgpu.Batch()
+        junk = [rng.integers(0, 256, n, dtype=np.uint8) for _ in range(k + m)]
+        miss = sorted(rng.choice(k + m, e, replace=False).tolist())
+        rc, exp = O.leo_reconstruct(16, k, m, [x.copy() for x in junk],
+                                    [0 if i in miss else 1 for i in range(k + m)])
+        assert rc == 0
+        slots = [rsgpu.ShardSlot(torch.zeros(n, dtype=torch.uint8, device="cuda"), False) if i in miss
+                 else rsgpu.ShardSlot(torch.from_numpy(junk[i].copy()).cuda()) for i in range(k + m)]
+        b.add_reconstruct(rs, slots)
+        assert (b.stats()["tiles"] > 0) == in_kernel, e
+        b.run()
+        torch.cuda.synchronize()
+        for i in miss:
+            assert slots[i].present and (to_host(slots[i].data) == exp[i]).all(), (e, i)
+
+
 def test_graph_replays_every_element_type():
     """One batch holding GF(2^8), tower GF(2^16) and split-layout Leopard GF(2^16)
     stripes is three kernel launches, so graph_launch captures and replays them.
