@@ -171,8 +171,12 @@ def test_native_install_matches_python_table():
         n_ops, n_buckets, b0 = rsgpu.dispatch_dims()
         for op in range(n_ops):
             for e in range(m + 1):
+                # an entry of -1 (or one not exact here) takes the library's default
+                # choice: the decoder matrix (variant 12) for a Leopard GF(2^16)
+                # reconstruct of 1..4 erasures, the default tiling otherwise
+                default = 12 if kind == "leopard_gf16" and op == 2 and 1 <= e <= 4 else -1
                 for b in range(n_buckets):
                     ln = int(round(2.0 ** ((b + b0) / 2.0)))
-                    assert rs.variant_for(op, e, ln) == (table[op, e, b] if table[op, e, b] < 0 or
-                                                         rs.variant_valid(op, e, table[op, e, b]) else -1), \
-                        (kind, op, e, b)
+                    v = int(table[op, e, b])
+                    want = v if v >= 0 and rs.variant_valid(op, e, v) else default
+                    assert rs.variant_for(op, e, ln) == want, (kind, op, e, b)
