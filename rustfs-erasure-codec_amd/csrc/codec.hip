@@ -528,16 +528,18 @@ static int size_bucket(size_t len) {
 }
 
 // The default choice when neither a forced variant nor a dispatch-table entry
-// applies.  A Leopard GF(2^16) reconstruct of 1..4 erasures runs the decoder's
-// own matrix with 64-lane tiles and one output per workgroup: of the 31 such
-// calls in the MI355X timing set (10+4 to 4096+1024, 5-260 KiB shards) that
-// variant beat the FFT decoder on every one, by 1.24x at least and 2.69x
-// geometric mean (the FFT costs the same for one erasure as for m).  Everything
-// else takes the default tiling (-1).
-constexpr int kLeo16FewErasureVariant = 12;
+// applies.  A Leopard reconstruct of 1..4 erasures runs its matrix form (the
+// decoder's own matrix, or the [I; G] solve with exactly k present) with 64-lane
+// tiles and one output per workgroup, since the FFT decoder costs the same for
+// one erasure as for m.  In the MI355X timing set that variant beat the FFT on
+// all 31 such GF(2^16) calls (10+4 to 4096+1024, 5-260 KiB shards; 1.24x at
+// least, 2.69x geometric mean) and on 44 of 46 GF(2^8) calls (2.06x geometric
+// mean, 0.87x at worst).  Everything else takes the default tiling (-1): the
+// matrix encode and verify lose to the FFT on most Leopard shapes.
+constexpr int kLeoFewErasureVariant = 12;
 static int default_variant(const Codec& c, int op, size_t erasures) {
-  if (c.family == RSGPU_FAMILY_LEOPARD_GF16 && op == kOpReconstruct && erasures >= 1 && erasures <= 4)
-    return kLeo16FewErasureVariant;
+  if (c.family != RSGPU_FAMILY_CLASSIC && op == kOpReconstruct && erasures >= 1 && erasures <= 4)
+    return kLeoFewErasureVariant;
   return -1;
 }
 

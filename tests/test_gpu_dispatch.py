@@ -172,9 +172,9 @@ def test_native_install_matches_python_table():
         for op in range(n_ops):
             for e in range(m + 1):
                 # an entry of -1 (or one not exact here) takes the library's default
-                # choice: the decoder matrix (variant 12) for a Leopard GF(2^16)
-                # reconstruct of 1..4 erasures, the default tiling otherwise
-                default = 12 if kind == "leopard_gf16" and op == 2 and 1 <= e <= 4 else -1
+                # choice: the matrix form (variant 12) for a Leopard reconstruct of
+                # 1..4 erasures, the default tiling otherwise
+                default = 12 if kind.startswith("leopard") and op == 2 and 1 <= e <= 4 else -1
                 for b in range(n_buckets):
                     ln = int(round(2.0 ** ((b + b0) / 2.0)))
                     v = int(table[op, e, b])
