@@ -463,9 +463,12 @@ int rsgpu_batch_stats(const rsgpu_batch* batch, uint64_t* n_ops, uint64_t* bytes
  *           aligned shards; others take the FFT);
  *   kind 1: Leopard FFT with a column tile of `leopard_tile_words` words.
  * A codec uses its forced variant if set, else its variant table, else the
- * default tiling; an entry that is not valid for a call falls back to the
- * default.  Setting either while calls run on the codec is not synchronised
- * with those calls (each call reads one consistent table). ---- */
+ * default choice; an entry of -1, or one that is not valid for a call, also
+ * takes the default choice.  The default choice is the default tiling, except
+ * that a Leopard reconstruct of 1..4 erased shards runs its matrix form with
+ * variant 12 (64-lane tiles, one output per workgroup).  Setting either while
+ * calls run on the codec is not synchronised with those calls (each call reads
+ * one consistent table). ---- */
 int rsgpu_variant_count(void);
 /* Table geometry: ops (0 encode, 1 verify, 2 reconstruct), size buckets, and the
  * bucket offset: bucket = clamp(round(2 * log2(shard_len)) - bucket0). */
@@ -477,7 +480,7 @@ int rsgpu_variant_valid(const rsgpu_codec* codec, int32_t op, size_t erasures, i
 /* Forces one variant for every call (-1 clears). */
 int rsgpu_codec_set_variant(rsgpu_codec* codec, int32_t variant);
 /* Installs table[op][e][bucket] (n_ops x n_erasures x n_buckets int16, -1 =
- * default tiling; erasure counts >= n_erasures use the last row); NULL clears. */
+ * default choice; erasure counts >= n_erasures use the last row); NULL clears. */
 int rsgpu_codec_set_variant_table(rsgpu_codec* codec, const int16_t* table, size_t n_erasures);
 /* The variant a call with these parameters would run (-1 = default tiling). */
 int rsgpu_codec_variant_for(const rsgpu_codec* codec, int32_t op, size_t erasures, size_t shard_len,
